@@ -1,0 +1,395 @@
+// adjoint.hip -- fermion force as a reverse (adjoint) sweep over the slice product.
+//
+// The reference force (organic_hmc.c:86-134) needs, for every slice tau, the bond entries
+//   Y_tau[i,j] + Y_tau[j,i],   p_dot[tau,i,k] = -kappa_k * flavors * (Y[i,nb] + Y[nb,i]),
+// of the exact derivative Y_tau of the slice polynomial p(A_tau) contracted with
+//   G_tau = B_{tau+1} ... B_{nt-1} M_hat^{-1} B_0 ... B_{tau-1}
+// (construct_all_QM1P + refine_QM1P, organic_dense_matrix.c:311-442).
+// With p(A) = sum_n c_n A^n, W_tau = B_{tau+1}..B_{nt-1} M_hat^{-1} and T_tau = B_{tau-1}..B_0
+// (so G = W T^T), the derivative is
+//   Y = sum_{a+b<=deg-1} c_{a+b+1} (A^a W)(A^b T)^T = sum_a W_a S_a^T,  S_a = sum_b c_{a+b+1} T_b,
+// W_a = A^a W, T_b = A^b T.  Bond (i,j): sum_c sum_a (W_a[i,c] S_a[j,c] + W_a[j,c] S_a[i,c]).
+// Columns c are independent, so a workgroup owns C of them and produces partial bond sums;
+// force_reduce_kernel adds the partials of all workgroups in a fixed order (deterministic
+// force, which keeps the molecular dynamics exactly reversible).
+//
+// Sweep: tau = nt-1 ... 0 with W_{nt-1} = M_hat^{-1}, W_{tau-1} = B_tau W_tau = sum_n c_n W_n.
+// T_tau comes from checkpoints T_{s k} stored by the forward chain every k slices and is
+// recomputed inside the segment (stored to a workgroup-private scratch, then read backwards).
+#include "internal.h"
+
+namespace ephmc {
+
+template <typename R>
+struct AdjArgs {
+	const R *coef;          // [nt][6][ns]
+	const int *nb;          // [6][ns]
+	R mu;
+	unsigned ns;
+	int deg;
+	R cm[MAX_DEG + 1];      // monomial coefficients of p
+	int np;                 // pair form, for the recompute of T (same arithmetic as the forward chain)
+	R a_lin[MAX_PAIRS], a_quad[MAX_PAIRS];
+	int cplx[MAX_PAIRS];
+	const R *ck;            // checkpoint T_{tau_b} of this segment (column-major N x N)
+	R *seg;                 // scratch: blockIdx.x * (k * C * ns) + (tau - tau_b) * C * ns + c * ns + site
+	unsigned seg_stride;    // k * C * ns
+	const double *w_init;   // M_hat^{-1} (first segment only), else null
+	R *wbuf;                // W state between segments (column-major N x N)
+	double *partial;        // [tau - tau_b][nwg][3 ns]
+	unsigned nwg;
+	int tau_b, tau_e;
+};
+
+template <typename R, int C>
+__device__ __forceinline__ void gather6(const R *__restrict__ buf, const int (&nbr)[6], const R (&cf)[6], R mu,
+		const R (&own)[C], R (&out)[C])
+{
+#pragma unroll
+	for(int c = 0; c < C; c++) out[c] = mu * own[c];
+#pragma unroll
+	for(int d = 0; d < 6; d++){
+		const R *q = buf + (size_t)nbr[d] * C;
+#pragma unroll
+		for(int c = 0; c < C; c++) out[c] += cf[d] * q[c];
+	}
+}
+
+template <typename R, int C, int BT, int S, int MAXD>
+__global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
+{
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	const unsigned ns = a.ns;
+	R *buf0 = reinterpret_cast<R *>(smem_raw);
+	R *buf1 = buf0 + (size_t)C * ns;
+	R *wb0 = buf1 + (size_t)C * ns;
+	R *wb1 = wb0 + (size_t)C * ns;
+	R *hx = wb1 + (size_t)C * ns;          // [site][3]
+	R *bufs[2] = {buf0, buf1};
+	R *wbs[2] = {wb0, wb1};
+
+	const unsigned col0 = blockIdx.x * C;
+	R *seg = a.seg + (size_t)blockIdx.x * a.seg_stride;
+	unsigned site[S];
+	bool act[S];
+	int nbr[S][6];
+	R cf[S][6];
+	R v[S][C];
+#pragma unroll
+	for(int j = 0; j < S; j++){
+		site[j] = threadIdx.x + j * BT;
+		act[j] = site[j] < ns;
+#pragma unroll
+		for(int d = 0; d < 6; d++) nbr[j][d] = act[j] ? a.nb[d * ns + site[j]] : 0;
+	}
+	auto colok = [&](int c){ return col0 + c < ns; };
+
+	// ---- recompute T_tau, tau in [tau_b, tau_e), from the checkpoint --------------------
+#pragma unroll
+	for(int j = 0; j < S; j++)
+#pragma unroll
+		for(int c = 0; c < C; c++)
+			v[j][c] = (act[j] && colok(c)) ? a.ck[(size_t)(col0 + c) * ns + site[j]] : (R)0;
+	for(int tau = a.tau_b; tau < a.tau_e; tau++){
+		R *dst = seg + (size_t)(tau - a.tau_b) * C * ns;
+#pragma unroll
+		for(int j = 0; j < S; j++)
+			if(act[j])
+#pragma unroll
+				for(int c = 0; c < C; c++) dst[(size_t)c * ns + site[j]] = v[j][c];
+		if(tau == a.tau_e - 1) break;
+		const R *cft = a.coef + (size_t)tau * 6 * ns;
+#pragma unroll
+		for(int j = 0; j < S; j++)
+#pragma unroll
+			for(int d = 0; d < 6; d++) cf[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
+		for(int p = 0; p < a.np; p++){
+			R w[S][C];
+#pragma unroll
+			for(int j = 0; j < S; j++)
+				if(act[j])
+#pragma unroll
+					for(int c = 0; c < C; c++) buf0[(size_t)site[j] * C + c] = v[j][c];
+			__syncthreads();
+#pragma unroll
+			for(int j = 0; j < S; j++)
+				if(act[j]) gather6<R, C>(buf0, nbr[j], cf[j], a.mu, v[j], w[j]);
+			if(a.cplx[p]){
+#pragma unroll
+				for(int j = 0; j < S; j++)
+					if(act[j])
+#pragma unroll
+						for(int c = 0; c < C; c++) buf1[(size_t)site[j] * C + c] = w[j][c];
+				__syncthreads();
+#pragma unroll
+				for(int j = 0; j < S; j++){
+					if(!act[j]) continue;
+					R z[C];
+					gather6<R, C>(buf1, nbr[j], cf[j], a.mu, w[j], z);
+#pragma unroll
+					for(int c = 0; c < C; c++) v[j][c] += a.a_lin[p] * w[j][c] + a.a_quad[p] * z[c];
+				}
+			}else{
+				__syncthreads();
+#pragma unroll
+				for(int j = 0; j < S; j++)
+#pragma unroll
+					for(int c = 0; c < C; c++) v[j][c] += a.a_lin[p] * w[j][c];
+			}
+		}
+	}
+
+	// ---- W state --------------------------------------------------------------------------
+	R wv[S][C];
+#pragma unroll
+	for(int j = 0; j < S; j++)
+#pragma unroll
+		for(int c = 0; c < C; c++){
+			R val = 0;
+			if(act[j] && colok(c)){
+				const size_t e = (size_t)(col0 + c) * ns + site[j];
+				val = a.w_init ? (R)a.w_init[e] : a.wbuf[e];
+			}
+			wv[j][c] = val;
+		}
+
+	const int deg = a.deg;
+	for(int tau = a.tau_e - 1; tau >= a.tau_b; tau--){
+		const R *cft = a.coef + (size_t)tau * 6 * ns;
+#pragma unroll
+		for(int j = 0; j < S; j++)
+#pragma unroll
+			for(int d = 0; d < 6; d++) cf[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
+		const R *tsrc = seg + (size_t)(tau - a.tau_b) * C * ns;
+		R t[S][C];
+#pragma unroll
+		for(int j = 0; j < S; j++)
+#pragma unroll
+			for(int c = 0; c < C; c++) t[j][c] = act[j] ? tsrc[(size_t)c * ns + site[j]] : (R)0;
+
+		// S_a = sum_b c_{a+b+1} T_b, built while the powers T_b = A^b T are formed
+		R sa[MAXD][S][C];
+#pragma unroll
+		for(int q = 0; q < MAXD; q++)
+#pragma unroll
+			for(int j = 0; j < S; j++)
+#pragma unroll
+				for(int c = 0; c < C; c++) sa[q][j][c] = (q < deg) ? a.cm[q + 1] * t[j][c] : (R)0;
+		for(int b = 1; b < deg; b++){
+			R *bb = bufs[(b - 1) & 1];
+#pragma unroll
+			for(int j = 0; j < S; j++)
+				if(act[j])
+#pragma unroll
+					for(int c = 0; c < C; c++) bb[(size_t)site[j] * C + c] = t[j][c];
+			__syncthreads();
+#pragma unroll
+			for(int j = 0; j < S; j++){
+				R tn[C];
+				if(act[j]) gather6<R, C>(bb, nbr[j], cf[j], a.mu, t[j], tn);
+				else
+#pragma unroll
+					for(int c = 0; c < C; c++) tn[c] = 0;
+#pragma unroll
+				for(int c = 0; c < C; c++) t[j][c] = tn[c];
+			}
+#pragma unroll
+			for(int q = 0; q < MAXD; q++)
+				if(q + b < deg)
+#pragma unroll
+					for(int j = 0; j < S; j++)
+#pragma unroll
+						for(int c = 0; c < C; c++) sa[q][j][c] += a.cm[q + b + 1] * t[j][c];
+		}
+
+		// W powers, half-bond sums H[i][d] = sum_{c,a} W_a[i,c] S_a[nb(i,d),c]
+		R h[S][6];
+		R acc[S][C];
+#pragma unroll
+		for(int j = 0; j < S; j++){
+#pragma unroll
+			for(int d = 0; d < 6; d++) h[j][d] = 0;
+#pragma unroll
+			for(int c = 0; c < C; c++) acc[j][c] = a.cm[0] * wv[j][c];
+		}
+		R wa[S][C];
+#pragma unroll
+		for(int j = 0; j < S; j++)
+#pragma unroll
+			for(int c = 0; c < C; c++) wa[j][c] = wv[j][c];
+#pragma unroll
+		for(int j = 0; j < S; j++)
+			if(act[j])
+#pragma unroll
+				for(int c = 0; c < C; c++) wbs[0][(size_t)site[j] * C + c] = wa[j][c];
+#pragma unroll
+		for(int q = 0; q < MAXD; q++){
+			if(q >= deg) break;
+			R *sb = bufs[(q + deg - 1) & 1];
+			R *wb = wbs[q & 1];
+#pragma unroll
+			for(int j = 0; j < S; j++)
+				if(act[j])
+#pragma unroll
+					for(int c = 0; c < C; c++) sb[(size_t)site[j] * C + c] = sa[q][j][c];
+			__syncthreads();
+#pragma unroll
+			for(int j = 0; j < S; j++){
+				if(!act[j]) continue;
+#pragma unroll
+				for(int d = 0; d < 6; d++){
+					const R *s2 = sb + (size_t)nbr[j][d] * C;
+					R hs = 0;
+#pragma unroll
+					for(int c = 0; c < C; c++) hs += wa[j][c] * s2[c];
+					h[j][d] += hs;
+				}
+				R wn[C];
+				gather6<R, C>(wb, nbr[j], cf[j], a.mu, wa[j], wn);
+#pragma unroll
+				for(int c = 0; c < C; c++){
+					wa[j][c] = wn[c];
+					acc[j][c] += a.cm[q + 1] * wn[c];
+				}
+			}
+			if(q + 1 < deg){
+				R *wnext = wbs[(q + 1) & 1];
+#pragma unroll
+				for(int j = 0; j < S; j++)
+					if(act[j])
+#pragma unroll
+						for(int c = 0; c < C; c++) wnext[(size_t)site[j] * C + c] = wa[j][c];
+			}
+		}
+		// bond(i, 2k) = H[i][2k] + H[nb(i,2k)][2k+1]
+#pragma unroll
+		for(int j = 0; j < S; j++)
+			if(act[j])
+#pragma unroll
+				for(int k = 0; k < 3; k++) hx[(size_t)site[j] * 3 + k] = h[j][2 * k + 1];
+		__syncthreads();
+		double *out = a.partial + ((size_t)(tau - a.tau_b) * a.nwg + blockIdx.x) * 3 * ns;
+#pragma unroll
+		for(int j = 0; j < S; j++)
+			if(act[j])
+#pragma unroll
+				for(int k = 0; k < 3; k++)
+					out[(size_t)site[j] * 3 + k] = (double)h[j][2 * k] + (double)hx[(size_t)nbr[j][2 * k] * 3 + k];
+#pragma unroll
+		for(int j = 0; j < S; j++)
+#pragma unroll
+			for(int c = 0; c < C; c++) wv[j][c] = acc[j][c];
+	}
+#pragma unroll
+	for(int j = 0; j < S; j++)
+#pragma unroll
+		for(int c = 0; c < C; c++)
+			if(act[j] && colok(c)) a.wbuf[(size_t)(col0 + c) * ns + site[j]] = wv[j][c];
+}
+
+// p_dot[tau][link] = -kappa_k flavors sum_wg partial[tau - tau_b][wg][link]; optional kick p += h F.
+__global__ void force_reduce_kernel(const double *__restrict__ partial, unsigned nwg, unsigned loc, int tau_b, int ntau,
+		double f0, double f1, double f2, double *__restrict__ pdot, double *__restrict__ p, double kick)
+{
+	const size_t n = (size_t)ntau * loc;
+	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x){
+		const size_t tl = e / loc;
+		const unsigned link = (unsigned)(e - tl * loc);
+		const double *src = partial + tl * nwg * loc + link;
+		double s = 0;
+		for(unsigned w = 0; w < nwg; w++) s += src[(size_t)w * loc];
+		const unsigned k = link % 3;
+		const double F = (k == 0 ? f0 : (k == 1 ? f1 : f2)) * s;
+		const size_t pos = (size_t)(tau_b + tl) * loc + link;
+		if(pdot) pdot[pos] = F;
+		if(p) p[pos] += kick * F;
+	}
+}
+
+unsigned adjoint_cols(const ephmc_ctx *c, int prec)
+{
+	return c->cols ? c->cols : (prec == EPHMC_FP64 ? 2u : 4u);
+}
+
+template <typename R, int C>
+static int launch_adjoint_c(ephmc_ctx *c, AdjArgs<R> a)
+{
+	const unsigned ns = a.ns;
+	const size_t lds = (4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R);
+	if(lds > 160 * 1024){ set_error("adjoint: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
+	const unsigned blocks = a.nwg;
+#define EPHMC_ADJ_LAUNCH2(BT, S, MAXD)                                                            \
+	do {                                                                                          \
+		(void)hipFuncSetAttribute((const void *)adjoint_kernel<R, C, BT, S, MAXD>,                      \
+			hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                \
+		adjoint_kernel<R, C, BT, S, MAXD><<<blocks, BT, lds, c->stream>>>(a);                    \
+	} while(0)
+#define EPHMC_ADJ_LAUNCH(BT, S)                                                                   \
+	do { if(a.deg <= 4) EPHMC_ADJ_LAUNCH2(BT, S, 4); else EPHMC_ADJ_LAUNCH2(BT, S, 8); } while(0)
+	if(ns <= 256) EPHMC_ADJ_LAUNCH(256, 1);
+	else if(ns <= 512) EPHMC_ADJ_LAUNCH(512, 1);
+	else if(ns <= 1024) EPHMC_ADJ_LAUNCH(1024, 1);
+	else if(ns <= 2048) EPHMC_ADJ_LAUNCH(1024, 2);
+	else { set_error("adjoint: lattice of %u sites exceeds 2048", ns); return EPHMC_ERR_UNSUPPORTED; }
+#undef EPHMC_ADJ_LAUNCH
+#undef EPHMC_ADJ_LAUNCH2
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+template <typename R>
+static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p)
+{
+	const unsigned ns = c->ns, nt = c->nt, k = c->ckpt, C = adjoint_cols(c, prec);
+	const size_t mat = c->mat;
+	AdjArgs<R> a{};
+	a.coef = prec == EPHMC_FP64 ? (const R *)c->d_coef64 : (const R *)c->d_coef32;
+	a.nb = c->d_nb;
+	a.mu = (R)c->cfg.mu;
+	a.ns = ns;
+	a.deg = c->poly.deg;
+	for(int q = 0; q <= c->poly.deg; q++) a.cm[q] = (R)c->poly.c[q];
+	a.np = c->poly.np;
+	for(int q = 0; q < c->poly.np; q++){
+		a.a_lin[q] = (R)c->poly.a_lin[q];
+		a.a_quad[q] = (R)c->poly.a_quad[q];
+		a.cplx[q] = c->poly.cplx[q];
+	}
+	a.seg_stride = k * C * ns;
+	a.nwg = (ns + C - 1) / C;
+	// W state between segments; T_nt is no longer needed once M_hat is formed
+	a.wbuf = prec == EPHMC_FP64 ? (R *)(void *)c->d_Tfin : (R *)(void *)c->d_Tfin32;
+	a.partial = c->d_partial;
+	const R *ck = (const R *)c->d_ck;
+	const double f[3] = {-c->cfg.kappa[0] * c->cfg.flavors, -c->cfg.kappa[1] * c->cfg.flavors, -c->cfg.kappa[2] * c->cfg.flavors};
+	for(int s = (int)c->nseg - 1; s >= 0; s--){
+		a.tau_b = s * (int)k;
+		a.tau_e = std::min<int>((s + 1) * (int)k, (int)nt);
+		a.ck = ck + (size_t)s * mat;
+		a.seg = (R *)c->d_seg;
+		a.w_init = (s == (int)c->nseg - 1) ? minv : nullptr;
+		int rc;
+		switch(C){
+			case 1: rc = launch_adjoint_c<R, 1>(c, a); break;
+			case 2: rc = launch_adjoint_c<R, 2>(c, a); break;
+			default: rc = launch_adjoint_c<R, 4>(c, a); break;
+		}
+		if(rc != EPHMC_OK) return rc;
+		const int ntau = a.tau_e - a.tau_b;
+		const size_t n = (size_t)ntau * c->loc;
+		force_reduce_kernel<<<(unsigned)std::min<size_t>((n + 255) / 256, 2048), 256, 0, c->stream>>>(
+			c->d_partial, a.nwg, c->loc, a.tau_b, ntau, f[0], f[1], f[2], pdot, p, kick);
+		EPHMC_LAUNCHED();
+	}
+	return EPHMC_OK;
+}
+
+// Force for the field whose coefficients and checkpoints are current in the context
+// (chain_forward with checkpoints must have run for this field).
+int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p)
+{
+	if(prec == EPHMC_FP64) return run_adjoint<double>(c, prec, minv, pdot, kick, p);
+	return run_adjoint<float>(c, prec, minv, pdot, kick, p);
+}
+
+} // namespace ephmc

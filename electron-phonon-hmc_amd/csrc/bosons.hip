@@ -1,0 +1,367 @@
+// bosons.hip -- Fourier-accelerated boson sector (organic_bosons.c) as batched FFT kernels,
+// plus the deterministic reductions of the Hamiltonian and the Metropolis step.
+//
+// Every link l carries a real series over tau (stride 3N = loc, organic_hmc.c:317-322).  A
+// workgroup loads LW complex series into LDS (z[t][l], links contiguous, coalesced rows),
+// transforms them (radix-2 decimation in frequency: natural order in, bit-reversed modes out),
+// applies the per-mode operation, and transforms back (decimation in time: bit-reversed in,
+// natural out), so no permutation pass is needed.  Non-power-of-two nt uses direct sums.
+// Two real series share one complex transform:
+//   evolve:  z = x + i p of one link; with X_n = (Z_n + conj Z_-n)/2, P_n = (Z_n - conj Z_-n)/2i
+//            the exact flow X' = cos h X + sin h P/f_n, P' = cos h P - sin h f_n X per mode
+//            (evolve_bosons, organic_bosons.c:126-150), optional kick p += h_k F fused in the load;
+//   filter:  z = a + i b of two links, Z' = g_n Z (real symmetric g), for the momentum refresh
+//            g = sqrt(f_n) and the free-boson draw g = 1/sqrt(f_n) (organic_bosons.c:55-100);
+//   energy:  sum_n |Z_n|^2 / (2 nt f_n) over the full spectrum = the two links' kinetic energies
+//            (energy_fourier_momenta, organic_bosons.c:102-124).
+// f_n = omega + (2 sin(pi n/nt))^2/omega.
+#include "internal.h"
+
+namespace ephmc {
+
+constexpr int FFT_LW = 16;     // complex series per workgroup
+constexpr int FFT_BT = 256;
+
+enum FftMode { FFT_EVOLVE = 0, FFT_FILTER = 1, FFT_ENERGY = 2 };
+
+struct FftArgs {
+	const double *in0;     // evolve: x; filter/energy: input series
+	const double *in1;     // evolve: p
+	const double *kickF;   // evolve: force for the fused kick (or null)
+	double kick;
+	double *out0, *out1;   // evolve: x, p; filter: output series
+	const double *twid;    // [nt] e^{-2 pi i j/nt} as (re, im)
+	const double *f2;      // [nt] f_n
+	double c, s;           // evolve: cos h / nt, sin h / nt
+	int gain_inv;          // filter: 0 -> sqrt(f)/nt, 1 -> 1/(sqrt(f) nt)
+	double *partial;       // energy: one value per workgroup
+	unsigned loc, nt, logn;
+	int pow2;
+	int mode;
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b){ return make_double2(a.x*b.x - a.y*b.y, a.x*b.y + a.y*b.x); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b){ return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b){ return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 conjg(double2 a){ return make_double2(a.x, -a.y); }
+
+// position of mode n in the transformed buffer
+__device__ __forceinline__ unsigned mode_pos(unsigned n, const FftArgs &a)
+{
+	if(!a.pow2) return n;
+	return a.logn ? (__brev(n) >> (32 - a.logn)) : 0u;
+}
+
+__global__ __launch_bounds__(FFT_BT) void fa_fft_kernel(FftArgs a)
+{
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	const unsigned nt = a.nt;
+	double2 *z = reinterpret_cast<double2 *>(smem_raw);            // [nt][LW]
+	double2 *z2 = z + (size_t)nt * FFT_LW;                         // second buffer (direct sums)
+	double2 *tw = z2 + (a.pow2 ? 0 : (size_t)nt * FFT_LW);         // [nt]
+	__shared__ double red[FFT_BT];
+	const int tid = threadIdx.x;
+	const unsigned pairs = (a.mode == FFT_EVOLVE) ? 1 : 2;         // real series per complex slot
+	const unsigned l0 = blockIdx.x * FFT_LW * pairs;
+
+	for(unsigned j = tid; j < nt; j += FFT_BT) tw[j] = make_double2(a.twid[2 * j], a.twid[2 * j + 1]);
+	// ---- load
+	for(unsigned e = tid; e < nt * FFT_LW; e += FFT_BT){
+		const unsigned t = e / FFT_LW, l = e % FFT_LW;
+		double re = 0, im = 0;
+		if(a.mode == FFT_EVOLVE){
+			const unsigned link = l0 + l;
+			if(link < a.loc){
+				const size_t q = (size_t)t * a.loc + link;
+				re = a.in0[q];
+				im = a.in1[q];
+				if(a.kickF) im += a.kick * a.kickF[q];
+			}
+		}else{
+			const unsigned la = l0 + l, lb = l0 + FFT_LW + l;
+			if(la < a.loc) re = a.in0[(size_t)t * a.loc + la];
+			if(lb < a.loc) im = a.in0[(size_t)t * a.loc + lb];
+		}
+		z[(size_t)t * FFT_LW + l] = make_double2(re, im);
+	}
+	__syncthreads();
+
+	// ---- forward transform
+	double2 *Z = z;
+	if(a.pow2){
+		for(unsigned half = nt >> 1; half >= 1; half >>= 1){
+			const unsigned stride = nt / (2 * half);
+			for(unsigned q = tid; q < (nt / 2) * FFT_LW; q += FFT_BT){
+				const unsigned l = q % FFT_LW, b = q / FFT_LW;
+				const unsigned grp = b / half, pos = b - grp * half;
+				const unsigned i = grp * 2 * half + pos, j = i + half;
+				const double2 u = z[(size_t)i * FFT_LW + l], v = z[(size_t)j * FFT_LW + l];
+				z[(size_t)i * FFT_LW + l] = cadd(u, v);
+				z[(size_t)j * FFT_LW + l] = cmul(csub(u, v), tw[pos * stride]);
+			}
+			__syncthreads();
+		}
+	}else{
+		for(unsigned q = tid; q < nt * FFT_LW; q += FFT_BT){
+			const unsigned l = q % FFT_LW, k = q / FFT_LW;
+			double2 acc = make_double2(0, 0);
+			for(unsigned t = 0; t < nt; t++) acc = cadd(acc, cmul(z[(size_t)t * FFT_LW + l], tw[((size_t)k * t) % nt]));
+			z2[(size_t)k * FFT_LW + l] = acc;
+		}
+		__syncthreads();
+		Z = z2;
+	}
+
+	// ---- per-mode operation
+	if(a.mode == FFT_ENERGY){
+		double e = 0;
+		for(unsigned q = tid; q < nt * FFT_LW; q += FFT_BT){
+			const unsigned l = q % FFT_LW, n = q / FFT_LW;
+			const double2 v = Z[(size_t)mode_pos(n, a) * FFT_LW + l];
+			e += (v.x * v.x + v.y * v.y) / (2.0 * nt * a.f2[n]);
+		}
+		red[tid] = e;
+		__syncthreads();
+		for(int off = FFT_BT / 2; off > 0; off >>= 1){
+			if(tid < off) red[tid] += red[tid + off];
+			__syncthreads();
+		}
+		if(tid == 0) a.partial[blockIdx.x] = red[0];
+		return;
+	}
+	if(a.mode == FFT_FILTER){
+		for(unsigned q = tid; q < nt * FFT_LW; q += FFT_BT){
+			const unsigned l = q % FFT_LW, n = q / FFT_LW;
+			const double f = a.f2[n];
+			const double g = a.gain_inv ? 1.0 / (sqrt(f) * nt) : sqrt(f) / nt;
+			double2 &v = Z[(size_t)mode_pos(n, a) * FFT_LW + l];
+			v = make_double2(g * v.x, g * v.y);
+		}
+	}else{
+		for(unsigned q = tid; q < (nt / 2 + 1) * FFT_LW; q += FFT_BT){
+			const unsigned l = q % FFT_LW, n = q / FFT_LW;
+			const unsigned m = (nt - n) % nt;
+			const unsigned pn = mode_pos(n, a), pm = mode_pos(m, a);
+			const double2 zn = Z[(size_t)pn * FFT_LW + l], zm = conjg(Z[(size_t)pm * FFT_LW + l]);
+			const double2 X = make_double2(0.5 * (zn.x + zm.x), 0.5 * (zn.y + zm.y));
+			const double2 P = make_double2(0.5 * (zn.y - zm.y), -0.5 * (zn.x - zm.x));   // (zn - zm)/(2i)
+			const double f = a.f2[n], sw = a.s * f, s1w = a.s / f;
+			const double2 Xn = make_double2(a.c * X.x + s1w * P.x, a.c * X.y + s1w * P.y);
+			const double2 Pn = make_double2(a.c * P.x - sw * X.x, a.c * P.y - sw * X.y);
+			// Z'_n = X' + i P',  Z'_-n = conj X' + i conj P'
+			Z[(size_t)pn * FFT_LW + l] = make_double2(Xn.x - Pn.y, Xn.y + Pn.x);
+			if(m != n) Z[(size_t)pm * FFT_LW + l] = make_double2(Xn.x + Pn.y, -Xn.y + Pn.x);
+		}
+	}
+	__syncthreads();
+
+	// ---- inverse transform (unnormalised; the 1/nt is in the per-mode factors)
+	if(a.pow2){
+		for(unsigned half = 1; half <= nt / 2; half <<= 1){
+			const unsigned stride = nt / (2 * half);
+			for(unsigned q = tid; q < (nt / 2) * FFT_LW; q += FFT_BT){
+				const unsigned l = q % FFT_LW, b = q / FFT_LW;
+				const unsigned grp = b / half, pos = b - grp * half;
+				const unsigned i = grp * 2 * half + pos, j = i + half;
+				const double2 u = z[(size_t)i * FFT_LW + l];
+				const double2 v = cmul(z[(size_t)j * FFT_LW + l], conjg(tw[pos * stride]));
+				z[(size_t)i * FFT_LW + l] = cadd(u, v);
+				z[(size_t)j * FFT_LW + l] = csub(u, v);
+			}
+			__syncthreads();
+		}
+	}else{
+		for(unsigned q = tid; q < nt * FFT_LW; q += FFT_BT){
+			const unsigned l = q % FFT_LW, t = q / FFT_LW;
+			double2 acc = make_double2(0, 0);
+			for(unsigned k = 0; k < nt; k++) acc = cadd(acc, cmul(z2[(size_t)k * FFT_LW + l], conjg(tw[((size_t)k * t) % nt])));
+			z[(size_t)t * FFT_LW + l] = acc;
+		}
+		__syncthreads();
+	}
+
+	// ---- store
+	for(unsigned e = tid; e < nt * FFT_LW; e += FFT_BT){
+		const unsigned t = e / FFT_LW, l = e % FFT_LW;
+		const double2 v = z[(size_t)t * FFT_LW + l];
+		if(a.mode == FFT_EVOLVE){
+			const unsigned link = l0 + l;
+			if(link < a.loc){
+				const size_t q = (size_t)t * a.loc + link;
+				a.out0[q] = v.x;
+				a.out1[q] = v.y;
+			}
+		}else{
+			const unsigned la = l0 + l, lb = l0 + FFT_LW + l;
+			if(la < a.loc) a.out0[(size_t)t * a.loc + la] = v.x;
+			if(lb < a.loc) a.out0[(size_t)t * a.loc + lb] = v.y;
+		}
+	}
+}
+
+static FftArgs fft_args(ephmc_ctx *c, int mode)
+{
+	FftArgs a{};
+	a.twid = c->d_twiddle;
+	a.f2 = c->d_f2;
+	a.loc = c->loc;
+	a.nt = c->nt;
+	a.pow2 = (c->nt & (c->nt - 1)) == 0;
+	unsigned lg = 0;
+	while((1u << lg) < c->nt) lg++;
+	a.logn = lg;
+	a.mode = mode;
+	return a;
+}
+
+static int launch_fft(ephmc_ctx *c, const FftArgs &a)
+{
+	const unsigned per = (a.mode == FFT_EVOLVE) ? FFT_LW : 2 * FFT_LW;
+	const unsigned blocks = (c->loc + per - 1) / per;
+	const size_t lds = (size_t)c->nt * FFT_LW * sizeof(double2) * (a.pow2 ? 1 : 2) + (size_t)c->nt * sizeof(double2);
+	if(lds > 160 * 1024){ set_error("fft: nt = %u needs %zu B of LDS", c->nt, lds); return EPHMC_ERR_UNSUPPORTED; }
+	(void)hipFuncSetAttribute((const void *)fa_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	fa_fft_kernel<<<blocks, FFT_BT, lds, c->stream>>>(a);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick)
+{
+	FftArgs a = fft_args(c, FFT_EVOLVE);
+	a.in0 = x; a.in1 = p; a.out0 = x; a.out1 = p;
+	a.kickF = pdot; a.kick = kick;
+	a.c = cos(h) / c->nt;
+	a.s = sin(h) / c->nt;
+	return launch_fft(c, a);
+}
+
+int fa_filter(ephmc_ctx *c, const double *in, double *out, int inverse_sqrt)
+{
+	FftArgs a = fft_args(c, FFT_FILTER);
+	a.in0 = in; a.out0 = out;
+	a.gain_inv = inverse_sqrt;
+	return launch_fft(c, a);
+}
+
+__global__ void sum_partials_kernel(const double *__restrict__ part, unsigned n, double *__restrict__ out)
+{
+	__shared__ double red[256];
+	double s = 0;
+	for(unsigned i = threadIdx.x; i < n; i += 256) s += part[i];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1){
+		if((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) *out = red[0];
+}
+
+int fa_energy(ephmc_ctx *c, const double *p, double *scal_out)
+{
+	FftArgs a = fft_args(c, FFT_ENERGY);
+	a.in0 = p;
+	a.partial = c->d_red;
+	EPHMC_TRY(launch_fft(c, a));
+	const unsigned blocks = (c->loc + 2 * FFT_LW - 1) / (2 * FFT_LW);
+	sum_partials_kernel<<<1, 256, 0, c->stream>>>(c->d_red, blocks, scal_out);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+// S_B(x) = (sum (x_{t+1}-x_t)^2 / omega + omega sum x^2) / 2   (hamilton, organic_hmc.c:67-84)
+__global__ __launch_bounds__(256) void boson_action_kernel(const double *__restrict__ x, size_t dim, unsigned loc,
+		double *__restrict__ part)
+{
+	__shared__ double rp[256], rk[256];
+	double pot = 0, kin = 0;
+	for(size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < dim; e += (size_t)gridDim.x * 256){
+		const double v = x[e];
+		const size_t nxt = (e + loc < dim) ? e + loc : e + loc - dim;
+		const double d = x[nxt] - v;
+		pot += v * v;
+		kin += d * d;
+	}
+	rp[threadIdx.x] = pot;
+	rk[threadIdx.x] = kin;
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1){
+		if((int)threadIdx.x < off){ rp[threadIdx.x] += rp[threadIdx.x + off]; rk[threadIdx.x] += rk[threadIdx.x + off]; }
+		__syncthreads();
+	}
+	if(threadIdx.x == 0){ part[2 * blockIdx.x] = rp[0]; part[2 * blockIdx.x + 1] = rk[0]; }
+}
+
+__global__ void boson_action_final(const double *__restrict__ part, unsigned n, double omega, double *__restrict__ out)
+{
+	__shared__ double rp[256], rk[256];
+	double pot = 0, kin = 0;
+	for(unsigned i = threadIdx.x; i < n; i += 256){ pot += part[2 * i]; kin += part[2 * i + 1]; }
+	rp[threadIdx.x] = pot;
+	rk[threadIdx.x] = kin;
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1){
+		if((int)threadIdx.x < off){ rp[threadIdx.x] += rp[threadIdx.x + off]; rk[threadIdx.x] += rk[threadIdx.x + off]; }
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) *out = (rk[0] / omega + rp[0] * omega) / 2;
+}
+
+int boson_action(ephmc_ctx *c, const double *x, double *scal_out)
+{
+	const unsigned blocks = c->red_blocks;
+	boson_action_kernel<<<blocks, 256, 0, c->stream>>>(x, c->dim, c->loc, c->d_red);
+	EPHMC_LAUNCHED();
+	boson_action_final<<<1, 256, 0, c->stream>>>(c->d_red, blocks, c->cfg.omega, scal_out);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+// H = -flavors logdet + E_p + S_B into scal[out_slot]
+__global__ void hamilton_kernel(double *scal, int logdet_slot, int ep_slot, int sb_slot, int out_slot, double flavors)
+{
+	scal[out_slot] = -scal[logdet_slot] * flavors + scal[ep_slot] + scal[sb_slot];
+}
+
+int hamilton_dev(ephmc_ctx *c, const double *x, const double *p, int logdet_slot, int out_slot)
+{
+	EPHMC_TRY(fa_energy(c, p, c->d_scal + SCAL_EP));
+	EPHMC_TRY(boson_action(c, x, c->d_scal + SCAL_BOSON));
+	hamilton_kernel<<<1, 1, 0, c->stream>>>(c->d_scal, logdet_slot, SCAL_EP, SCAL_BOSON, out_slot, (double)c->cfg.flavors);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+// Metropolis (organic_hmc.c:233-243): accept iff exp(H_old - H_new) > U, NaN rejects.
+__global__ void metropolis_kernel(double *scal, double uniform)
+{
+	const double hold = scal[SCAL_H_OLD], hnew = scal[SCAL_H_NEW];
+	const double boltz = exp(hold - hnew);
+	const int acc = (boltz > uniform) ? 1 : 0;
+	scal[SCAL_ACC] = acc;
+	scal[SCAL_BOLTZ] = boltz;
+	scal[SCAL_DH] = hnew - hold;
+	scal[SCAL_ENERGY] = acc ? hnew : hold;
+	if(acc) scal[SCAL_LOGDET_CHAIN] = scal[SCAL_LOGDET];
+}
+
+// dst <- src when the proposal was accepted
+__global__ void accept_copy_kernel(const double *__restrict__ scal, const double *__restrict__ src, double *__restrict__ dst, size_t n)
+{
+	if(scal[SCAL_ACC] == 0) return;
+	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) dst[e] = src[e];
+}
+
+int metropolis(ephmc_ctx *c, double uniform)
+{
+	metropolis_kernel<<<1, 1, 0, c->stream>>>(c->d_scal, uniform);
+	EPHMC_LAUNCHED();
+	accept_copy_kernel<<<1024, 256, 0, c->stream>>>(c->d_scal, c->d_xnew, c->d_x, c->dim);
+	EPHMC_LAUNCHED();
+	accept_copy_kernel<<<1024, 256, 0, c->stream>>>(c->d_scal, c->d_mhat, c->d_minv, c->mat);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+} // namespace ephmc

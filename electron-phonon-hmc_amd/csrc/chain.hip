@@ -1,0 +1,351 @@
+// chain.hip -- slice coefficients and the chained slice product on column blocks.
+//
+// A slice factor B_tau = p(A_tau) = prod_pairs (1 + a_lin A + a_quad A^2) with
+// A_tau = mu + K(x_tau) the symmetric 7-point stencil of the triangular lattice
+// (apply_mu_K_to_v, organic_dense_matrix.c:96-120; pair expansion :168-207).  The reference
+// applies p(A) as stencils to every column of a dense N x N matrix (:209-216); this file does
+// the same on the GPU: one workgroup owns C columns for the whole chain, keeps them in LDS
+// (layout [site][column] so a neighbour read fetches C adjacent values) and applies all
+// slices in sequence.  Each output site gathers its six bond neighbours:
+//   w[i] = mu v[i] + sum_d coef[tau][d][i] v[nb(i,d)],  d = R, L, TR, BL, TL, BR,
+// where coef[tau][d][i] = t_k - kappa_k x[tau][link(i,d)] is the coefficient of the bond
+// between i and nb(i,d) (k = d/2).  This gather is exactly the reference's scatter over
+// forward bonds, including lattices of extent 1 or 2 where bonds alias.
+#include "internal.h"
+#include <cstdlib>
+
+namespace ephmc {
+
+__global__ void coef_kernel(const double *__restrict__ x, const unsigned *__restrict__ link,
+		double t0, double t1, double t2, double k0, double k1, double k2,
+		unsigned ns, unsigned nt, double *__restrict__ c64, float *__restrict__ c32)
+{
+	const size_t n = (size_t)nt * 6 * ns;
+	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x){
+		const size_t tau = e / (6 * ns);
+		const unsigned r = (unsigned)(e - tau * 6 * ns);
+		const unsigned d = r / ns, s = r - d * ns;
+		const unsigned k = d >> 1;
+		const double tk = k == 0 ? t0 : (k == 1 ? t1 : t2);
+		const double kk = k == 0 ? k0 : (k == 1 ? k1 : k2);
+		const double f = tk - kk * x[tau * 3 * ns + link[d * ns + s]];
+		if(c64) c64[e] = f;
+		if(c32) c32[e] = (float)f;
+	}
+}
+
+int coefficients(ephmc_ctx *c, const double *x, bool want32)
+{
+	const size_t n = (size_t)c->nt * 6 * c->ns;
+	const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+	coef_kernel<<<blocks, 256, 0, c->stream>>>(x, c->d_link, c->cfg.t[0], c->cfg.t[1], c->cfg.t[2],
+		c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], c->ns, c->nt, c->d_coef64, want32 ? c->d_coef32 : nullptr);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+struct ChainArgs {
+	const R *coef;          // [nt][6][ns]
+	const int *nb;          // [6][ns]
+	R mu;
+	unsigned ns;
+	const R *src;           // start columns (column-major), or null for the identity
+	R *dst;                 // result columns
+	R *ck;                  // checkpoints: state before every ck_every-th slice, or null
+	unsigned ck_every;
+	int tau0, dir, ntau;    // slices tau0, tau0+dir, ... (ntau of them)
+	int np;
+	R a_lin[MAX_PAIRS], a_quad[MAX_PAIRS];
+	int cplx[MAX_PAIRS];
+};
+
+// out = mu*own + sum_d cf[d] * buf[nb[d]][c]
+template <typename R, int C>
+__device__ __forceinline__ void stencil_gather(const R *__restrict__ buf, const int (&nbr)[6], const R (&cf)[6],
+		R mu, const R (&own)[C], R (&out)[C])
+{
+#pragma unroll
+	for(int c = 0; c < C; c++) out[c] = mu * own[c];
+#pragma unroll
+	for(int d = 0; d < 6; d++){
+		const R *p = buf + (size_t)nbr[d] * C;
+#pragma unroll
+		for(int c = 0; c < C; c++) out[c] += cf[d] * p[c];
+	}
+}
+
+template <typename R, int C, int BT, int S>
+__global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
+{
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	R *V = reinterpret_cast<R *>(smem_raw);
+	R *W = V + (size_t)C * a.ns;
+	const unsigned ns = a.ns;
+	const unsigned col0 = blockIdx.x * C;
+	const size_t mat = (size_t)ns * ns;
+
+	unsigned site[S];
+	bool act[S];
+	int nbr[S][6];
+	R v[S][C], w[S][C], cf[S][6];
+#pragma unroll
+	for(int j = 0; j < S; j++){
+		site[j] = threadIdx.x + j * BT;
+		act[j] = site[j] < ns;
+#pragma unroll
+		for(int d = 0; d < 6; d++) nbr[j][d] = act[j] ? a.nb[d * ns + site[j]] : 0;
+#pragma unroll
+		for(int c = 0; c < C; c++){
+			const unsigned col = col0 + c;
+			R val = 0;
+			if(act[j] && col < ns) val = a.src ? a.src[(size_t)col * ns + site[j]] : (R)(site[j] == col ? 1 : 0);
+			v[j][c] = val;
+		}
+	}
+
+	for(int i = 0; i < a.ntau; i++){
+		const int tau = a.tau0 + a.dir * i;
+		if(a.ck && (i % a.ck_every) == 0){
+			R *ck = a.ck + (size_t)(i / a.ck_every) * mat;
+#pragma unroll
+			for(int j = 0; j < S; j++)
+#pragma unroll
+				for(int c = 0; c < C; c++)
+					if(act[j] && col0 + c < ns) ck[(size_t)(col0 + c) * ns + site[j]] = v[j][c];
+		}
+		const R *cft = a.coef + (size_t)tau * 6 * ns;
+#pragma unroll
+		for(int j = 0; j < S; j++)
+#pragma unroll
+			for(int d = 0; d < 6; d++) cf[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
+
+		for(int p = 0; p < a.np; p++){
+#pragma unroll
+			for(int j = 0; j < S; j++)
+				if(act[j])
+#pragma unroll
+					for(int c = 0; c < C; c++) V[(size_t)site[j] * C + c] = v[j][c];
+			__syncthreads();
+#pragma unroll
+			for(int j = 0; j < S; j++)
+				if(act[j]) stencil_gather<R, C>(V, nbr[j], cf[j], a.mu, v[j], w[j]);
+			if(a.cplx[p]){
+				// W is free: its last readers passed the barrier above.  The barrier below also
+				// retires every read of V before the next pair overwrites it.
+#pragma unroll
+				for(int j = 0; j < S; j++)
+					if(act[j])
+#pragma unroll
+						for(int c = 0; c < C; c++) W[(size_t)site[j] * C + c] = w[j][c];
+				__syncthreads();
+#pragma unroll
+				for(int j = 0; j < S; j++){
+					if(!act[j]) continue;
+					R z[C];
+					stencil_gather<R, C>(W, nbr[j], cf[j], a.mu, w[j], z);
+#pragma unroll
+					for(int c = 0; c < C; c++) v[j][c] += a.a_lin[p] * w[j][c] + a.a_quad[p] * z[c];
+				}
+			}else{
+				__syncthreads();
+#pragma unroll
+				for(int j = 0; j < S; j++)
+#pragma unroll
+					for(int c = 0; c < C; c++) v[j][c] += a.a_lin[p] * w[j][c];
+			}
+		}
+	}
+	if(a.dst){
+#pragma unroll
+		for(int j = 0; j < S; j++)
+#pragma unroll
+			for(int c = 0; c < C; c++)
+				if(act[j] && col0 + c < ns) a.dst[(size_t)(col0 + c) * ns + site[j]] = v[j][c];
+	}
+}
+
+// Launch shape: threads per workgroup and sites per thread from the lattice size; columns
+// per workgroup from the tuning table (ctx->cols) capped by LDS.
+template <typename R, int C>
+static int launch_chain_c(ephmc_ctx *c, const ChainArgs<R> &a)
+{
+	const unsigned ns = a.ns;
+	const unsigned blocks = (ns + C - 1) / C;
+	const size_t lds = 2 * (size_t)C * ns * sizeof(R);
+	if(lds > 160 * 1024){ set_error("chain: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
+#define EPHMC_CHAIN_LAUNCH(BT, S)                                                                  \
+	do {                                                                                           \
+		(void)hipFuncSetAttribute((const void *)chain_kernel<R, C, BT, S>,                               \
+			hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                 \
+		chain_kernel<R, C, BT, S><<<blocks, BT, lds, c->stream>>>(a);                             \
+	} while(0)
+	if(ns <= 256) EPHMC_CHAIN_LAUNCH(256, 1);
+	else if(ns <= 512) EPHMC_CHAIN_LAUNCH(512, 1);
+	else if(ns <= 1024) EPHMC_CHAIN_LAUNCH(1024, 1);
+	else if(ns <= 2048) EPHMC_CHAIN_LAUNCH(1024, 2);
+	else if(ns <= 4096) EPHMC_CHAIN_LAUNCH(1024, 4);
+	else { set_error("chain: lattice of %u sites exceeds 4096", ns); return EPHMC_ERR_UNSUPPORTED; }
+#undef EPHMC_CHAIN_LAUNCH
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+template <typename R>
+static int launch_chain(ephmc_ctx *c, const ChainArgs<R> &a, unsigned cols)
+{
+	// LDS budget: 2 * C * ns * sizeof(R) <= 160 KiB
+	while(cols > 1 && 2 * (size_t)cols * a.ns * sizeof(R) > 160 * 1024) cols >>= 1;
+	switch(cols){
+		case 1: return launch_chain_c<R, 1>(c, a);
+		case 2: return launch_chain_c<R, 2>(c, a);
+		case 4: return launch_chain_c<R, 4>(c, a);
+		default: return launch_chain_c<R, 8>(c, a);
+	}
+}
+
+template <typename R>
+static ChainArgs<R> make_args(ephmc_ctx *c, const R *coef)
+{
+	ChainArgs<R> a{};
+	a.coef = coef;
+	a.nb = c->d_nb;
+	a.mu = (R)c->cfg.mu;
+	a.ns = c->ns;
+	a.np = c->poly.np;
+	for(int p = 0; p < c->poly.np; p++){
+		a.a_lin[p] = (R)c->poly.a_lin[p];
+		a.a_quad[p] = (R)c->poly.a_quad[p];
+		a.cplx[p] = c->poly.cplx[p];
+	}
+	return a;
+}
+
+unsigned chain_cols(const ephmc_ctx *c, int prec)
+{
+	// columns per workgroup of the forward chain: enough workgroups to cover 256 CUs;
+	// EPHMC_CHAIN_COLS overrides it for tuning sweeps
+	const unsigned ns = c->ns;
+	if(const char *e = getenv("EPHMC_CHAIN_COLS")){
+		const unsigned v = (unsigned)atoi(e);
+		if(v == 1 || v == 2 || v == 4 || v == 8) return v;
+	}
+	unsigned cols = prec == EPHMC_FP64 ? 4 : 8;
+	while(cols > 1 && (ns + cols - 1) / cols < 256) cols >>= 1;
+	return cols;
+}
+
+// Chain of slices over all columns: dst = B_{tau_last} ... B_{tau_first} src, with the
+// slices tau0, tau0+dir, ... (nt of them); src null = identity.  Stores the state before
+// every ck_every-th slice into ck.  prec selects fp64 or fp32 storage and arithmetic.
+int chain_forward(ephmc_ctx *c, int prec, const void *src, void *dst, void *ck, unsigned ck_every,
+		int descending, bool from_identity)
+{
+	if(prec == EPHMC_FP64){
+		ChainArgs<double> a = make_args<double>(c, c->d_coef64);
+		a.src = from_identity ? nullptr : (const double *)src;
+		a.dst = (double *)dst;
+		a.ck = (double *)ck;
+		a.ck_every = ck_every ? ck_every : 1;
+		a.tau0 = descending ? (int)c->nt - 1 : 0;
+		a.dir = descending ? -1 : 1;
+		a.ntau = (int)c->nt;
+		return launch_chain<double>(c, a, chain_cols(c, prec));
+	}else{
+		ChainArgs<float> a = make_args<float>(c, c->d_coef32);
+		a.src = from_identity ? nullptr : (const float *)src;
+		a.dst = (float *)dst;
+		a.ck = (float *)ck;
+		a.ck_every = ck_every ? ck_every : 1;
+		a.tau0 = descending ? (int)c->nt - 1 : 0;
+		a.dir = descending ? -1 : 1;
+		a.ntau = (int)c->nt;
+		return launch_chain<float>(c, a, chain_cols(c, prec));
+	}
+}
+
+// apply_exp_mu_K_to_M with partial pair products (revert/stop, organic_dense_matrix.c:186-194)
+// for a single slice whose 3N couplings are x_tau.  Uses d_coef64 slot 0 as scratch.
+int apply_exp_partial(ephmc_ctx *c, const double *x_tau, double *M, int revert, int stop)
+{
+	const unsigned ns = c->ns;
+	coef_kernel<<<(6 * ns + 255) / 256, 256, 0, c->stream>>>(x_tau, c->d_link, c->cfg.t[0], c->cfg.t[1], c->cfg.t[2],
+		c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], ns, 1, c->d_coef64, nullptr);
+	EPHMC_LAUNCHED();
+	ChainArgs<double> a = make_args<double>(c, c->d_coef64);
+	a.np = 0;
+	auto add = [&](int p){
+		a.a_lin[a.np] = c->poly.a_lin[p];
+		a.a_quad[a.np] = c->poly.a_quad[p];
+		a.cplx[a.np] = c->poly.cplx[p];
+		a.np++;
+	};
+	if(revert){ for(int p = c->poly.np - 1; p > stop; p--) add(p); }
+	else{ for(int p = 0; p < stop; p++) add(p); }
+	if(a.np == 0) return EPHMC_OK;
+	a.src = M;
+	a.dst = M;       // each workgroup reads its columns before writing them
+	a.ck = nullptr;
+	a.ck_every = 1;
+	a.tau0 = 0; a.dir = 1; a.ntau = 1;
+	return launch_chain<double>(c, a, chain_cols(c, EPHMC_FP64));
+}
+
+// ---------------------------------------------------------------------------------------------
+// M = T^T (+ identity): 32x32 tiles through LDS; T in R, M in fp64.
+template <typename R>
+__global__ __launch_bounds__(256) void transpose_kernel(const R *__restrict__ T, double *__restrict__ M,
+		unsigned n, int add_identity)
+{
+	__shared__ double tile[32][33];
+	const unsigned bx = blockIdx.x * 32, by = blockIdx.y * 32;
+	const unsigned tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+	for(unsigned r = ty; r < 32; r += 8){
+		const unsigned col = by + r, row = bx + tx;          // T[col][row] (column col of T)
+		if(col < n && row < n) tile[r][tx] = (double)T[(size_t)col * n + row];
+	}
+	__syncthreads();
+	for(unsigned r = ty; r < 32; r += 8){
+		const unsigned col = bx + r, row = by + tx;          // M column col = row col of T
+		if(col < n && row < n) M[(size_t)col * n + row] = tile[tx][r] + ((add_identity && col == row) ? 1.0 : 0.0);
+	}
+}
+
+int transpose_plus_identity(ephmc_ctx *c, int prec, const void *T, double *M, int add_identity)
+{
+	const unsigned n = c->ns;
+	dim3 grid((n + 31) / 32, (n + 31) / 32);
+	if(prec == EPHMC_FP64) transpose_kernel<double><<<grid, 256, 0, c->stream>>>((const double *)T, M, n, add_identity);
+	else transpose_kernel<float><<<grid, 256, 0, c->stream>>>((const float *)T, M, n, add_identity);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+__global__ void add_identity_kernel(double *__restrict__ M, unsigned n)
+{
+	for(unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) M[(size_t)i * (n + 1)] += 1.0;
+}
+
+int add_identity(ephmc_ctx *c, double *M)
+{
+	add_identity_kernel<<<(c->ns + 255) / 256, 256, 0, c->stream>>>(M, c->ns);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+__global__ void f64_to_f32_kernel(const double *__restrict__ a, float *__restrict__ b, size_t n)
+{
+	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+		b[e] = (float)a[e];
+}
+
+int convert_f64_to_f32(ephmc_ctx *c, const double *a, float *b, size_t n)
+{
+	const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+	f64_to_f32_kernel<<<blocks, 256, 0, c->stream>>>(a, b, n);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+} // namespace ephmc

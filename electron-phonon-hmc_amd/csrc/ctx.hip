@@ -1,0 +1,598 @@
+// ctx.hip -- context, memory, C-ABI entry points and the device-resident HMC driver.
+#include "internal.h"
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+#include <cstdlib>
+
+namespace ephmc {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_err, sizeof(g_err), fmt, ap);
+	va_end(ap);
+}
+
+unsigned chain_cols(const ephmc_ctx *c, int prec);
+
+// Taylor pair coefficients of the truncated exponential series (the tables of
+// taylor-coeff_pairs.h for the even orders; their pair products equal sum_{k<=order} z^k/k!)
+// and the application order of set_taylor_pair_order (organic_dense_matrix.c:133-159).
+static int make_poly(int order, Poly &P)
+{
+	static const double t2[][2] = {{1, 1}};
+	static const double t4[][2] = {{1.8294933739891903, 0.9404019519941707}, {0.1705066260108097, 1.5785318126884535}};
+	static const double t6[][2] = {{2.2561549994606459, 0.8008468192975424}, {1.0805822170351833, 1.8245938269821296},
+		{-0.3367372164958292, 1.5494473211106643}};
+	static const double t8[][2] = {{2.5115385528762511, 0.6852625793459637}, {1.6810015288012661, 1.7481130845728929},
+		{0.4249750401763088, 2.0321227226190966}, {-0.6175151218538260, 1.4284655522676216}};
+	const double (*tab)[2];
+	int np;
+	switch(order){
+		case 2: tab = t2; np = 1; break;
+		case 4: tab = t4; np = 2; break;
+		case 6: tab = t6; np = 3; break;
+		case 8: tab = t8; np = 4; break;
+		default: return EPHMC_ERR_ARG;
+	}
+	int ord[MAX_PAIRS];
+	const int tuples = np / 2;
+	int lo = 0, hi = tuples - 1;
+	for(int q = 0; q < tuples; q++){
+		const int tup = (q % 3 == 1) ? lo++ : hi--;
+		ord[2 * q] = tup;
+		ord[2 * q + 1] = np - 1 - tup;
+	}
+	if(np % 2) ord[2 * tuples] = tuples;
+	P.np = np;
+	double poly[MAX_DEG + 1] = {1};
+	int deg = 0;
+	for(int i = 0; i < np; i++){
+		const double re = .5 * tab[ord[i]][0] / np, im = .5 * tab[ord[i]][1] / np;
+		P.a_lin[i] = 2 * re;
+		P.a_quad[i] = re * re + im * im;
+		P.cplx[i] = im != 0;
+		const double f[3] = {1, P.a_lin[i], P.cplx[i] ? P.a_quad[i] : 0};
+		const int fd = P.cplx[i] ? 2 : 1;
+		double nxt[MAX_DEG + 1] = {0};
+		for(int a = 0; a <= deg; a++) for(int b = 0; b <= fd; b++) nxt[a + b] += poly[a] * f[b];
+		deg += fd;
+		std::copy(nxt, nxt + MAX_DEG + 1, poly);
+	}
+	P.deg = deg;
+	std::copy(poly, poly + MAX_DEG + 1, P.c);
+	return EPHMC_OK;
+}
+
+// Periodic triangular parallelogram, organic_lattice.c:10-46 (same table layout).
+static void make_lattice(unsigned l1, unsigned l2, std::vector<unsigned> &nnt)
+{
+	const unsigned ns = l1 * l2;
+	nnt.assign(12 * (size_t)ns, 0);
+	for(unsigned i = 0; i < ns; i++){
+		const unsigned x1 = i % l1, x2 = i / l1;
+		const unsigned up = (x2 + 1) % l2, dn = (x2 + l2 - 1) % l2;
+		unsigned *n = &nnt[6 * i], *k = &nnt[6 * ((size_t)ns + i)];
+		n[0] = x2 * l1 + (x1 + 1) % l1;
+		n[1] = x2 * l1 + (x1 + l1 - 1) % l1;
+		n[2] = up * l1 + x1;
+		n[3] = dn * l1 + x1;
+		n[4] = up * l1 + (x1 + l1 - 1) % l1;
+		n[5] = dn * l1 + (x1 + 1) % l1;
+		k[0] = 3 * i;        k[1] = 3 * n[1];
+		k[2] = 3 * i + 1;    k[3] = 3 * n[3] + 1;
+		k[4] = 3 * i + 2;    k[5] = 3 * n[5] + 2;
+	}
+}
+
+template <typename T>
+static int dalloc(T **p, size_t n)
+{
+	*p = nullptr;
+	if(n == 0) return EPHMC_OK;
+	if(hipMalloc((void **)p, n * sizeof(T)) != hipSuccess){
+		(void)hipGetLastError();
+		set_error("hipMalloc of %zu bytes failed", n * sizeof(T));
+		return EPHMC_ERR_ALLOC;
+	}
+	return EPHMC_OK;
+}
+
+static int timer_harvest(TimerGroup &g)
+{
+	for(size_t i = 0; i + 1 < g.used; i += 2){
+		float ms = 0;
+		EPHMC_HIP(hipEventSynchronize(g.ev[i + 1]));
+		EPHMC_HIP(hipEventElapsedTime(&ms, g.ev[i], g.ev[i + 1]));
+		g.total_ms += ms;
+		g.calls++;
+	}
+	g.used = 0;
+	return EPHMC_OK;
+}
+
+int timer_begin(ephmc_ctx *c, const char *group)
+{
+	if(!c->timing) return EPHMC_OK;
+	TimerGroup &g = c->timers[group];
+	if(g.used + 2 > 4096) EPHMC_TRY(timer_harvest(g));
+	while(g.ev.size() < g.used + 2){
+		hipEvent_t e;
+		EPHMC_HIP(hipEventCreate(&e));
+		g.ev.push_back(e);
+	}
+	EPHMC_HIP(hipEventRecord(g.ev[g.used], c->stream));
+	g.open = true;
+	return EPHMC_OK;
+}
+
+int timer_end(ephmc_ctx *c, const char *group)
+{
+	if(!c->timing) return EPHMC_OK;
+	TimerGroup &g = c->timers[group];
+	if(!g.open) return EPHMC_OK;
+	EPHMC_HIP(hipEventRecord(g.ev[g.used + 1], c->stream));
+	g.used += 2;
+	g.open = false;
+	return EPHMC_OK;
+}
+
+// Columns per workgroup of the adjoint sweep: the widest that fits LDS, keeps at least one
+// workgroup per CU and (fp64) stays within the register file of a 16-wave workgroup.
+// EPHMC_ADJOINT_COLS overrides it for tuning sweeps.
+static unsigned adjoint_cols_auto(unsigned ns, size_t rsize)
+{
+	if(const char *e = getenv("EPHMC_ADJOINT_COLS")){
+		const unsigned v = (unsigned)atoi(e);
+		if(v == 1 || v == 2 || v == 4) return v;
+	}
+	const unsigned widest = rsize == 8 ? 2u : 4u;
+	for(unsigned C = widest; C > 1; C >>= 1){
+		const size_t lds = (4 * (size_t)C * ns + 3 * (size_t)ns) * rsize;
+		if(lds <= 160 * 1024 && (ns + C - 1) / C >= 256) return C;
+	}
+	return 1;
+}
+
+// ---- one MD force evaluation at field x: coefficients, forward chain with checkpoints,
+// M_hat, Gauss-Jordan inverse, adjoint sweep; p += kick * F when p != null.
+static int md_force(ephmc_ctx *c, const double *x, double *pdot, double kick, double *p)
+{
+	const int prec = c->cfg.precision;
+	EPHMC_TRY(timer_begin(c, "chain"));
+	EPHMC_TRY(coefficients(c, x, prec != EPHMC_FP64));
+	void *tfin = prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32;
+	EPHMC_TRY(chain_forward(c, prec, nullptr, tfin, c->d_ck, c->ckpt, 0, true));
+	EPHMC_TRY(transpose_plus_identity(c, prec, tfin, c->d_mhat, 1));
+	EPHMC_TRY(timer_end(c, "chain"));
+	EPHMC_TRY(timer_begin(c, "gj"));
+	EPHMC_TRY(gj_logdet_inverse(c, c->d_mhat));
+	EPHMC_TRY(timer_end(c, "gj"));
+	EPHMC_TRY(timer_begin(c, "adjoint"));
+	EPHMC_TRY(force_adjoint(c, prec, c->d_mhat, pdot, kick, p));
+	EPHMC_TRY(timer_end(c, "adjoint"));
+	return EPHMC_OK;
+}
+
+// FP64 M_hat^{-1} and log det of field x into d_mhat / scal[SCAL_LOGDET] (Metropolis quality)
+// (slices applied from nt-1 down to 0 as in construct_M_hat, organic_dense_matrix.c:256-258)
+static int exact_logdet(ephmc_ctx *c, const double *x)
+{
+	EPHMC_TRY(coefficients(c, x, false));
+	EPHMC_TRY(chain_forward(c, EPHMC_FP64, nullptr, c->d_mhat, nullptr, 0, 1, true));
+	EPHMC_TRY(add_identity(c, c->d_mhat));
+	EPHMC_TRY(gj_logdet_inverse(c, c->d_mhat));
+	return EPHMC_OK;
+}
+
+// organic_hmc.c:167-190 on (x, p): x(h/2), then [force, p += h F, x(h)] with the final drift h/2.
+static int leap_frog_dev(ephmc_ctx *c, double *x, double *p)
+{
+	const unsigned steps = c->cfg.steps;
+	const double h = steps ? c->traj_length / steps : 2 * c->traj_length;
+	EPHMC_TRY(timer_begin(c, "fft"));
+	EPHMC_TRY(fa_evolve(c, x, p, 0.5 * h, nullptr, 0));
+	EPHMC_TRY(timer_end(c, "fft"));
+	for(unsigned s = 1; s <= steps; s++){
+		EPHMC_TRY(md_force(c, x, c->d_pdot, 0, nullptr));
+		const double drift = (s == steps) ? 0.5 * h : h;
+		// kick p += h F fused into the drift's load
+		EPHMC_TRY(timer_begin(c, "fft"));
+		EPHMC_TRY(fa_evolve(c, x, p, drift, c->d_pdot, h));
+		EPHMC_TRY(timer_end(c, "fft"));
+	}
+	return EPHMC_OK;
+}
+
+static int upload_normals(ephmc_ctx *c, const double *normals)
+{
+	// double-buffered pinned staging so the caller may reuse its buffer on return
+	const int s = c->stage_next;
+	c->stage_next ^= 1;
+	EPHMC_HIP(hipEventSynchronize(c->stage_ev[s]));
+	std::memcpy(c->h_stage[s], normals, c->dim * sizeof(double));
+	EPHMC_HIP(hipMemcpyAsync(c->d_normals, c->h_stage[s], c->dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
+	EPHMC_HIP(hipEventRecord(c->stage_ev[s], c->stream));
+	return EPHMC_OK;
+}
+
+static int read_scal(ephmc_ctx *c, int slot, double *out)
+{
+	EPHMC_HIP(hipMemcpyAsync(out, c->d_scal + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	return EPHMC_OK;
+}
+
+} // namespace ephmc
+
+using namespace ephmc;
+
+// =============================================================================================
+extern "C" {
+
+const char *ephmc_last_error(void){ return g_err; }
+
+int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
+{
+	if(!cfg || !out){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	*out = nullptr;
+	if(cfg->l1 == 0 || cfg->l2 == 0 || cfg->nt == 0){ set_error("empty lattice"); return EPHMC_ERR_ARG; }
+	if(cfg->precision != EPHMC_FP64 && cfg->precision != EPHMC_MIXED){ set_error("precision"); return EPHMC_ERR_ARG; }
+	if(cfg->zero_filling || cfg->single_particle || cfg->constant_force || cfg->no_fourier_acc || cfg->classical_fourier_acc){
+		set_error("flags ZERO_FILLING, SINGLE_PARTICLE, CONSTANT_FORCE, NO_FOURIER_ACC and CLASSICAL_FOURIER_ACC are not provided by this build");
+		return EPHMC_ERR_UNSUPPORTED;
+	}
+	const unsigned ns = cfg->l1 * cfg->l2;
+	if(ns > 2048){ set_error("lattice of %u sites exceeds 2048", ns); return EPHMC_ERR_UNSUPPORTED; }
+	ephmc_ctx *c = new ephmc_ctx();
+	c->cfg = *cfg;
+	c->l1 = cfg->l1; c->l2 = cfg->l2; c->ns = ns; c->nt = cfg->nt; c->loc = 3 * ns;
+	c->dim = (size_t)c->nt * c->loc;
+	c->mat = (size_t)ns * ns;
+	c->traj_length = cfg->traj_length == 0 ? 1.570796326794897 : cfg->traj_length;
+	if(c->cfg.taylor_order == 0) c->cfg.taylor_order = 4;
+	if(c->cfg.flavors == 0) c->cfg.flavors = 1;
+	int rc = make_poly(c->cfg.taylor_order, c->poly);
+	if(rc != EPHMC_OK){ set_error("taylor order %d not provided (2, 4, 6, 8)", cfg->taylor_order); delete c; return rc; }
+	c->device = cfg->device;
+	if(hipSetDevice(c->device) != hipSuccess){ set_error("hipSetDevice(%d) failed", c->device); delete c; return EPHMC_ERR_HIP; }
+	if(cfg->stream){ c->stream = (hipStream_t)cfg->stream; c->own_stream = false; }
+	else{
+		if(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess){ set_error("stream"); delete c; return EPHMC_ERR_HIP; }
+		c->own_stream = true;
+	}
+	make_lattice(c->l1, c->l2, c->nnt);
+	c->ckpt = cfg->checkpoint ? std::min(cfg->checkpoint, c->nt) : std::min(16u, c->nt);
+	c->nseg = (c->nt + c->ckpt - 1) / c->ckpt;
+	const size_t rsz = cfg->precision == EPHMC_FP64 ? 8 : 4;
+	c->cols = adjoint_cols_auto(ns, rsz);
+	c->nwg = (ns + c->cols - 1) / c->cols;
+	c->red_blocks = (unsigned)std::min<size_t>((c->dim + 255) / 256, 1024);
+	const size_t fft_blocks = (c->loc + 31) / 32 + 1;
+
+	auto fail = [&](int r){ ephmc_destroy(c); return r; };
+	std::vector<int> nb(6 * (size_t)ns);
+	std::vector<unsigned> link(6 * (size_t)ns);
+	for(unsigned i = 0; i < ns; i++)
+		for(int d = 0; d < 6; d++){
+			nb[(size_t)d * ns + i] = (int)c->nnt[6 * i + d];
+			link[(size_t)d * ns + i] = c->nnt[6 * ((size_t)ns + i) + d];
+		}
+	const bool mixed = cfg->precision == EPHMC_MIXED;
+	if((rc = dalloc(&c->d_nb, nb.size())) || (rc = dalloc(&c->d_link, link.size()))
+		|| (rc = dalloc(&c->d_coef64, (size_t)c->nt * 6 * ns))
+		|| (rc = dalloc(&c->d_coef32, mixed ? (size_t)c->nt * 6 * ns : 0))
+		|| (rc = dalloc((unsigned char **)&c->d_ck, (size_t)c->nseg * c->mat * rsz))
+		|| (rc = dalloc((unsigned char **)&c->d_seg, (size_t)c->nwg * c->cols * ns * c->ckpt * rsz))
+		|| (rc = dalloc(&c->d_partial, (size_t)c->ckpt * c->nwg * c->loc))
+		|| (rc = dalloc(&c->d_Tfin, c->mat))
+		|| (rc = dalloc(&c->d_Tfin32, mixed ? c->mat : 0))
+		|| (rc = dalloc(&c->d_gj[0], c->mat)) || (rc = dalloc(&c->d_gj[1], c->mat))
+		|| (rc = dalloc(&c->d_piv, ns)) || (rc = dalloc(&c->d_src, ns)) || (rc = dalloc(&c->d_diag, ns))
+		|| (rc = dalloc(&c->d_scal, SCAL_COUNT))
+		|| (rc = dalloc(&c->d_twiddle, 2 * (size_t)c->nt)) || (rc = dalloc(&c->d_f2, c->nt))
+		|| (rc = dalloc(&c->d_red, std::max<size_t>(2 * c->red_blocks, fft_blocks)))
+		|| (rc = dalloc(&c->d_x, c->dim)) || (rc = dalloc(&c->d_p, c->dim)) || (rc = dalloc(&c->d_pdot, c->dim))
+		|| (rc = dalloc(&c->d_xnew, c->dim)) || (rc = dalloc(&c->d_normals, c->dim))
+		|| (rc = dalloc(&c->d_minv, c->mat)) || (rc = dalloc(&c->d_mhat, c->mat)))
+		return fail(rc);
+	for(int s = 0; s < 2; s++){
+		if(hipHostMalloc((void **)&c->h_stage[s], c->dim * sizeof(double), hipHostMallocDefault) != hipSuccess){
+			set_error("pinned staging allocation failed");
+			return fail(EPHMC_ERR_ALLOC);
+		}
+		if(hipEventCreateWithFlags(&c->stage_ev[s], hipEventDisableTiming) != hipSuccess){ set_error("event"); return fail(EPHMC_ERR_HIP); }
+		hipEventRecord(c->stage_ev[s], c->stream);
+	}
+	std::vector<double> tw(2 * (size_t)c->nt), f2(c->nt);
+	const double pi = 3.141592653589793;
+	for(unsigned j = 0; j < c->nt; j++){
+		tw[2 * j] = cos(2 * pi * j / c->nt);
+		tw[2 * j + 1] = -sin(2 * pi * j / c->nt);
+		const double s = 2 * sin(pi / c->nt * j);
+		f2[j] = cfg->omega + s * s / cfg->omega;   // organic_bosons.c:137
+	}
+	if(hipMemcpy(c->d_nb, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess
+		|| hipMemcpy(c->d_link, link.data(), link.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess
+		|| hipMemcpy(c->d_twiddle, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+		|| hipMemcpy(c->d_f2, f2.data(), f2.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+		|| hipMemset(c->d_scal, 0, SCAL_COUNT * sizeof(double)) != hipSuccess){
+		set_error("initial uploads failed");
+		return fail(EPHMC_ERR_HIP);
+	}
+	*out = c;
+	return EPHMC_OK;
+}
+
+int ephmc_destroy(ephmc_ctx *c)
+{
+	if(!c) return EPHMC_OK;
+	if(c->stream) hipStreamSynchronize(c->stream);
+	void *bufs[] = {c->d_nb, c->d_link, c->d_coef64, c->d_coef32, c->d_ck, c->d_seg, c->d_partial, c->d_Tfin, c->d_Tfin32,
+		c->d_gj[0], c->d_gj[1], c->d_piv, c->d_src, c->d_diag, c->d_scal, c->d_twiddle, c->d_f2, c->d_red,
+		c->d_x, c->d_p, c->d_pdot, c->d_xnew, c->d_normals, c->d_minv, c->d_mhat};
+	for(void *b : bufs) if(b) hipFree(b);
+	for(int s = 0; s < 2; s++){
+		if(c->h_stage[s]) hipHostFree(c->h_stage[s]);
+		if(c->stage_ev[s]) hipEventDestroy(c->stage_ev[s]);
+	}
+	for(auto &kv : c->timers) for(hipEvent_t e : kv.second.ev) hipEventDestroy(e);
+	if(c->own_stream && c->stream) hipStreamDestroy(c->stream);
+	delete c;
+	return EPHMC_OK;
+}
+
+int ephmc_synchronize(ephmc_ctx *c)
+{
+	if(!c){ set_error("null context"); return EPHMC_ERR_ARG; }
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	return EPHMC_OK;
+}
+
+int ephmc_dims(const ephmc_ctx *c, unsigned *ns, unsigned *nt, size_t *dim)
+{
+	if(!c){ set_error("null context"); return EPHMC_ERR_ARG; }
+	if(ns) *ns = c->ns;
+	if(nt) *nt = c->nt;
+	if(dim) *dim = c->dim;
+	return EPHMC_OK;
+}
+
+int ephmc_lattice(const ephmc_ctx *c, unsigned *nnt)
+{
+	if(!c || !nnt){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	std::memcpy(nnt, c->nnt.data(), c->nnt.size() * sizeof(unsigned));
+	return EPHMC_OK;
+}
+
+int ephmc_malloc(size_t bytes, void **dev)
+{
+	if(!dev){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	if(hipMalloc(dev, bytes) != hipSuccess){ (void)hipGetLastError(); set_error("hipMalloc(%zu) failed", bytes); return EPHMC_ERR_ALLOC; }
+	return EPHMC_OK;
+}
+
+int ephmc_free(void *dev)
+{
+	if(dev) EPHMC_HIP(hipFree(dev));
+	return EPHMC_OK;
+}
+
+int ephmc_memcpy_h2d(ephmc_ctx *c, void *dev, const void *host, size_t bytes)
+{
+	if(!c){ set_error("null context"); return EPHMC_ERR_ARG; }
+	EPHMC_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream));
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	return EPHMC_OK;
+}
+
+int ephmc_memcpy_d2h(ephmc_ctx *c, void *host, const void *dev, size_t bytes)
+{
+	if(!c){ set_error("null context"); return EPHMC_ERR_ARG; }
+	EPHMC_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	return EPHMC_OK;
+}
+
+int ephmc_apply_exp_M(ephmc_ctx *c, const double *x_tau, double *M, int revert, int stop)
+{
+	if(!c || !x_tau || !M){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	return apply_exp_partial(c, x_tau, M, revert, stop);
+}
+
+int ephmc_construct_M_hat(ephmc_ctx *c, const double *x, double *M_hat)
+{
+	if(!c || !x || !M_hat){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	EPHMC_TRY(coefficients(c, x, false));
+	// B_0 (B_1 ( ... B_{nt-1} I)) in the reference's order (slices applied from nt-1 down to 0)
+	EPHMC_TRY(chain_forward(c, EPHMC_FP64, nullptr, M_hat, nullptr, 0, 1, true));
+	return add_identity(c, M_hat);
+}
+
+int ephmc_logdet_inv(ephmc_ctx *c, double *M_hat, double *logdet, int *sign)
+{
+	if(!c || !M_hat){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	EPHMC_TRY(timer_begin(c, "gj"));
+	EPHMC_TRY(gj_logdet_inverse(c, M_hat));
+	EPHMC_TRY(timer_end(c, "gj"));
+	if(logdet) EPHMC_TRY(read_scal(c, SCAL_LOGDET, logdet));
+	if(sign){ double s; EPHMC_TRY(read_scal(c, SCAL_SIGN, &s)); *sign = s < 0 ? -1 : 1; }
+	return EPHMC_OK;
+}
+
+int ephmc_force(ephmc_ctx *c, const double *x, const double *M_inv, double logdet, double *p_dot)
+{
+	(void)logdet;
+	if(!c || !x || !M_inv || !p_dot){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	const int prec = c->cfg.precision;
+	EPHMC_TRY(coefficients(c, x, prec != EPHMC_FP64));
+	void *tfin = prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32;
+	EPHMC_TRY(chain_forward(c, prec, nullptr, tfin, c->d_ck, c->ckpt, 0, true));
+	EPHMC_TRY(timer_begin(c, "adjoint"));
+	EPHMC_TRY(force_adjoint(c, prec, M_inv, p_dot, 0, nullptr));
+	EPHMC_TRY(timer_end(c, "adjoint"));
+	return EPHMC_OK;
+}
+
+int ephmc_sample_free_bosons(ephmc_ctx *c, const double *normals, double *x)
+{
+	if(!c || !normals || !x){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	return fa_filter(c, normals, x, 1);
+}
+
+int ephmc_sample_fourier_momenta(ephmc_ctx *c, const double *normals, double *p)
+{
+	if(!c || !normals || !p){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	return fa_filter(c, normals, p, 0);
+}
+
+int ephmc_energy_momenta(ephmc_ctx *c, const double *p, double *energy)
+{
+	if(!c || !p || !energy){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	EPHMC_TRY(fa_energy(c, p, c->d_scal + SCAL_EP));
+	return read_scal(c, SCAL_EP, energy);
+}
+
+int ephmc_evolve_bosons(ephmc_ctx *c, double *x, double *p, double h)
+{
+	if(!c || !x || !p){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	return fa_evolve(c, x, p, h, nullptr, 0);
+}
+
+int ephmc_hamilton(ephmc_ctx *c, const double *x, double logdet, double energyP, double *H)
+{
+	if(!c || !x || !H){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	EPHMC_TRY(boson_action(c, x, c->d_scal + SCAL_BOSON));
+	double sb;
+	EPHMC_TRY(read_scal(c, SCAL_BOSON, &sb));
+	*H = -logdet * c->cfg.flavors + energyP + sb;
+	return EPHMC_OK;
+}
+
+int ephmc_leap_frog(ephmc_ctx *c, double *x, double *p)
+{
+	if(!c || !x || !p){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	return leap_frog_dev(c, x, p);
+}
+
+int ephmc_chain_init(ephmc_ctx *c, const double *normals)
+{
+	if(!c || !normals){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	EPHMC_TRY(upload_normals(c, normals));
+	EPHMC_TRY(fa_filter(c, c->d_normals, c->d_x, 1));
+	EPHMC_TRY(exact_logdet(c, c->d_x));
+	EPHMC_HIP(hipMemcpyAsync(c->d_minv, c->d_mhat, c->mat * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+	EPHMC_HIP(hipMemcpyAsync(c->d_scal + SCAL_LOGDET_CHAIN, c->d_scal + SCAL_LOGDET, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+	c->chain_ready = true;
+	return EPHMC_OK;
+}
+
+int ephmc_chain_set_x(ephmc_ctx *c, const double *x_host)
+{
+	if(!c || !x_host){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	EPHMC_HIP(hipMemcpyAsync(c->d_x, x_host, c->dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
+	EPHMC_TRY(exact_logdet(c, c->d_x));
+	EPHMC_HIP(hipMemcpyAsync(c->d_minv, c->d_mhat, c->mat * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+	EPHMC_HIP(hipMemcpyAsync(c->d_scal + SCAL_LOGDET_CHAIN, c->d_scal + SCAL_LOGDET, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	c->chain_ready = true;
+	return EPHMC_OK;
+}
+
+int ephmc_trajectory(ephmc_ctx *c, const double *normals, double uniform, ephmc_traj_record *rec)
+{
+	if(!c || !normals){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	if(!c->chain_ready){ set_error("ephmc_chain_init has not run"); return EPHMC_ERR_ARG; }
+	// organic_hmc.c:213-243
+	EPHMC_TRY(upload_normals(c, normals));
+	EPHMC_HIP(hipMemcpyAsync(c->d_xnew, c->d_x, c->dim * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+	EPHMC_TRY(fa_filter(c, c->d_normals, c->d_p, 0));
+	EPHMC_TRY(hamilton_dev(c, c->d_xnew, c->d_p, SCAL_LOGDET_CHAIN, SCAL_H_OLD));
+	EPHMC_TRY(leap_frog_dev(c, c->d_xnew, c->d_p));
+	EPHMC_TRY(timer_begin(c, "metropolis"));
+	EPHMC_TRY(exact_logdet(c, c->d_xnew));
+	EPHMC_TRY(hamilton_dev(c, c->d_xnew, c->d_p, SCAL_LOGDET, SCAL_H_NEW));
+	EPHMC_TRY(metropolis(c, uniform));
+	EPHMC_TRY(timer_end(c, "metropolis"));
+	if(rec) return ephmc_last_record(c, rec);
+	return EPHMC_OK;
+}
+
+int ephmc_last_record(ephmc_ctx *c, ephmc_traj_record *rec)
+{
+	if(!c || !rec){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	double s[SCAL_COUNT];
+	EPHMC_HIP(hipMemcpyAsync(s, c->d_scal, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	rec->accepted = s[SCAL_ACC] != 0;
+	rec->boltzmann = s[SCAL_BOLTZ];
+	rec->energy = s[SCAL_ENERGY];
+	rec->logdet = s[SCAL_LOGDET_CHAIN];
+	rec->dH = s[SCAL_DH];
+	return EPHMC_OK;
+}
+
+int ephmc_chain_get_x(ephmc_ctx *c, double *x)
+{
+	if(!c || !x){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	return ephmc_memcpy_d2h(c, x, c->d_x, c->dim * sizeof(double));
+}
+
+int ephmc_chain_get_minv(ephmc_ctx *c, double *m)
+{
+	if(!c || !m){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	return ephmc_memcpy_d2h(c, m, c->d_minv, c->mat * sizeof(double));
+}
+
+int ephmc_chain_get_logdet(ephmc_ctx *c, double *ld)
+{
+	if(!c || !ld){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	return read_scal(c, SCAL_LOGDET_CHAIN, ld);
+}
+
+int ephmc_chain_buffers(ephmc_ctx *c, double **x, double **p, double **minv)
+{
+	if(!c){ set_error("null context"); return EPHMC_ERR_ARG; }
+	if(x) *x = c->d_x;
+	if(p) *p = c->d_p;
+	if(minv) *minv = c->d_minv;
+	return EPHMC_OK;
+}
+
+int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamples, const double *contr_mat, double width,
+		unsigned nbins, double *greens, double *results)
+{
+	(void)c; (void)normals; (void)nsamples; (void)contr_mat; (void)width; (void)nbins; (void)greens; (void)results;
+	set_error("transient localisation is not provided by this build yet");
+	return EPHMC_ERR_UNSUPPORTED;
+}
+
+int ephmc_timing(ephmc_ctx *c, int enable)
+{
+	if(!c){ set_error("null context"); return EPHMC_ERR_ARG; }
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	for(auto &kv : c->timers) for(hipEvent_t e : kv.second.ev) hipEventDestroy(e);
+	c->timers.clear();
+	c->timing = enable != 0;
+	return EPHMC_OK;
+}
+
+int ephmc_timing_get(ephmc_ctx *c, const char *group, double *avg_ms, unsigned long *calls)
+{
+	if(!c || !group){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	auto it = c->timers.find(group);
+	if(it == c->timers.end()){ if(avg_ms) *avg_ms = 0; if(calls) *calls = 0; return EPHMC_OK; }
+	TimerGroup &g = it->second;
+	EPHMC_TRY(timer_harvest(g));
+	if(avg_ms) *avg_ms = g.calls ? g.total_ms / g.calls : 0;
+	if(calls) *calls = g.calls;
+	return EPHMC_OK;
+}
+
+} // extern "C"

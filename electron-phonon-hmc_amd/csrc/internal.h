@@ -1,0 +1,166 @@
+// internal.h -- context and kernel interfaces of libephmc (HIP, gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include <map>
+#include "../../include/ephmc.h"
+
+namespace ephmc {
+
+void set_error(const char *fmt, ...);
+
+#define EPHMC_HIP(call)                                                                      \
+	do {                                                                                     \
+		hipError_t e_ = (call);                                                              \
+		if(e_ != hipSuccess){                                                                \
+			ephmc::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+			return EPHMC_ERR_HIP;                                                            \
+		}                                                                                    \
+	} while(0)
+
+#define EPHMC_LAUNCHED()                                                                     \
+	do {                                                                                     \
+		hipError_t e_ = hipGetLastError();                                                   \
+		if(e_ != hipSuccess){                                                                \
+			ephmc::set_error("%s:%d launch: %s", __FILE__, __LINE__, hipGetErrorString(e_)); \
+			return EPHMC_ERR_HIP;                                                            \
+		}                                                                                    \
+	} while(0)
+
+#define EPHMC_TRY(expr) do { int r_ = (expr); if(r_ != EPHMC_OK) return r_; } while(0)
+
+constexpr int MAX_DEG = 8;      // polynomial degree of the slice exponential (2 * pairs)
+constexpr int MAX_PAIRS = 4;
+
+// Taylor polynomial of one slice: pair factors in application order and the monomial
+// coefficients of their product p(z) = sum_n c[n] z^n.
+struct Poly {
+	int np;
+	double a_lin[MAX_PAIRS], a_quad[MAX_PAIRS];
+	int cplx[MAX_PAIRS];
+	int deg;
+	double c[MAX_DEG + 1];
+};
+
+// Kernel-argument bundle of the slice stencil.
+template <typename R>
+struct Stencil {
+	const R *coef;        // [nt][6][ns] bond coefficients t_k - kappa_k x
+	const int *nb;        // [6][ns] neighbour sites
+	R mu;
+	unsigned ns;
+};
+
+// Kernel-group timing: event pairs recorded on the context stream, read back on demand.
+struct TimerGroup {
+	std::vector<hipEvent_t> ev;   // begin/end pairs
+	size_t used;                  // events recorded since the last harvest
+	bool open;
+	double total_ms;
+	unsigned long calls;
+};
+
+} // namespace ephmc
+
+struct ephmc_ctx {
+	ephmc_config cfg;
+	unsigned l1, l2, ns, nt, loc;     // loc = 3 ns links per slice
+	size_t dim, mat;                  // dim = nt*loc, mat = ns*ns
+	double traj_length;
+	ephmc::Poly poly;
+	hipStream_t stream;
+	bool own_stream;
+	int device;
+
+	std::vector<unsigned> nnt;        // host copy of the lattice table (12 ns)
+	int *d_nb;                        // [6][ns]
+	unsigned *d_link;                 // [6][ns]
+
+	// slice coefficients
+	double *d_coef64;                 // [nt][6][ns]
+	float *d_coef32;
+
+	// chain / force workspace
+	unsigned ckpt;                    // k
+	unsigned nseg;                    // ceil(nt / k)
+	unsigned cols;                    // columns per workgroup of the force sweep
+	unsigned nwg;                     // ceil(ns / cols)
+	void *d_ck;                       // checkpoints T_{s k}: nseg+1 matrices (R = fp64 or fp32)
+	void *d_seg;                      // per-workgroup segment scratch: k * ns * ns
+	double *d_partial;                // [k][nwg][3 ns] force partial sums
+	double *d_Tfin;                   // T_nt (fp64) or scratch for conversions
+	float *d_Tfin32;
+
+	// Gauss-Jordan
+	double *d_gj[2];
+	int *d_piv, *d_src;
+	double *d_diag;
+	double *d_scal;                   // device scalars (see SCAL_* below)
+
+	// bosons
+	double *d_twiddle;                // [nt/2] complex e^{-2 pi i k/nt}
+	double *d_f2;                     // [nt] freq2(n)
+	double *d_red;                    // reduction partials
+	unsigned red_blocks;
+
+	// chain state
+	double *d_x, *d_p, *d_pdot, *d_xnew, *d_minv, *d_mhat, *d_normals;
+	double *h_stage[2];               // pinned staging for normals
+	hipEvent_t stage_ev[2];
+	int stage_next;
+	bool chain_ready;
+
+	// timing
+	bool timing;
+	std::map<std::string, ephmc::TimerGroup> timers;
+};
+
+namespace ephmc {
+
+enum Scalar {            // layout of d_scal
+	SCAL_LOGDET = 0,      // log|det| of the last factorisation
+	SCAL_SIGN = 1,
+	SCAL_EP = 2,          // kinetic energy
+	SCAL_H_OLD = 3,
+	SCAL_H_NEW = 4,
+	SCAL_LOGDET_CHAIN = 5,// log det of the accepted state
+	SCAL_ACC = 6,
+	SCAL_BOLTZ = 7,
+	SCAL_ENERGY = 8,
+	SCAL_DH = 9,
+	SCAL_LOGDET_NEW = 10,
+	SCAL_BOSON = 11,      // S_B(x)
+	SCAL_COUNT = 16
+};
+
+// timing helpers
+int timer_begin(ephmc_ctx *c, const char *group);
+int timer_end(ephmc_ctx *c, const char *group);
+
+// chain.hip
+int coefficients(ephmc_ctx *c, const double *x, bool want32);
+int chain_forward(ephmc_ctx *c, int prec, const void *src, void *dst, void *ck, unsigned ck_every,
+		int descending, bool from_identity);
+int apply_exp_partial(ephmc_ctx *c, const double *x_tau, double *M, int revert, int stop);
+int transpose_plus_identity(ephmc_ctx *c, int prec, const void *T, double *M, int add_identity);
+int convert_f64_to_f32(ephmc_ctx *c, const double *a, float *b, size_t n);
+int add_identity(ephmc_ctx *c, double *M);
+
+// gj.hip
+int gj_logdet_inverse(ephmc_ctx *c, double *M);
+
+// adjoint.hip
+int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p);
+
+// bosons.hip
+int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick);
+int fa_filter(ephmc_ctx *c, const double *in, double *out, int inverse_sqrt);
+int fa_energy(ephmc_ctx *c, const double *p, double *scal_out);
+int boson_action(ephmc_ctx *c, const double *x, double *scal_out);
+int hamilton_dev(ephmc_ctx *c, const double *x, const double *p, int logdet_slot, int out_slot);
+int metropolis(ephmc_ctx *c, double uniform);
+
+} // namespace ephmc

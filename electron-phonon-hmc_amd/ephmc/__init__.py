@@ -1,0 +1,285 @@
+"""ctypes binding of libephmc.so (the C-ABI declared in include/ephmc.h).
+
+Host arrays are numpy float64; device buffers are DeviceArray objects allocated through the
+library's own ephmc_malloc.  Importing this module raises if the HIP library is missing: there
+is no CPU fallback.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "libephmc.so")
+
+FP64, MIXED = 0, 1
+ERRORS = {-1: "argument", -2: "HIP", -3: "unsupported", -4: "allocation", -5: "numeric"}
+
+
+class Config(C.Structure):
+    """ephmc_config (include/ephmc.h)."""
+    _fields_ = [("l1", C.c_uint), ("l2", C.c_uint), ("nt", C.c_uint),
+                ("t", C.c_double * 3), ("kappa", C.c_double * 3), ("mu", C.c_double), ("omega", C.c_double),
+                ("flavors", C.c_uint), ("steps", C.c_uint), ("traj_length", C.c_double),
+                ("taylor_order", C.c_int), ("precision", C.c_int), ("checkpoint", C.c_uint),
+                ("zero_filling", C.c_int), ("single_particle", C.c_int), ("constant_force", C.c_int),
+                ("no_fourier_acc", C.c_int), ("classical_fourier_acc", C.c_int),
+                ("cfa_steps", C.c_uint), ("cfa_mass", C.c_double),
+                ("device", C.c_int), ("stream", C.c_void_p)]
+
+
+class Record(C.Structure):
+    _fields_ = [("accepted", C.c_int), ("boltzmann", C.c_double), ("energy", C.c_double),
+                ("logdet", C.c_double), ("dH", C.c_double)]
+
+
+class EphmcError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise EphmcError("libephmc.so not built at %s (run __graft_entry__.build())" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    vp, dp = C.c_void_p, C.POINTER(C.c_double)
+    sig = {
+        "ephmc_create": [C.POINTER(Config), C.POINTER(C.c_void_p)],
+        "ephmc_destroy": [vp],
+        "ephmc_synchronize": [vp],
+        "ephmc_dims": [vp, C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_size_t)],
+        "ephmc_lattice": [vp, C.POINTER(C.c_uint)],
+        "ephmc_malloc": [C.c_size_t, C.POINTER(C.c_void_p)],
+        "ephmc_free": [vp],
+        "ephmc_memcpy_h2d": [vp, vp, vp, C.c_size_t],
+        "ephmc_memcpy_d2h": [vp, vp, vp, C.c_size_t],
+        "ephmc_apply_exp_M": [vp, vp, vp, C.c_int, C.c_int],
+        "ephmc_construct_M_hat": [vp, vp, vp],
+        "ephmc_logdet_inv": [vp, vp, dp, C.POINTER(C.c_int)],
+        "ephmc_force": [vp, vp, vp, C.c_double, vp],
+        "ephmc_sample_free_bosons": [vp, vp, vp],
+        "ephmc_sample_fourier_momenta": [vp, vp, vp],
+        "ephmc_energy_momenta": [vp, vp, dp],
+        "ephmc_evolve_bosons": [vp, vp, vp, C.c_double],
+        "ephmc_hamilton": [vp, vp, C.c_double, C.c_double, dp],
+        "ephmc_leap_frog": [vp, vp, vp],
+        "ephmc_chain_init": [vp, dp],
+        "ephmc_trajectory": [vp, dp, C.c_double, C.POINTER(Record)],
+        "ephmc_last_record": [vp, C.POINTER(Record)],
+        "ephmc_chain_get_x": [vp, dp],
+        "ephmc_chain_set_x": [vp, dp],
+        "ephmc_chain_get_minv": [vp, dp],
+        "ephmc_chain_get_logdet": [vp, dp],
+        "ephmc_chain_buffers": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)],
+        "ephmc_tl_hist": [vp, dp, C.c_uint, dp, C.c_double, C.c_uint, dp, dp],
+        "ephmc_timing": [vp, C.c_int],
+        "ephmc_timing_get": [vp, C.c_char_p, dp, C.POINTER(C.c_ulong)],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.ephmc_last_error.restype = C.c_char_p
+    lib.ephmc_last_error.argtypes = []
+    return lib
+
+
+LIB = _load()
+
+
+def check(rc):
+    if rc != 0:
+        raise EphmcError("ephmc error %d (%s): %s" % (rc, ERRORS.get(rc, "?"), LIB.ephmc_last_error().decode()))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class DeviceArray:
+    """A float64 buffer in HBM allocated by the library."""
+
+    def __init__(self, ctx, n):
+        self.ctx, self.n = ctx, int(n)
+        p = C.c_void_p()
+        check(LIB.ephmc_malloc(max(self.n, 1) * 8, C.byref(p)))
+        self.ptr = p
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        assert host.size == self.n
+        check(LIB.ephmc_memcpy_h2d(self.ctx.h, self.ptr, host.ctypes.data_as(C.c_void_p), self.n * 8))
+        return self
+
+    def download(self):
+        out = np.empty(self.n)
+        check(LIB.ephmc_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, self.n * 8))
+        return out
+
+    def free(self):
+        if self.ptr:
+            LIB.ephmc_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def scale_params(beta, J, lam, mu0, omega0, nt, static_disorder=False):
+    """Lattice-unit couplings of organic_main.c:99-116 (lambda relative to J)."""
+    x_avg = 1.0 if static_disorder else math.sqrt(.5 / math.tanh(.5 * omega0 / 0.025))
+    scale = -beta if static_disorder else beta / nt
+    mu = mu0 * scale
+    t = [j * scale for j in J]
+    kappa = [l * tk / x_avg for l, tk in zip(lam, t)]
+    mu -= sum(2 * abs(tk) for tk in t)
+    return t, kappa, mu, omega0 * scale
+
+
+class Context:
+    """One chain on one GPU (ephmc_ctx)."""
+
+    def __init__(self, l1, l2, nt, t, kappa, mu, omega, flavors=2, steps=10, traj_length=0.0,
+                 precision=FP64, checkpoint=16, taylor_order=4, device=0, **flags):
+        cfg = Config()
+        cfg.l1, cfg.l2, cfg.nt = l1, l2, nt
+        cfg.t[:] = list(t)
+        cfg.kappa[:] = list(kappa)
+        cfg.mu, cfg.omega, cfg.flavors = mu, omega, flavors
+        cfg.steps, cfg.traj_length = steps, traj_length
+        cfg.taylor_order, cfg.precision, cfg.checkpoint = taylor_order, precision, checkpoint
+        for k, v in flags.items():
+            setattr(cfg, k, v)
+        cfg.device = device
+        h = C.c_void_p()
+        check(LIB.ephmc_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.ns, self.nt = l1 * l2, nt
+        self.dim = nt * 3 * self.ns
+        self.mat = self.ns * self.ns
+
+    def close(self):
+        if self.h:
+            LIB.ephmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def array(self, n, host=None):
+        a = DeviceArray(self, n)
+        if host is not None:
+            a.upload(host)
+        return a
+
+    def sync(self):
+        check(LIB.ephmc_synchronize(self.h))
+
+    def lattice(self):
+        out = np.zeros(12 * self.ns, np.uint32)
+        check(LIB.ephmc_lattice(self.h, out.ctypes.data_as(C.POINTER(C.c_uint))))
+        return out
+
+    # -- host-array conveniences (each uploads, runs, downloads) ------------------------
+    def M_hat(self, x):
+        dx, dm = self.array(self.dim, x), self.array(self.mat)
+        check(LIB.ephmc_construct_M_hat(self.h, dx.ptr, dm.ptr))
+        return dm.download()
+
+    def logdet_inv(self, M):
+        dm = self.array(self.mat, M)
+        ld, sg = C.c_double(), C.c_int()
+        check(LIB.ephmc_logdet_inv(self.h, dm.ptr, C.byref(ld), C.byref(sg)))
+        return ld.value, sg.value, dm.download()
+
+    def force(self, x, Minv, logdet=0.0):
+        dx, dm, dp_ = self.array(self.dim, x), self.array(self.mat, Minv), self.array(self.dim)
+        check(LIB.ephmc_force(self.h, dx.ptr, dm.ptr, logdet, dp_.ptr))
+        return dp_.download()
+
+    def apply_exp_M(self, x_tau, M, revert, stop):
+        dx, dm = self.array(3 * self.ns, x_tau), self.array(self.mat, M)
+        check(LIB.ephmc_apply_exp_M(self.h, dx.ptr, dm.ptr, revert, stop))
+        return dm.download()
+
+    def sample_free_bosons(self, normals):
+        dn, dx = self.array(self.dim, normals), self.array(self.dim)
+        check(LIB.ephmc_sample_free_bosons(self.h, dn.ptr, dx.ptr))
+        return dx.download()
+
+    def sample_fourier_momenta(self, normals):
+        dn, dp_ = self.array(self.dim, normals), self.array(self.dim)
+        check(LIB.ephmc_sample_fourier_momenta(self.h, dn.ptr, dp_.ptr))
+        return dp_.download()
+
+    def energy_momenta(self, p):
+        dp_ = self.array(self.dim, p)
+        e = C.c_double()
+        check(LIB.ephmc_energy_momenta(self.h, dp_.ptr, C.byref(e)))
+        return e.value
+
+    def evolve_bosons(self, x, p, h):
+        dx, dp_ = self.array(self.dim, x), self.array(self.dim, p)
+        check(LIB.ephmc_evolve_bosons(self.h, dx.ptr, dp_.ptr, h))
+        return dx.download(), dp_.download()
+
+    def hamilton(self, x, logdet, energyP):
+        dx = self.array(self.dim, x)
+        H = C.c_double()
+        check(LIB.ephmc_hamilton(self.h, dx.ptr, logdet, energyP, C.byref(H)))
+        return H.value
+
+    def leap_frog(self, x, p):
+        dx, dp_ = self.array(self.dim, x), self.array(self.dim, p)
+        check(LIB.ephmc_leap_frog(self.h, dx.ptr, dp_.ptr))
+        return dx.download(), dp_.download()
+
+    # -- chain ------------------------------------------------------------------------
+    def chain_init(self, normals):
+        normals = np.ascontiguousarray(normals, np.float64)
+        check(LIB.ephmc_chain_init(self.h, _dp(normals)))
+
+    def trajectory(self, normals, uniform, wait=True):
+        normals = np.ascontiguousarray(normals, np.float64)
+        rec = Record()
+        check(LIB.ephmc_trajectory(self.h, _dp(normals), uniform, C.byref(rec) if wait else None))
+        return rec if wait else None
+
+    def last_record(self):
+        rec = Record()
+        check(LIB.ephmc_last_record(self.h, C.byref(rec)))
+        return rec
+
+    def chain_x(self):
+        out = np.empty(self.dim)
+        check(LIB.ephmc_chain_get_x(self.h, _dp(out)))
+        return out
+
+    def chain_set_x(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        check(LIB.ephmc_chain_set_x(self.h, _dp(x)))
+
+    def chain_minv(self):
+        out = np.empty(self.mat)
+        check(LIB.ephmc_chain_get_minv(self.h, _dp(out)))
+        return out
+
+    def chain_logdet(self):
+        v = C.c_double()
+        check(LIB.ephmc_chain_get_logdet(self.h, C.byref(v)))
+        return v.value
+
+    def timing(self, enable=True):
+        check(LIB.ephmc_timing(self.h, int(enable)))
+
+    def timing_get(self, group):
+        ms, calls = C.c_double(), C.c_ulong()
+        check(LIB.ephmc_timing_get(self.h, group.encode(), C.byref(ms), C.byref(calls)))
+        return ms.value, calls.value

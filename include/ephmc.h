@@ -1,0 +1,160 @@
+/*
+ * ephmc.h -- C-ABI of the MI355X electron-phonon HMC engine (libephmc.so).
+ *
+ * Plain C: opaque context, plain pointers and sizes, int status codes.  Every pointer
+ * argument named *_dev is a device (HBM) pointer; every *_host pointer is host memory.
+ * Matrices are N x N column-major, i.e. the reference's row-major chunks read as LAPACK
+ * columns (organic_dense_matrix.c:580).  Fields x, p, p_dot use the reference layout
+ * x[tau*3N + site*3 + direction] (organic_hmc.c:195-197).
+ *
+ * One context = one chain on one GPU, bound to one HIP stream.  Calls on a context are
+ * asynchronous on its stream unless they return a host value (documented per call).
+ * Contexts are independent; one context must not be used from two threads at once.
+ *
+ * Each entry point names the reference function it replaces (file:line in
+ * j-ostmeyer/electron-phonon-hmc, directory simulation/).
+ */
+#ifndef EPHMC_H
+#define EPHMC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EPHMC_OK              0
+#define EPHMC_ERR_ARG        -1   /* invalid argument or size                      */
+#define EPHMC_ERR_HIP        -2   /* HIP runtime error (message in ephmc_last_error) */
+#define EPHMC_ERR_UNSUPPORTED -3  /* flag combination not provided by this build    */
+#define EPHMC_ERR_ALLOC      -4   /* device allocation failed                       */
+#define EPHMC_ERR_NUMERIC    -5   /* singular matrix (zero pivot)                   */
+
+#define EPHMC_FP64  0   /* plain FP64 drop-in path                                     */
+#define EPHMC_MIXED 1   /* FP32 slice products and force in the molecular dynamics;
+                           FP64 factorisations, log-det and dH of the Metropolis step */
+
+typedef struct ephmc_ctx ephmc_ctx;
+
+/* Model and run configuration.  Couplings are in lattice units, i.e. after the scaling of
+ * organic_main.c:99-116 (t *= beta/Nt, kappa = lambda*t/x_avg, mu = mu0*beta/Nt - 2 sum|t|,
+ * omega *= beta/Nt).  Flags mirror organic_flags (organic_flags.h:4-14). */
+typedef struct ephmc_config {
+	unsigned l1, l2, nt;
+	double t[3], kappa[3], mu, omega;
+	unsigned flavors;
+	unsigned steps;          /* molecular-dynamics steps per trajectory (Nmd)            */
+	double traj_length;      /* 0 selects pi/2 (organic_main.c:114-116)                  */
+	int taylor_order;        /* 2, 4 (reference default), 6 or 8                          */
+	int precision;           /* EPHMC_FP64 or EPHMC_MIXED                                */
+	unsigned checkpoint;     /* slices between stored partial products in the force (k) */
+	int zero_filling, single_particle, constant_force, no_fourier_acc, classical_fourier_acc;
+	unsigned cfa_steps;
+	double cfa_mass;
+	int device;              /* HIP device ordinal                                       */
+	void *stream;            /* hipStream_t to run on; NULL creates a default-priority one */
+} ephmc_config;
+
+/* Per-trajectory record (organic_hmc.c:233-245, results columns 3,4,7,8). */
+typedef struct ephmc_traj_record {
+	int accepted;
+	double boltzmann;        /* exp(H_old - H_new)                      */
+	double energy;           /* H after the Metropolis decision         */
+	double logdet;           /* log det(1 + B..B) after the decision    */
+	double dH;               /* H_new - H_old of the proposal           */
+} ephmc_traj_record;
+
+/* ---- lifecycle ---------------------------------------------------------------------- */
+int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out);
+int ephmc_destroy(ephmc_ctx *ctx);
+int ephmc_synchronize(ephmc_ctx *ctx);
+const char *ephmc_last_error(void);
+int ephmc_dims(const ephmc_ctx *ctx, unsigned *ns, unsigned *nt, size_t *dim);
+/* neighbour table nnt[12N] of construct_triangular (organic_lattice.c:10-46) */
+int ephmc_lattice(const ephmc_ctx *ctx, unsigned *nnt_host);
+
+/* ---- device memory helpers (for hosts that do not include HIP headers) --------------- */
+int ephmc_malloc(size_t bytes, void **dev);
+int ephmc_free(void *dev);
+int ephmc_memcpy_h2d(ephmc_ctx *ctx, void *dev, const void *host, size_t bytes);  /* synchronous */
+int ephmc_memcpy_d2h(ephmc_ctx *ctx, void *host, const void *dev, size_t bytes);  /* synchronous */
+
+/* ---- (a) slice exponential ----------------------------------------------------------- */
+/* M <- p(A_tau) M with the pair factors [0, stop) (revert = 0) or np-1 down to stop+1
+ * (revert != 0); replaces apply_exp_mu_K_to_M (organic_dense_matrix.c:209-216).
+ * x_tau_dev: 3N couplings of one slice; M_dev: N x N, FP64. */
+int ephmc_apply_exp_M(ephmc_ctx *ctx, const double *x_tau_dev, double *M_dev, int revert, int stop);
+
+/* ---- (b) chained slice product -------------------------------------------------------- */
+/* M_hat = 1 + B_0 B_1 ... B_{Nt-1} (no +1 for ZERO_FILLING / SINGLE_PARTICLE);
+ * replaces construct_M_hat (organic_dense_matrix.c:251-265).  FP64. */
+int ephmc_construct_M_hat(ephmc_ctx *ctx, const double *x_dev, double *M_hat_dev);
+
+/* ---- (c) log-det, inverse, force ------------------------------------------------------ */
+/* log|det M_hat| and M_hat <- M_hat^{-1} in place (FP64, partial pivoting); replaces
+ * logDetM_and_inv_M_hat (organic_dense_matrix.c:547-607).  logdet_host / sign_host may be
+ * NULL; when given, the call waits for the result. */
+int ephmc_logdet_inv(ephmc_ctx *ctx, double *M_hat_dev, double *logdet_host, int *sign_host);
+/* p_dot = fermion force for field x given M_inv = (1 + B..B)^{-1}; replaces force
+ * (organic_hmc.c:86-134) with construct_all_QM1P + refine_QM1P
+ * (organic_dense_matrix.c:311-442) computed as an adjoint sweep.  Uses the context's
+ * precision; logdet is used by SINGLE_PARTICLE only. */
+int ephmc_force(ephmc_ctx *ctx, const double *x_dev, const double *M_inv_dev, double logdet, double *p_dot_dev);
+
+/* ---- (d) Fourier-accelerated bosons ---------------------------------------------------- */
+/* x = free-boson sample coloured from dim normals; replaces sample_free_bosons
+ * (organic_bosons.c:55-74) after its random_vector draw. */
+int ephmc_sample_free_bosons(ephmc_ctx *ctx, const double *normals_dev, double *x_dev);
+/* p = Fourier-accelerated momenta coloured from dim normals; replaces
+ * sample_fourier_momenta (organic_bosons.c:76-100) after its random_vector draw. */
+int ephmc_sample_fourier_momenta(ephmc_ctx *ctx, const double *normals_dev, double *p_dev);
+/* kinetic energy of p; replaces energy_momenta / energy_fourier_momenta
+ * (organic_hmc.c:56-65, organic_bosons.c:102-124).  Waits for the result. */
+int ephmc_energy_momenta(ephmc_ctx *ctx, const double *p_dev, double *energy_host);
+/* exact harmonic flow of every Matsubara mode for time h; replaces evolve_bosons
+ * (organic_bosons.c:126-150) */
+int ephmc_evolve_bosons(ephmc_ctx *ctx, double *x_dev, double *p_dev, double h);
+/* H = -flavors logdet + energyP + S_B(x); replaces hamilton (organic_hmc.c:67-84).  Waits. */
+int ephmc_hamilton(ephmc_ctx *ctx, const double *x_dev, double logdet, double energyP, double *H_host);
+/* position-Verlet leapfrog of Nmd steps on x, p (the context's precision); replaces
+ * leap_frog (organic_hmc.c:167-190) */
+int ephmc_leap_frog(ephmc_ctx *ctx, double *x_dev, double *p_dev);
+
+/* ---- HMC chain with device-resident state ---------------------------------------------- */
+/* Initial field and M_hat^{-1}: replaces run_hmc's start (organic_hmc.c:325-335).
+ * normals_host: dim normals of the free-boson draw. */
+int ephmc_chain_init(ephmc_ctx *ctx, const double *normals_host);
+/* One trajectory (organic_hmc.c:192-243): momenta from dim normals, leapfrog, FP64
+ * M_hat / log-det / H, Metropolis with the given uniform (the reference's genrand64_real2
+ * draw that follows the normals).  Asynchronous when rec == NULL; with rec the call waits. */
+int ephmc_trajectory(ephmc_ctx *ctx, const double *normals_host, double uniform, ephmc_traj_record *rec);
+/* Record of the last trajectory (waits). */
+int ephmc_last_record(ephmc_ctx *ctx, ephmc_traj_record *rec);
+/* Copies of the chain state (wait): x (dim), M_hat^{-1} (N*N). */
+int ephmc_chain_get_x(ephmc_ctx *ctx, double *x_host);
+int ephmc_chain_set_x(ephmc_ctx *ctx, const double *x_host);   /* also rebuilds M_hat^{-1} */
+int ephmc_chain_get_minv(ephmc_ctx *ctx, double *minv_host);
+int ephmc_chain_get_logdet(ephmc_ctx *ctx, double *logdet_host);
+/* device pointers of the chain state (valid until ephmc_destroy) */
+int ephmc_chain_buffers(ephmc_ctx *ctx, double **x_dev, double **p_dev, double **minv_dev);
+
+/* ---- (e) transient localisation ---------------------------------------------------------- */
+/* Batched disorder samples; replaces hist_trans_loc x meas (organic_trans_loc.c:25-83,
+ * :101-105) with the normals of each sample given (3N per sample, sample-major).
+ * greens_host[nbins] is accumulated (+=, not yet divided by the sample count);
+ * results_host[nsamples] receives sum exp(-E)/N per sample.  contr_mat: 2 x 3. */
+int ephmc_tl_hist(ephmc_ctx *ctx, const double *normals_host, unsigned nsamples,
+		const double *contr_mat_host, double width, unsigned nbins,
+		double *greens_host, double *results_host);
+
+/* ---- diagnostics ---------------------------------------------------------------------------- */
+/* Average duration (ms, HIP events on the context's stream) of the named kernel group since
+ * the last reset; groups: "chain", "gj", "adjoint", "reduce", "fft", "tl".  Timing is only
+ * recorded after ephmc_timing(ctx, 1). */
+int ephmc_timing(ephmc_ctx *ctx, int enable);
+int ephmc_timing_get(ephmc_ctx *ctx, const char *group, double *avg_ms, unsigned long *calls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

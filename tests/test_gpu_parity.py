@@ -1,0 +1,206 @@
+"""Parity of the HIP engine with the oracle (same seeded inputs, sizes the oracle finishes in
+seconds) and, at the L=32, Nt=128 size, with closed forms.  Tolerances: FP64 path 1e-10
+relative (log-det, force, M_hat), FP32-mixed force 1e-5 relative."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import _dense as D
+from _gpu import gpu_context, rel, say
+from _oracle import MT, OracleModel, oracle, ptr
+
+pytestmark = pytest.mark.gpu
+
+
+def field(om, seed, scale=1.0):
+    return np.random.default_rng(seed).standard_normal(om.dim) * scale
+
+
+@pytest.mark.parametrize("l1,l2,nt", [(1, 1, 4), (2, 2, 4), (2, 3, 8), (6, 6, 32), (8, 12, 16), (16, 16, 8)])
+def test_M_hat(l1, l2, nt):
+    say("M_hat", l1, l2, nt)
+    om = OracleModel(l1, l2, nt, beta=nt * 40 / 32)
+    om.set_couplings(kappa=om.kappa * 10)
+    ctx = gpu_context(om)
+    np.testing.assert_array_equal(ctx.lattice(), om.nnt)
+    x = field(om, nt)
+    Mg = ctx.M_hat(x)
+    Mo = om.M_hat(x)
+    assert rel(Mg, Mo) < 1e-13
+
+
+@pytest.mark.parametrize("revert,stop", [(0, 2), (0, 1), (0, 0), (1, 0), (1, -1)])
+def test_apply_exp_partial(revert, stop):
+    say("apply_exp_M", revert, stop)
+    om = OracleModel(5, 4, 2)
+    om.set_couplings(kappa=om.kappa * 10)
+    ctx = gpu_context(om)
+    x = field(om, 3)
+    n = om.ns
+    M = np.random.default_rng(1).standard_normal(n * n)
+    g = ctx.apply_exp_M(x[:3 * n], M, revert, stop)
+    o = M.copy()
+    w = np.zeros(2 * n * n)
+    oracle().orc_apply_exp_mu_K_to_M(om.ref, ptr(x), ptr(o), ptr(w), revert, stop)
+    assert rel(g, o) < 1e-14
+
+
+@pytest.mark.parametrize("n", [1, 5, 36, 100, 256, 300, 1024])
+def test_logdet_inverse_random(n):
+    say("gj random", n)
+    om = OracleModel(n, 1, 1)
+    ctx = gpu_context(om)
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + np.eye(n) * (0 if n < 50 else 3)
+    ld, sg, inv = ctx.logdet_inv(A.T.copy().ravel())
+    s0, l0 = np.linalg.slogdet(A)
+    assert ld == pytest.approx(l0, rel=1e-11, abs=1e-11) and sg == int(s0)
+    assert rel(inv.reshape(n, n).T, np.linalg.inv(A)) < 1e-10
+
+
+@pytest.mark.parametrize("l1,l2,nt", [(6, 6, 32), (16, 16, 64)])
+def test_logdet_inverse_M_hat(l1, l2, nt):
+    say("gj M_hat", l1, l2, nt)
+    om = OracleModel(l1, l2, nt)
+    ctx = gpu_context(om)
+    x = field(om, 5)
+    M = om.M_hat(x)
+    ldo, so, invo = om.logdet_inv(M)
+    ldg, sg, invg = ctx.logdet_inv(M)
+    assert ldg == pytest.approx(ldo, rel=1e-12) and sg == so
+    assert rel(invg, invo) < 1e-12
+
+
+@pytest.mark.parametrize("l1,l2,nt,k", [(3, 3, 4, 16), (6, 6, 32, 16), (6, 6, 32, 5), (6, 6, 32, 1), (5, 7, 12, 4),
+                                        (2, 2, 8, 3), (16, 16, 16, 8)])
+def test_force_fp64(l1, l2, nt, k):
+    say("force fp64", l1, l2, nt, k)
+    om = OracleModel(l1, l2, nt, beta=nt * 40 / 32)
+    om.set_couplings(kappa=om.kappa * 10)
+    ctx = gpu_context(om, checkpoint=k)
+    x = field(om, l1 + nt)
+    ld, _, Minv = om.logdet_inv(om.M_hat(x))
+    Fo = om.force(x, Minv, ld)
+    Fg = ctx.force(x, Minv, ld)
+    assert rel(Fg, Fo) < 1e-11
+
+
+def test_force_mixed():
+    say("force mixed 6x6x32")
+    om = OracleModel(6, 6, 32)
+    ctx = gpu_context(om, precision=1)
+    x = field(om, 2)
+    ld, _, Minv = om.logdet_inv(om.M_hat(x))
+    Fo = om.force(x, Minv, ld)
+    Fg = ctx.force(x, Minv, ld)
+    say("mixed force rel err", rel(Fg, Fo))
+    assert rel(Fg, Fo) < 1e-5
+
+
+@pytest.mark.parametrize("nt", [1, 2, 8, 12, 32, 48])
+def test_boson_kernels(nt):
+    say("bosons", nt)
+    om = OracleModel(3, 2, nt)
+    ctx = gpu_context(om)
+    lib = oracle()
+    rng = np.random.default_rng(nt)
+    x = rng.standard_normal(om.dim)
+    p = rng.standard_normal(om.dim)
+    xg, pg = ctx.evolve_bosons(x, p, 0.41)
+    xo, po = x.copy(), p.copy()
+    lib.orc_evolve_bosons(om.ref, ptr(xo), ptr(po), 0.41)
+    assert rel(xg, xo) < 1e-13 and rel(pg, po) < 1e-13
+    assert ctx.energy_momenta(p) == pytest.approx(lib.orc_energy_momenta(om.ref, ptr(p)), rel=1e-13)
+    assert ctx.hamilton(x, 1.25, 3.5) == pytest.approx(lib.orc_hamilton(om.ref, ptr(x), 1.25, 3.5), rel=1e-13)
+    for fn_g, fn_o in ((ctx.sample_fourier_momenta, lib.orc_sample_fourier_momenta),
+                       (ctx.sample_free_bosons, lib.orc_sample_free_bosons)):
+        st, st2 = MT(), MT()
+        lib.orc_mt_seed(C.byref(st), 77)
+        lib.orc_mt_seed(C.byref(st2), 77)
+        normals = np.zeros(om.dim)
+        lib.orc_random_vector(C.byref(st), ptr(normals), om.dim)
+        out_o = np.zeros(om.dim)
+        fn_o(om.ref, C.byref(st2), ptr(out_o))
+        assert rel(fn_g(normals), out_o) < 1e-13
+
+
+def test_hmc_accept_sequence_and_logdet():
+    """Seeded FP64 chain at L=6, Nt=32, Nmd=10: identical accept/reject decisions and log-det
+    within 1e-10 relative over the first 100 trajectories (oracle with the same RNG stream)."""
+    om = OracleModel(6, 6, 32)
+    lib = oracle()
+    from _oracle import HMC
+    h = HMC()
+    assert lib.orc_hmc_init(C.byref(h), C.byref(om.m), 1, 10, 0.0, None) == 0
+    ctx = gpu_context(om, steps=10)
+    # same RNG stream: the chain's free-boson draw, then per trajectory dim normals + 1 uniform
+    rng = MT()
+    lib.orc_mt_seed(C.byref(rng), 1)
+    normals = np.zeros(om.dim)
+    lib.orc_random_vector(C.byref(rng), ptr(normals), om.dim)
+    ctx.chain_init(normals)
+    assert ctx.chain_logdet() == pytest.approx(h.logdet, rel=1e-12)
+    dH = C.c_double()
+    worst = 0.0
+    for it in range(100):
+        lib.orc_random_vector(C.byref(rng), ptr(normals), om.dim)
+        u = lib.orc_mt_real2(C.byref(rng))
+        acc_o = lib.orc_trajectory(C.byref(h), None, None, C.byref(dH), None, None)
+        rec = ctx.trajectory(normals, u)
+        worst = max(worst, abs(rec.logdet - h.logdet) / abs(h.logdet))
+        assert rec.accepted == acc_o, "trajectory %d: gpu %d oracle %d (dH gpu %g oracle %g)" % (
+            it, rec.accepted, acc_o, rec.dH, dH.value)
+        assert abs(rec.dH - dH.value) < 1e-8
+    say("100 trajectories, worst logdet rel diff", worst)
+    assert worst < 1e-10
+    lib.orc_hmc_free(C.byref(h))
+
+
+# ---- full-size (L = 32, Nt = 128) properties -----------------------------------------------
+
+def test_full_size_logdet_closed_form():
+    say("closed-form logdet 32x32x128")
+    om = OracleModel(32, 32, 128)
+    ctx = gpu_context(om)
+    u = np.random.default_rng(3).standard_normal((128, 3))
+    x = D.uniform_field(32, 32, 128, u)
+    ld, sg, _ = ctx.logdet_inv(ctx.M_hat(x))
+    ref = D.closed_form_logdet(32, 32, 128, om.t, om.kappa, om.mu, u)
+    say("logdet", ld, "closed form", ref)
+    assert ld == pytest.approx(ref, rel=1e-10) and sg == 1
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-5)])
+def test_full_size_force_closed_form(precision, tol):
+    say("closed-form force 32x32x128 precision", precision)
+    om = OracleModel(32, 32, 128)
+    ctx = gpu_context(om, precision=precision, checkpoint=16)
+    u = np.random.default_rng(4).standard_normal((128, 3)) * 2
+    x = D.uniform_field(32, 32, 128, u)
+    ld, _, minv = ctx.logdet_inv(ctx.M_hat(x))
+    F = ctx.force(x, minv, ld)
+    ref = D.closed_form_force(32, 32, 128, om.t, om.kappa, om.mu, u, om.flavors)
+    say("force rel err", rel(F, ref))
+    assert rel(F, ref) < tol
+
+
+def test_full_size_force_finite_differences():
+    """Adjoint force vs central differences (eps = 1e-5, FP64) on 64 random components."""
+    om = OracleModel(32, 32, 128)
+    ctx = gpu_context(om, checkpoint=16)
+    x = field(om, 9)
+    ld, _, minv = ctx.logdet_inv(ctx.M_hat(x))
+    F = ctx.force(x, minv, ld)
+    idx = np.random.default_rng(64).choice(om.dim, 64, replace=False)
+    fd = []
+    for i in idx:
+        xp, xm = x.copy(), x.copy()
+        xp[i] += 1e-5
+        xm[i] -= 1e-5
+        lp = ctx.logdet_inv(ctx.M_hat(xp))[0]
+        lm = ctx.logdet_inv(ctx.M_hat(xm))[0]
+        fd.append(om.flavors * (lp - lm) / 2e-5)
+    err = rel(F[idx], np.array(fd))
+    say("finite-difference rel err", err)
+    assert err < 1e-5
