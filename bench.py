@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""HMC trajectories/s at L1=L2=32 (N=1024), Nt=128 on one MI355X: mixed precision vs the plain
+FP64 path, with the force, dH and log-det errors of the mixed path against FP64.
+
+Workload (one "step" = one HMC trajectory): triangular lattice 32x32, Nt=128, beta=40/eV,
+J=(0.10,-0.03,0.02) eV, lambda=0.3 J (input line 0.3 0.3 0.3), mu0=0, omega0=0.006 eV, Nmd=10,
+trajectory length pi/2 (Fourier acceleration default), 2 flavours, partial products
+checkpointed every 16 slices, initial field = free-boson draw with seed 3.  Momentum normals and
+accept uniforms are synthetic (numpy, seed 3) and resident in host memory; the chain state
+(field, M_hat^{-1}) lives in HBM.
+
+Prints one JSON line (rank 0, one GPU).
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "electron-phonon-hmc_amd"))
+
+L, NT, NMD, K_CKPT, SEED = 32, 128, 10, 16, 3
+BETA, J, LAM, MU0, OMEGA0, FLAVORS = 40.0, (0.10, -0.03, 0.02), (0.3, 0.3, 0.3), 0.0, 0.006, 2
+PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix), FLOP/s
+PEAK_FP32 = 157.3e12     # MI355X FP32 published dense peak
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def stencil_flops_adjoint_per_launch(ns, k):
+    """Algorithmic FP operations of one adjoint launch (one segment of k slices, all columns),
+    recomputation from the checkpoint excluded: per slice and column site
+    3 stencils for A^b T, 4 for A^a W (13 flops each), 16 for the S_a sums, 8 for W_{tau-1},
+    48 for the half-bond sums."""
+    return (3 * 13 + 4 * 13 + 16 + 8 + 48) * k * ns * ns
+
+
+def chain_flops(ns, nt):
+    """60 flops per site, column and slice (two Taylor pairs: 4 stencils + 2 combines)."""
+    return 60 * ns * ns * nt
+
+
+def run_chain(E, precision, normals_list, uniforms, x0_normals, warmup, steps):
+    t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, NT)
+    ctx = E.Context(L, L, NT, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=precision,
+                    checkpoint=K_CKPT)
+    ctx.chain_init(x0_normals)
+    for i in range(warmup):
+        ctx.trajectory(normals_list[i % len(normals_list)], uniforms[i % len(uniforms)], wait=False)
+    ctx.sync()
+    ctx.timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        j = (warmup + i) % len(normals_list)
+        ctx.trajectory(normals_list[j], uniforms[j], wait=False)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    groups = {g: ctx.timing_get(g) for g in ("chain", "gj", "adjoint", "adjoint_kernel", "force_reduce", "fft",
+                                             "metropolis")}
+    ctx.timing(False)
+    acc = ctx.last_record()
+    return ctx, el, groups, acc
+
+
+def accuracy(E, x_host, normals, uniform):
+    """Mixed vs FP64 on the same field and the same trajectory inputs."""
+    t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, NT)
+    out = {}
+    ctxs = {}
+    for name, prec in (("fp64", E.FP64), ("mixed", E.MIXED)):
+        c = E.Context(L, L, NT, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=prec, checkpoint=K_CKPT)
+        c.chain_set_x(x_host)
+        ctxs[name] = c
+    minv = ctxs["fp64"].chain_minv()
+    ld = ctxs["fp64"].chain_logdet()
+    F64 = ctxs["fp64"].force(x_host, minv, ld)
+    F32 = ctxs["mixed"].force(x_host, minv, ld)
+    out["force_rel_err"] = float(np.linalg.norm(F32 - F64) / np.linalg.norm(F64))
+    out["logdet_rel_err"] = abs(ctxs["mixed"].chain_logdet() - ld) / abs(ld)
+    r64 = ctxs["fp64"].trajectory(normals, uniform)
+    r32 = ctxs["mixed"].trajectory(normals, uniform)
+    out["dH_fp64"] = r64.dH
+    out["dH_mixed"] = r32.dH
+    out["dH_abs_err"] = abs(r32.dH - r64.dH)
+    out["same_decision"] = bool(r64.accepted == r32.accepted)
+    for c in ctxs.values():
+        c.close()
+    return out
+
+
+def cpu_baseline(sample_threads):
+    """The oracle (CPU restatement of the reference algorithm, C/OpenMP) timed on this host:
+    one M_hat + LU log-det/inverse and one force at the full size, extrapolated to one
+    trajectory as (Nmd+1) (M_hat + LU) + Nmd force (boson FFT work is below 1%)."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    os.environ["OMP_NUM_THREADS"] = str(sample_threads)
+    from _oracle import OracleModel
+    om = OracleModel(L, L, NT)
+    x = np.random.default_rng(SEED).standard_normal(om.dim) * 0.5
+    t0 = time.perf_counter()
+    M = om.M_hat(x)
+    ld, _, minv = om.logdet_inv(M)
+    t_mlu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    om.force(x, minv, ld)
+    t_force = time.perf_counter() - t0
+    t_traj = (NMD + 1) * t_mlu + NMD * t_force
+    return {"value": 1.0 / t_traj, "unit": "traj/s", "cores": sample_threads, "kind": "port",
+            "sample": "one M_hat+LU (%.2f s) and one force (%.2f s) at 32x32, Nt=128 on %d OpenMP threads, "
+                      "extrapolated to (Nmd+1)(M_hat+LU)+Nmd force = %.1f s per trajectory; CPU %s"
+                      % (t_mlu, t_force, sample_threads, t_traj, platform.processor() or platform.machine())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; a single chain does not span GPUs here "
+              "(independent chains and disorder samples shard trivially, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    import ephmc as E
+    rng = np.random.default_rng(SEED)
+    dim = NT * 3 * L * L
+    nbuf = min(args.steps + args.warmup, 8)
+    normals_list = [rng.standard_normal(dim) for _ in range(nbuf)]
+    uniforms = rng.random(nbuf)
+    x0 = rng.standard_normal(dim)
+
+    res = {}
+    for name, prec in (("fp64", E.FP64), ("mixed", E.MIXED)):
+        ctx, el, groups, rec = run_chain(E, prec, normals_list, uniforms, x0, args.warmup, args.steps)
+        res[name] = {"el": el, "groups": groups, "ctx": ctx}
+        log(name, "%.2f ms/traj" % (1e3 * el / args.steps), {g: "%.3f ms x %d" % v for g, v in groups.items()})
+    x_state = res["fp64"]["ctx"].chain_x()
+    for r in res.values():
+        r["ctx"].close()
+    acc = accuracy(E, x_state, normals_list[0], float(uniforms[0]))
+    log("accuracy", acc)
+
+    el64, elmx = res["fp64"]["el"], res["mixed"]["el"]
+    tps64, tpsmx = args.steps / el64, args.steps / elmx
+    # dominant kernel of the measured (mixed) path: the adjoint force sweep, one launch per
+    # checkpoint segment; HIP-event average over the timed region
+    per_launch_ms, adj_calls = res["mixed"]["groups"]["adjoint_kernel"]
+    flops = stencil_flops_adjoint_per_launch(L * L, K_CKPT)
+    achieved = flops / (per_launch_ms * 1e-3)
+    roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32, "traffic": None,
+            "kernel": "adjoint_kernel<float,4,1024,1,4> (FP32 VALU stencils; one launch = one 16-slice segment)"}
+    out = {
+        "metric": "HMC traj/s at 32x32, Nt=128: mixed-prec vs FP64 GPU; force rel err vs FP64",
+        "value": tpsmx, "unit": "traj/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elmx / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (numpy normals, seed 3)",
+        "config": {"workload": "HMC 32x32 triangular, Nt=128, Nmd=10, traj=pi/2, 2 flavours, "
+                               "checkpoint 16, mixed precision", "global_batch": 1, "seq_len": NT,
+                   "parallelism": "none (1 GPU)"},
+        "fp64_traj_per_s": tps64, "fp64_ms_per_step": 1e3 * el64 / args.steps,
+        "mixed_over_fp64": tpsmx / tps64,
+        "force_rel_err": acc["force_rel_err"], "dH_abs_err": acc["dH_abs_err"],
+        "logdet_rel_err_metropolis": acc["logdet_rel_err"],
+        "kernel_ms": {p: {g: round(v[0], 4) for g, v in res[p]["groups"].items()} for p in res},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

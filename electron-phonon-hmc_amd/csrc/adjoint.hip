@@ -39,7 +39,52 @@ struct AdjArgs {
 	double *partial;        // [tau - tau_b][nwg][3 ns]
 	unsigned nwg;
 	int tau_b, tau_e;
+	// reduction of the previous segment's partials, folded into this launch
+	const double *red_src;  // partials of the segment reduced here, or null
+	int red_tau_b, red_ntau;
+	double f[3];            // -kappa_k flavors
+	double *pdot, *p;       // outputs of the reduction
+	double kick;            // p += kick F when p != null
 };
+
+// Deterministic sum over workgroups of the partial bond sums of one segment:
+// F[tau][link] = f_k sum_w partial[tau - tau_b][w][link].  The links [l0, l0 + nl) are summed by
+// `nthreads` threads: thread t handles link t % P and the workgroup chunk t / P (P = min(nl,
+// nthreads)); chunk sums are combined in chunk order through `scratch` (nthreads doubles).
+__device__ void reduce_links(const double *__restrict__ part, unsigned nwg, unsigned loc, unsigned l0, unsigned nl,
+		unsigned nlinks_total, int tau_b, const double (&f)[3], double *__restrict__ pdot, double *__restrict__ p,
+		double kick, double *scratch, unsigned nthreads)
+{
+	const unsigned t = threadIdx.x;
+	const unsigned P = nl < nthreads ? nl : nthreads;
+	if(P == 0) return;
+	unsigned CH = nthreads / P;
+	if(CH > nwg) CH = nwg;
+	for(unsigned g0 = 0; g0 < nl; g0 += P){
+		const unsigned ll = t % P, ch = t / P;
+		double s = 0;
+		const unsigned lk = l0 + g0 + ll;
+		if(ch < CH && g0 + ll < nl && lk < nlinks_total){
+			const unsigned tl = lk / loc, link = lk - tl * loc;
+			const unsigned w0 = (unsigned)((size_t)nwg * ch / CH), w1 = (unsigned)((size_t)nwg * (ch + 1) / CH);
+			const double *src = part + (size_t)tl * nwg * loc + link;
+			for(unsigned w = w0; w < w1; w++) s += src[(size_t)w * loc];
+		}
+		if(ch < CH) scratch[ch * P + ll] = s;
+		__syncthreads();
+		if(t < P && g0 + t < nl && l0 + g0 + t < nlinks_total){
+			double tot = 0;
+			for(unsigned q = 0; q < CH; q++) tot += scratch[q * P + t];
+			const unsigned lk2 = l0 + g0 + t;
+			const unsigned tl = lk2 / loc, link = lk2 - tl * loc;
+			const double F = f[link % 3] * tot;
+			const size_t pos = (size_t)(tau_b + tl) * loc + link;
+			if(pdot) pdot[pos] = F;
+			if(p) p[pos] += kick * F;
+		}
+		__syncthreads();
+	}
+}
 
 template <typename R, int C>
 __device__ __forceinline__ void gather6(const R *__restrict__ buf, const int (&nbr)[6], const R (&cf)[6], R mu,
@@ -65,7 +110,14 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 	R *wb0 = buf1 + (size_t)C * ns;
 	R *wb1 = wb0 + (size_t)C * ns;
 	R *hx = wb1 + (size_t)C * ns;          // [site][3]
+	double *rscratch = reinterpret_cast<double *>(smem_raw + (((4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7));
 	R *bufs[2] = {buf0, buf1};
+	if(a.red_src){
+		const unsigned total = (unsigned)a.red_ntau * 3 * ns;
+		const unsigned per = (total + a.nwg - 1) / a.nwg;
+		reduce_links(a.red_src, a.nwg, 3 * ns, blockIdx.x * per, per, total, a.red_tau_b, a.f, a.pdot, a.p, a.kick,
+			rscratch, BT);
+	}
 	R *wbs[2] = {wb0, wb1};
 
 	const unsigned col0 = blockIdx.x * C;
@@ -287,23 +339,14 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			if(act[j] && colok(c)) a.wbuf[(size_t)(col0 + c) * ns + site[j]] = wv[j][c];
 }
 
-// p_dot[tau][link] = -kappa_k flavors sum_wg partial[tau - tau_b][wg][link]; optional kick p += h F.
-__global__ void force_reduce_kernel(const double *__restrict__ partial, unsigned nwg, unsigned loc, int tau_b, int ntau,
-		double f0, double f1, double f2, double *__restrict__ pdot, double *__restrict__ p, double kick)
+// Stand-alone form of reduce_links for the last segment: 64 links x 4 chunks per workgroup.
+__global__ __launch_bounds__(256) void force_reduce_kernel(const double *__restrict__ partial, unsigned nwg, unsigned loc,
+		int tau_b, int ntau, double f0, double f1, double f2, double *__restrict__ pdot, double *__restrict__ p, double kick)
 {
-	const size_t n = (size_t)ntau * loc;
-	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x){
-		const size_t tl = e / loc;
-		const unsigned link = (unsigned)(e - tl * loc);
-		const double *src = partial + tl * nwg * loc + link;
-		double s = 0;
-		for(unsigned w = 0; w < nwg; w++) s += src[(size_t)w * loc];
-		const unsigned k = link % 3;
-		const double F = (k == 0 ? f0 : (k == 1 ? f1 : f2)) * s;
-		const size_t pos = (size_t)(tau_b + tl) * loc + link;
-		if(pdot) pdot[pos] = F;
-		if(p) p[pos] += kick * F;
-	}
+	__shared__ double scratch[256];
+	const double f[3] = {f0, f1, f2};
+	const unsigned total = (unsigned)ntau * loc;
+	reduce_links(partial, nwg, loc, blockIdx.x * 64, 64, total, tau_b, f, pdot, p, kick, scratch, 256);
 }
 
 unsigned adjoint_cols(const ephmc_ctx *c, int prec)
@@ -315,7 +358,8 @@ template <typename R, int C>
 static int launch_adjoint_c(ephmc_ctx *c, AdjArgs<R> a)
 {
 	const unsigned ns = a.ns;
-	const size_t lds = (4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R);
+	const unsigned bt = ns <= 256 ? 256 : (ns <= 512 ? 512 : 1024);
+	const size_t lds = (((4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7) + bt * sizeof(double);
 	if(lds > 160 * 1024){ set_error("adjoint: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
 	const unsigned blocks = a.nwg;
 #define EPHMC_ADJ_LAUNCH2(BT, S, MAXD)                                                            \
@@ -359,28 +403,43 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 	a.nwg = (ns + C - 1) / C;
 	// W state between segments; T_nt is no longer needed once M_hat is formed
 	a.wbuf = prec == EPHMC_FP64 ? (R *)(void *)c->d_Tfin : (R *)(void *)c->d_Tfin32;
-	a.partial = c->d_partial;
 	const R *ck = (const R *)c->d_ck;
 	const double f[3] = {-c->cfg.kappa[0] * c->cfg.flavors, -c->cfg.kappa[1] * c->cfg.flavors, -c->cfg.kappa[2] * c->cfg.flavors};
+	for(int q = 0; q < 3; q++) a.f[q] = f[q];
+	a.pdot = pdot;
+	a.p = p;
+	a.kick = kick;
+	int prev_b = 0, prev_n = 0;
+	const double *prev = nullptr;
 	for(int s = (int)c->nseg - 1; s >= 0; s--){
 		a.tau_b = s * (int)k;
 		a.tau_e = std::min<int>((s + 1) * (int)k, (int)nt);
 		a.ck = ck + (size_t)s * mat;
 		a.seg = (R *)c->d_seg;
 		a.w_init = (s == (int)c->nseg - 1) ? minv : nullptr;
+		a.partial = c->d_partial[s & 1];
+		a.red_src = prev;
+		a.red_tau_b = prev_b;
+		a.red_ntau = prev_n;
 		int rc;
+		EPHMC_TRY(timer_begin(c, "adjoint_kernel"));
 		switch(C){
 			case 1: rc = launch_adjoint_c<R, 1>(c, a); break;
 			case 2: rc = launch_adjoint_c<R, 2>(c, a); break;
 			default: rc = launch_adjoint_c<R, 4>(c, a); break;
 		}
 		if(rc != EPHMC_OK) return rc;
-		const int ntau = a.tau_e - a.tau_b;
-		const size_t n = (size_t)ntau * c->loc;
-		force_reduce_kernel<<<(unsigned)std::min<size_t>((n + 255) / 256, 2048), 256, 0, c->stream>>>(
-			c->d_partial, a.nwg, c->loc, a.tau_b, ntau, f[0], f[1], f[2], pdot, p, kick);
-		EPHMC_LAUNCHED();
+		EPHMC_TRY(timer_end(c, "adjoint_kernel"));
+		prev = a.partial;
+		prev_b = a.tau_b;
+		prev_n = a.tau_e - a.tau_b;
 	}
+	const size_t n = (size_t)prev_n * c->loc;
+	EPHMC_TRY(timer_begin(c, "force_reduce"));
+	force_reduce_kernel<<<(unsigned)((n + 63) / 64), 256, 0, c->stream>>>(prev, a.nwg, c->loc, prev_b, prev_n,
+		f[0], f[1], f[2], pdot, p, kick);
+	EPHMC_LAUNCHED();
+	EPHMC_TRY(timer_end(c, "force_reduce"));
 	return EPHMC_OK;
 }
 

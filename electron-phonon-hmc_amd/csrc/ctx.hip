@@ -290,11 +290,13 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 		|| (rc = dalloc(&c->d_coef32, mixed ? (size_t)c->nt * 6 * ns : 0))
 		|| (rc = dalloc((unsigned char **)&c->d_ck, (size_t)c->nseg * c->mat * rsz))
 		|| (rc = dalloc((unsigned char **)&c->d_seg, (size_t)c->nwg * c->cols * ns * c->ckpt * rsz))
-		|| (rc = dalloc(&c->d_partial, (size_t)c->ckpt * c->nwg * c->loc))
+		|| (rc = dalloc(&c->d_partial[0], (size_t)c->ckpt * c->nwg * c->loc))
+		|| (rc = dalloc(&c->d_partial[1], (size_t)c->ckpt * c->nwg * c->loc))
 		|| (rc = dalloc(&c->d_Tfin, c->mat))
 		|| (rc = dalloc(&c->d_Tfin32, mixed ? c->mat : 0))
 		|| (rc = dalloc(&c->d_gj[0], c->mat)) || (rc = dalloc(&c->d_gj[1], c->mat))
 		|| (rc = dalloc(&c->d_piv, ns)) || (rc = dalloc(&c->d_src, ns)) || (rc = dalloc(&c->d_diag, ns))
+		|| (rc = dalloc(&c->d_dlu, 32 * 32))
 		|| (rc = dalloc(&c->d_scal, SCAL_COUNT))
 		|| (rc = dalloc(&c->d_twiddle, 2 * (size_t)c->nt)) || (rc = dalloc(&c->d_f2, c->nt))
 		|| (rc = dalloc(&c->d_red, std::max<size_t>(2 * c->red_blocks, fft_blocks)))
@@ -334,8 +336,8 @@ int ephmc_destroy(ephmc_ctx *c)
 {
 	if(!c) return EPHMC_OK;
 	if(c->stream) hipStreamSynchronize(c->stream);
-	void *bufs[] = {c->d_nb, c->d_link, c->d_coef64, c->d_coef32, c->d_ck, c->d_seg, c->d_partial, c->d_Tfin, c->d_Tfin32,
-		c->d_gj[0], c->d_gj[1], c->d_piv, c->d_src, c->d_diag, c->d_scal, c->d_twiddle, c->d_f2, c->d_red,
+	void *bufs[] = {c->d_nb, c->d_link, c->d_coef64, c->d_coef32, c->d_ck, c->d_seg, c->d_partial[0], c->d_partial[1], c->d_Tfin,
+		c->d_Tfin32, c->d_gj[0], c->d_gj[1], c->d_piv, c->d_src, c->d_diag, c->d_dlu, c->d_scal, c->d_twiddle, c->d_f2, c->d_red,
 		c->d_x, c->d_p, c->d_pdot, c->d_xnew, c->d_normals, c->d_minv, c->d_mhat};
 	for(void *b : bufs) if(b) hipFree(b);
 	for(int s = 0; s < 2; s++){
@@ -571,6 +573,15 @@ int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamples, const 
 	(void)c; (void)normals; (void)nsamples; (void)contr_mat; (void)width; (void)nbins; (void)greens; (void)results;
 	set_error("transient localisation is not provided by this build yet");
 	return EPHMC_ERR_UNSUPPORTED;
+}
+
+int ephmc_gj_fallbacks(ephmc_ctx *c, unsigned long *count)
+{
+	if(!c || !count){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	double v;
+	EPHMC_TRY(read_scal(c, SCAL_GJ_FALLBACKS, &v));
+	*count = (unsigned long)v;
+	return EPHMC_OK;
 }
 
 int ephmc_timing(ephmc_ctx *c, int enable)

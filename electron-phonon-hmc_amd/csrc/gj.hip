@@ -1,159 +1,144 @@
-// gj.hip -- log|det| and inverse of M_hat by blocked Gauss-Jordan elimination with partial
-// pivoting (FP64), replacing LAPACK dgetrf + dgetri in logDetM_and_inv_M_hat
-// (organic_dense_matrix.c:547-607).
+// gj.hip -- log|det| and inverse of M_hat (FP64), replacing LAPACK dgetrf + dgetri in
+// logDetM_and_inv_M_hat (organic_dense_matrix.c:547-607).
 //
-// Row pivots are chosen exactly as in LU with partial pivoting (largest |a| among the rows not
-// yet pivoted, first index on ties, like idamax), so the pivot values are LU's U_ii and
-// log|det| = sum log|d_g|, sign = prod sign(d_g) * (-1)^(#row swaps) (organic_dense_matrix.c:582-588).
+// LAPACK's partial pivoting picks, at elimination step g, the row with the largest |a_rg|
+// among rows r >= g (first index on ties).  Elimination without row exchanges makes the same
+// choices exactly when every multiplier |l_rg| = |a_rg| / |a_gg| is <= 1; the pivots are then
+// the same U_ii, so log|det| = sum log|U_ii| and the sign (organic_dense_matrix.c:582-588)
+// agree with LAPACK, and the inverse is the same matrix up to rounding.  For M_hat = 1 + B..B at
+// the lattice parameters of the reference (mu at the band bottom) the multipliers are of order
+// 1e-2, so:
 //
-// Block step for columns K = [k0, k0+nb):
-//   panel  (one workgroup, rows in registers): the nb unblocked in-place Gauss-Jordan steps on
-//          the N x nb block column; yields E (the block's non-trivial transform columns, which
-//          are also the final in-place contents of columns K), the pivots and the composed row
-//          permutation src.
-//   update (all other columns, out of place):  W'[i,c] = [i not in K] W[src i, c]
-//          + sum_kk E[i,kk] W[src(k0+kk), c].
-// After the last block the in-place result is A^{-1} with its columns permuted; undoing the
-// swaps in reverse order (a gather by the composed permutation) gives the inverse.
+//   fast path: blocked Gauss-Jordan without row exchanges, 32 columns per block K = [k0, k0+nb):
+//     diag   one wave: LU of the diagonal block D = A[K,K]; multipliers checked |l| <= 1
+//     panel  every row i: E[i,K] = -R_i D^{-1} (i not in K), D^{-1} rows (i in K), through the
+//            triangular solves y = R_i U^{-1}, e = y L^{-1}; rows below the block also check
+//            |y| <= 1 (y is row i of LU's L for those rows)
+//     update every column c outside K: A'[:,c] = [i not in K] A[:,c] + E[:,K] A[K,c]
+//   check:   any multiplier > 1 sets a device flag; the single-workgroup fallback kernel, always
+//            launched, then inverts the untouched input with partial pivoting (in-place
+//            Gauss-Jordan with row exchanges and the columns un-permuted at the end); without
+//            the flag it returns at once.  No host round trip either way.
 #include "internal.h"
 
 namespace ephmc {
 
-template <int NB, int RT, int BT>
-__global__ __launch_bounds__(BT) void gj_panel_kernel(const double *__restrict__ A, double *__restrict__ An,
-		unsigned n, unsigned k0, unsigned nbe, int *__restrict__ piv, double *__restrict__ diag, int *__restrict__ src)
+constexpr int GJ_NB = 32;
+
+__global__ void gj_clear_kernel(double *scal)
 {
-	extern __shared__ int rowid[];             // n entries
-	__shared__ double red_val[BT / 64];
-	__shared__ int red_idx[BT / 64];
-	__shared__ double rowA[NB], rowB[NB];
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	scal[SCAL_GJ_FLAG] = 0;
+}
 
-	double P[RT][NB];
+// LU (no row exchanges) of the nbe x nbe diagonal block, one wave; lane c holds column c.
+// dlu (NB x NB, column-major) receives L (strict lower, unit diagonal implied) and U, padded
+// with the identity beyond nbe.
+template <int NB>
+__global__ __launch_bounds__(64) void gj_diag_kernel(const double *__restrict__ A, unsigned n, unsigned k0, unsigned nbe,
+		double *__restrict__ dlu, double *__restrict__ diag, int *__restrict__ piv, double *__restrict__ scal)
+{
+	__shared__ double lb[NB];
+	const int lane = threadIdx.x;
+	double col[NB];
 #pragma unroll
-	for(int j = 0; j < RT; j++){
-		const unsigned r = tid + j * BT;
+	for(int r = 0; r < NB; r++)
+		col[r] = (lane < (int)nbe && r < (int)nbe) ? A[(size_t)(k0 + lane) * n + k0 + r] : (r == lane ? 1.0 : 0.0);
+	bool bad = false;
+	for(int g = 0; g < (int)nbe; g++){
+		if(lane == g){
+			double d = 0;
 #pragma unroll
-		for(int kk = 0; kk < NB; kk++)
-			P[j][kk] = (r < n && kk < (int)nbe) ? A[(size_t)(k0 + kk) * n + r] : 0.0;
-	}
-	for(unsigned r = tid; r < n; r += BT) rowid[r] = (int)r;
-	__syncthreads();
-
-	for(int jj = 0; jj < (int)nbe; jj++){
-		const unsigned g = k0 + jj;
-		// pivot search among rows >= g
-		double best = -1.0;
-		int bidx = 0x7fffffff;
+			for(int r = 0; r < NB; r++) if(r == g) d = col[r];
+			const double inv = 1.0 / d;
 #pragma unroll
-		for(int j = 0; j < RT; j++){
-			const unsigned r = tid + j * BT;
-			if(r < n && r >= g){
-				double a = 0;
-#pragma unroll
-				for(int kk = 0; kk < NB; kk++) if(kk == jj) a = fabs(P[j][kk]);
-				if(a > best){ best = a; bidx = (int)r; }
-			}
-		}
-#pragma unroll
-		for(int off = 32; off >= 1; off >>= 1){
-			const double ob = __shfl_xor(best, off, 64);
-			const int oi = __shfl_xor(bidx, off, 64);
-			if(ob > best || (ob == best && oi < bidx)){ best = ob; bidx = oi; }
-		}
-		if(lane == 0){ red_val[wave] = best; red_idx[wave] = bidx; }
-		__syncthreads();
-		best = red_val[0]; bidx = red_idx[0];
-#pragma unroll
-		for(int w = 1; w < BT / 64; w++){
-			const double ob = red_val[w];
-			const int oi = red_idx[w];
-			if(ob > best || (ob == best && oi < bidx)){ best = ob; bidx = oi; }
-		}
-		const unsigned p = (bidx < (int)n) ? (unsigned)bidx : g;   // all-NaN column: no swap
-		// publish pivot row (old row p) and, when swapping, old row g
-#pragma unroll
-		for(int j = 0; j < RT; j++){
-			const unsigned r = tid + j * BT;
-			if(r == p){
-#pragma unroll
-				for(int kk = 0; kk < NB; kk++) rowA[kk] = P[j][kk];
-			}
-			if(p != g && r == g){
-#pragma unroll
-				for(int kk = 0; kk < NB; kk++) rowB[kk] = P[j][kk];
-			}
-		}
-		if(tid == 0){
-			piv[g] = (int)p;
-			if(p != g){ const int t = rowid[g]; rowid[g] = rowid[p]; rowid[p] = t; }
+			for(int r = 0; r < NB; r++)
+				if(r > g){
+					col[r] *= inv;
+					lb[r] = col[r];
+					bad |= !(fabs(col[r]) <= 1.0);
+				}
+			diag[k0 + g] = d;
+			piv[k0 + g] = (int)(k0 + g);
 		}
 		__syncthreads();
-		const double *u = rowA;    // pivot row, read as LDS broadcasts
+		if(lane > g && lane < NB){
+			double u = 0;
 #pragma unroll
-		for(int j = 0; j < RT; j++){
-			const unsigned r = tid + j * BT;
-			if(r == g){
+			for(int r = 0; r < NB; r++) if(r == g) u = col[r];
 #pragma unroll
-				for(int kk = 0; kk < NB; kk++) P[j][kk] = u[kk];
-			}else if(p != g && r == p){
-#pragma unroll
-				for(int kk = 0; kk < NB; kk++) P[j][kk] = rowB[kk];
-			}
+			for(int r = 0; r < NB; r++) if(r > g) col[r] -= lb[r] * u;
 		}
-		double d = 0;
-#pragma unroll
-		for(int kk = 0; kk < NB; kk++) if(kk == jj) d = u[kk];
-		const double inv = 1.0 / d;
-		if(tid == 0) diag[g] = d;
-#pragma unroll
-		for(int j = 0; j < RT; j++){
-			const unsigned r = tid + j * BT;
-			if(r >= n) continue;
-			if(r == g){
-#pragma unroll
-				for(int kk = 0; kk < NB; kk++) P[j][kk] = (kk == jj) ? inv : u[kk] * inv;
-			}else{
-				double f = 0;
-#pragma unroll
-				for(int kk = 0; kk < NB; kk++) if(kk == jj) f = P[j][kk];
-				const double fi = f * inv;
-#pragma unroll
-				for(int kk = 0; kk < NB; kk++) P[j][kk] = (kk == jj) ? -fi : P[j][kk] - fi * u[kk];
-			}
-		}
+		__syncthreads();
 	}
+	if(lane < NB){
+#pragma unroll
+		for(int r = 0; r < NB; r++) dlu[lane * NB + r] = col[r];
+	}
+	if(bad) scal[SCAL_GJ_FLAG] = 1;
+}
+
+// Transform columns E[:, K] for every row (see header); one thread per row.
+template <int NB>
+__global__ __launch_bounds__(256) void gj_panel_np_kernel(const double *__restrict__ A, double *__restrict__ An,
+		unsigned n, unsigned k0, unsigned nbe, const double *__restrict__ dlu, double *__restrict__ scal)
+{
+	__shared__ double lu[NB * NB];
+	for(int e = threadIdx.x; e < NB * NB; e += 256) lu[e] = dlu[e];
 	__syncthreads();
+	const unsigned i = blockIdx.x * 256 + threadIdx.x;
+	if(i >= n) return;
+	const bool inK = i >= k0 && i < k0 + nbe;
+	double y[NB];
 #pragma unroll
-	for(int j = 0; j < RT; j++){
-		const unsigned r = tid + j * BT;
-		if(r < n)
-#pragma unroll
-			for(int kk = 0; kk < NB; kk++)
-				if(kk < (int)nbe) An[(size_t)(k0 + kk) * n + r] = P[j][kk];
+	for(int j = 0; j < NB; j++){
+		double r = 0;
+		if(j < (int)nbe) r = inK ? ((int)(i - k0) == j ? 1.0 : 0.0) : A[(size_t)(k0 + j) * n + i];
+		y[j] = r;
 	}
-	for(unsigned r = tid; r < n; r += BT) src[r] = rowid[r];
+	// y <- y U^{-1}  (U[m][j] = lu[j*NB + m], m <= j)
+#pragma unroll
+	for(int j = 0; j < NB; j++){
+		double s = y[j];
+#pragma unroll
+		for(int m = 0; m < j; m++) s -= y[m] * lu[j * NB + m];
+		y[j] = s / lu[j * NB + j];
+	}
+	if(i >= k0 + nbe){
+		bool bad = false;
+#pragma unroll
+		for(int j = 0; j < NB; j++) bad |= !(fabs(y[j]) <= 1.0);
+		if(bad) scal[SCAL_GJ_FLAG] = 1;
+	}
+	// e <- y L^{-1}  (L[m][j] = lu[j*NB + m], m > j, unit diagonal)
+#pragma unroll
+	for(int j = NB - 1; j >= 0; j--){
+		double s = y[j];
+#pragma unroll
+		for(int m = j + 1; m < NB; m++) s -= y[m] * lu[j * NB + m];
+		y[j] = s;
+	}
+#pragma unroll
+	for(int j = 0; j < NB; j++)
+		if(j < (int)nbe) An[(size_t)(k0 + j) * n + i] = inK ? y[j] : -y[j];
 }
 
 // Rank-nb update of every column outside K (64 x 64 tiles, 4 x 4 outputs per thread).
 template <int NB>
 __global__ __launch_bounds__(256) void gj_update_kernel(const double *__restrict__ A, double *__restrict__ An,
-		const int *__restrict__ src, unsigned n, unsigned k0, unsigned nbe)
+		unsigned n, unsigned k0, unsigned nbe)
 {
-	__shared__ double Y[NB][64 + 1];      // pivot rows of this column tile
-	__shared__ double E[NB][64 + 1];      // transform rows of this row tile (transposed)
-	__shared__ int rsrc[64];
+	__shared__ double Y[NB][64 + 1];      // rows K of this column tile
+	__shared__ double E[NB][64 + 1];      // transform columns of this row tile (transposed)
 	const unsigned row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
 	const int tid = threadIdx.x;
 	if(col0 >= k0 && col0 + 64 <= k0 + nbe) return;   // tile entirely inside K
 	for(int e = tid; e < NB * 64; e += 256){
 		const int kk = e / 64, cc = e % 64;
 		const unsigned col = col0 + cc;
-		Y[kk][cc] = (kk < (int)nbe && col < n) ? A[(size_t)col * n + src[k0 + kk]] : 0.0;
+		Y[kk][cc] = (kk < (int)nbe && col < n) ? A[(size_t)col * n + k0 + kk] : 0.0;
 		const unsigned row = row0 + cc;
 		E[kk][cc] = (kk < (int)nbe && row < n) ? An[(size_t)(k0 + kk) * n + row] : 0.0;
 	}
-	if(tid < 64) rsrc[tid] = (row0 + tid < n) ? src[row0 + tid] : 0;
 	__syncthreads();
 	const int tr = tid & 15, tc = tid >> 4;           // rows tr + 16 i, columns tc + 16 j
 	double acc[4][4];
@@ -164,7 +149,7 @@ __global__ __launch_bounds__(256) void gj_update_kernel(const double *__restrict
 #pragma unroll
 		for(int j = 0; j < 4; j++){
 			const unsigned col = col0 + tc + 16 * j;
-			acc[i][j] = (row < n && col < n && !inK) ? A[(size_t)col * n + rsrc[tr + 16 * i]] : 0.0;
+			acc[i][j] = (row < n && col < n && !inK) ? A[(size_t)col * n + row] : 0.0;
 		}
 	}
 #pragma unroll 8
@@ -191,24 +176,22 @@ __global__ __launch_bounds__(256) void gj_update_kernel(const double *__restrict
 	}
 }
 
-// log|det|, sign and the column permutation that undoes the pivoting.
-__global__ __launch_bounds__(1024) void gj_finalize_kernel(const double *__restrict__ diag, const int *__restrict__ piv,
-		unsigned n, int *__restrict__ colsrc, double *__restrict__ scal)
+// log|det| and sign from the pivots of the fast path (no row exchanges).
+__global__ __launch_bounds__(1024) void gj_logdet_kernel(const double *__restrict__ diag, unsigned n, double *__restrict__ scal)
 {
-	extern __shared__ int perm[];
 	__shared__ double part[1024];
 	__shared__ int negs[1024];
 	const int tid = threadIdx.x;
+	if(scal[SCAL_GJ_FLAG] != 0) return;
 	double s = 0;
 	int neg = 0;
 	for(unsigned i = tid; i < n; i += 1024){
 		const double d = diag[i];
 		s += log(fabs(d));
-		neg += (d < 0) + (piv[i] != (int)i);
+		neg += d < 0;
 	}
 	part[tid] = s;
 	negs[tid] = neg;
-	for(unsigned i = tid; i < n; i += 1024) perm[i] = (int)i;
 	__syncthreads();
 	for(int off = 512; off > 0; off >>= 1){
 		if(tid < off){ part[tid] += part[tid + off]; negs[tid] += negs[tid + off]; }
@@ -217,62 +200,120 @@ __global__ __launch_bounds__(1024) void gj_finalize_kernel(const double *__restr
 	if(tid == 0){
 		scal[SCAL_LOGDET] = part[0];
 		scal[SCAL_SIGN] = (negs[0] & 1) ? -1.0 : 1.0;
-		// columns swapped back in reverse pivot order: final column c = in-place column perm[c]
-		for(int g = (int)n - 1; g >= 0; g--){
-			const int p = piv[g];
-			if(p != g){ const int t = perm[g]; perm[g] = perm[p]; perm[p] = t; }
-		}
 	}
-	__syncthreads();
-	for(unsigned i = tid; i < n; i += 1024) colsrc[i] = perm[i];
 }
 
-__global__ void gather_columns_kernel(const double *__restrict__ W, double *__restrict__ out,
-		const int *__restrict__ colsrc, unsigned n)
+__global__ void gj_copy_kernel(const double *__restrict__ W, double *__restrict__ out, size_t n, const double *__restrict__ scal)
 {
-	const unsigned col = blockIdx.y;
-	const size_t from = (size_t)colsrc[col] * n, to = (size_t)col * n;
-	for(unsigned r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
-		out[to + r] = W[from + r];
+	if(scal[SCAL_GJ_FLAG] != 0) return;   // the fallback inverts the untouched input in place
+	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) out[e] = W[e];
 }
 
-template <int NB, int RT, int BT>
-static int launch_panel(ephmc_ctx *c, const double *A, double *An, unsigned k0, unsigned nbe)
+// Fallback: in-place Gauss-Jordan with partial pivoting on M (one workgroup).  Runs only when
+// the fast path saw a multiplier above 1.
+__global__ __launch_bounds__(1024) void gj_fallback_kernel(double *__restrict__ M, unsigned n, double *__restrict__ scal)
 {
-	gj_panel_kernel<NB, RT, BT><<<1, BT, c->ns * sizeof(int), c->stream>>>(A, An, c->ns, k0, nbe, c->d_piv, c->d_diag, c->d_src);
-	EPHMC_LAUNCHED();
-	return EPHMC_OK;
+	extern __shared__ double fsh[];
+	double *colg = fsh;                                   // n: multipliers of the step
+	int *piv = reinterpret_cast<int *>(fsh + n);          // n
+	__shared__ double rv[16];
+	__shared__ int ri[16];
+	__shared__ double ld_acc;
+	__shared__ int neg_acc;
+	if(scal[SCAL_GJ_FLAG] == 0) return;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	if(tid == 0){ ld_acc = 0; neg_acc = 0; }
+	for(unsigned g = 0; g < n; g++){
+		double best = -1;
+		int bidx = 0x7fffffff;
+		for(unsigned r = g + tid; r < n; r += 1024){
+			const double a = fabs(M[(size_t)g * n + r]);
+			if(a > best){ best = a; bidx = (int)r; }
+		}
+		for(int off = 32; off >= 1; off >>= 1){
+			const double ob = __shfl_xor(best, off, 64);
+			const int oi = __shfl_xor(bidx, off, 64);
+			if(ob > best || (ob == best && oi < bidx)){ best = ob; bidx = oi; }
+		}
+		if(lane == 0){ rv[wave] = best; ri[wave] = bidx; }
+		__syncthreads();
+		best = rv[0]; bidx = ri[0];
+		for(int w = 1; w < 16; w++) if(rv[w] > best || (rv[w] == best && ri[w] < bidx)){ best = rv[w]; bidx = ri[w]; }
+		const unsigned p = bidx < (int)n ? (unsigned)bidx : g;
+		if(tid == 0) piv[g] = (int)p;
+		if(p != g)
+			for(unsigned cc = tid; cc < n; cc += 1024){
+				const double t = M[(size_t)cc * n + g];
+				M[(size_t)cc * n + g] = M[(size_t)cc * n + p];
+				M[(size_t)cc * n + p] = t;
+			}
+		__syncthreads();
+		const double d = M[(size_t)g * n + g];
+		for(unsigned r = tid; r < n; r += 1024) colg[r] = M[(size_t)g * n + r];
+		if(tid == 0){ ld_acc += log(fabs(d)); neg_acc += (d < 0) + (p != g); }
+		__syncthreads();
+		const double inv = 1.0 / d;
+		for(unsigned cc = tid; cc < n; cc += 1024) if(cc != g) M[(size_t)cc * n + g] *= inv;
+		__syncthreads();
+		for(unsigned cc = 0; cc < n; cc++){
+			const double u = cc == g ? 0.0 : M[(size_t)cc * n + g];
+			for(unsigned r = tid; r < n; r += 1024){
+				if(r == g) continue;
+				if(cc == g) M[(size_t)cc * n + r] = -colg[r] * inv;
+				else M[(size_t)cc * n + r] -= colg[r] * u;
+			}
+		}
+		if(tid == 0) M[(size_t)g * n + g] = inv;
+		__syncthreads();
+	}
+	// undo the row exchanges on the columns, in reverse order
+	for(int g = (int)n - 1; g >= 0; g--){
+		const int p = piv[g];
+		if(p != g)
+			for(unsigned r = tid; r < n; r += 1024){
+				const double t = M[(size_t)g * n + r];
+				M[(size_t)g * n + r] = M[(size_t)p * n + r];
+				M[(size_t)p * n + r] = t;
+			}
+		__syncthreads();
+	}
+	if(tid == 0){
+		scal[SCAL_LOGDET] = ld_acc;
+		scal[SCAL_SIGN] = (neg_acc & 1) ? -1.0 : 1.0;
+		scal[SCAL_GJ_FALLBACKS] += 1;
+	}
 }
 
 // M (column-major, N x N, FP64) <- M^{-1}; log|det| and sign into the device scalars.
 int gj_logdet_inverse(ephmc_ctx *c, double *M)
 {
 	const unsigned n = c->ns;
-	// panel width: 32 columns for 256-thread panels, 16 for 1024-thread panels (the
-	// register-resident panel row then stays within the 128 VGPRs of a 16-wave workgroup)
-	int nb;
-	if(n <= 256) nb = 32;
-	else if(n <= 2048) nb = 16;
-	else { set_error("gj: N = %u exceeds 2048", n); return EPHMC_ERR_UNSUPPORTED; }
+	if(n > 4096){ set_error("gj: N = %u exceeds 4096", n); return EPHMC_ERR_UNSUPPORTED; }
+	gj_clear_kernel<<<1, 1, 0, c->stream>>>(c->d_scal);
+	EPHMC_LAUNCHED();
 	const double *cur = M;
 	double *buf[2] = {c->d_gj[0], c->d_gj[1]};
 	int which = 0;
 	const dim3 ugrid((n + 63) / 64, (n + 63) / 64);
-	for(unsigned k0 = 0; k0 < n; k0 += nb){
-		const unsigned nbe = std::min<unsigned>(nb, n - k0);
+	for(unsigned k0 = 0; k0 < n; k0 += GJ_NB){
+		const unsigned nbe = std::min<unsigned>(GJ_NB, n - k0);
 		double *next = buf[which];
-		if(n <= 256) EPHMC_TRY((launch_panel<32, 1, 256>(c, cur, next, k0, nbe)));
-		else if(n <= 1024) EPHMC_TRY((launch_panel<16, 1, 1024>(c, cur, next, k0, nbe)));
-		else EPHMC_TRY((launch_panel<16, 2, 1024>(c, cur, next, k0, nbe)));
-		if(nb == 32) gj_update_kernel<32><<<ugrid, 256, 0, c->stream>>>(cur, next, c->d_src, n, k0, nbe);
-		else gj_update_kernel<16><<<ugrid, 256, 0, c->stream>>>(cur, next, c->d_src, n, k0, nbe);
+		gj_diag_kernel<GJ_NB><<<1, 64, 0, c->stream>>>(cur, n, k0, nbe, c->d_dlu, c->d_diag, c->d_piv, c->d_scal);
+		EPHMC_LAUNCHED();
+		gj_panel_np_kernel<GJ_NB><<<(n + 255) / 256, 256, 0, c->stream>>>(cur, next, n, k0, nbe, c->d_dlu, c->d_scal);
+		EPHMC_LAUNCHED();
+		gj_update_kernel<GJ_NB><<<ugrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe);
 		EPHMC_LAUNCHED();
 		cur = next;
 		which ^= 1;
 	}
-	gj_finalize_kernel<<<1, 1024, n * sizeof(int), c->stream>>>(c->d_diag, c->d_piv, n, c->d_src, c->d_scal);
+	gj_logdet_kernel<<<1, 1024, 0, c->stream>>>(c->d_diag, n, c->d_scal);
 	EPHMC_LAUNCHED();
-	gather_columns_kernel<<<dim3((n + 255) / 256, n), 256, 0, c->stream>>>(cur, M, c->d_src, n);
+	gj_copy_kernel<<<1024, 256, 0, c->stream>>>(cur, M, c->mat, c->d_scal);
+	EPHMC_LAUNCHED();
+	const size_t lds = (size_t)n * (sizeof(double) + sizeof(int));
+	(void)hipFuncSetAttribute((const void *)gj_fallback_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	gj_fallback_kernel<<<1, 1024, lds, c->stream>>>(M, n, c->d_scal);
 	EPHMC_LAUNCHED();
 	return EPHMC_OK;
 }

@@ -90,14 +90,15 @@ struct ephmc_ctx {
 	unsigned nwg;                     // ceil(ns / cols)
 	void *d_ck;                       // checkpoints T_{s k}: nseg+1 matrices (R = fp64 or fp32)
 	void *d_seg;                      // per-workgroup segment scratch: k * ns * ns
-	double *d_partial;                // [k][nwg][3 ns] force partial sums
+	double *d_partial[2];             // [k][nwg][3 ns] force partial sums, double-buffered
 	double *d_Tfin;                   // T_nt (fp64) or scratch for conversions
 	float *d_Tfin32;
 
 	// Gauss-Jordan
 	double *d_gj[2];
 	int *d_piv, *d_src;
-	double *d_diag;
+ 	double *d_diag;
+	double *d_dlu;                    // LU of the current diagonal block (32 x 32)
 	double *d_scal;                   // device scalars (see SCAL_* below)
 
 	// bosons
@@ -133,6 +134,8 @@ enum Scalar {            // layout of d_scal
 	SCAL_DH = 9,
 	SCAL_LOGDET_NEW = 10,
 	SCAL_BOSON = 11,      // S_B(x)
+	SCAL_GJ_FLAG = 12,    // a multiplier above 1 was seen by the Gauss-Jordan fast path
+	SCAL_GJ_FALLBACKS = 13, // number of pivoted fallback inversions run
 	SCAL_COUNT = 16
 };
 

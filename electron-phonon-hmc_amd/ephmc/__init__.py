@@ -73,6 +73,7 @@ def _load():
         "ephmc_chain_get_logdet": [vp, dp],
         "ephmc_chain_buffers": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)],
         "ephmc_tl_hist": [vp, dp, C.c_uint, dp, C.c_double, C.c_uint, dp, dp],
+        "ephmc_gj_fallbacks": [vp, C.POINTER(C.c_ulong)],
         "ephmc_timing": [vp, C.c_int],
         "ephmc_timing_get": [vp, C.c_char_p, dp, C.POINTER(C.c_ulong)],
     }
@@ -274,6 +275,11 @@ class Context:
     def chain_logdet(self):
         v = C.c_double()
         check(LIB.ephmc_chain_get_logdet(self.h, C.byref(v)))
+        return v.value
+
+    def gj_fallbacks(self):
+        v = C.c_ulong()
+        check(LIB.ephmc_gj_fallbacks(self.h, C.byref(v)))
         return v.value
 
     def timing(self, enable=True):

@@ -148,6 +148,9 @@ int ephmc_tl_hist(ephmc_ctx *ctx, const double *normals_host, unsigned nsamples,
 		double *greens_host, double *results_host);
 
 /* ---- diagnostics ---------------------------------------------------------------------------- */
+/* Number of inversions that needed row exchanges (a multiplier above 1 in the elimination
+ * without exchanges) and ran the pivoted fallback; waits. */
+int ephmc_gj_fallbacks(ephmc_ctx *ctx, unsigned long *count);
 /* Average duration (ms, HIP events on the context's stream) of the named kernel group since
  * the last reset; groups: "chain", "gj", "adjoint", "reduce", "fft", "tl".  Timing is only
  * recorded after ephmc_timing(ctx, 1). */

@@ -57,6 +57,26 @@ def test_logdet_inverse_random(n):
     s0, l0 = np.linalg.slogdet(A)
     assert ld == pytest.approx(l0, rel=1e-11, abs=1e-11) and sg == int(s0)
     assert rel(inv.reshape(n, n).T, np.linalg.inv(A)) < 1e-10
+    # row exchanges are needed for these matrices exactly when LU without exchanges has a
+    # multiplier above 1; then the pivoted fallback must have run
+    import scipy.linalg as sl
+    lu, piv = sl.lu_factor(A)
+    needs = bool(np.any(piv != np.arange(n)))
+    say("gj random", n, "row exchanges", needs, "fallbacks", ctx.gj_fallbacks())
+    assert (ctx.gj_fallbacks() > 0) == needs
+
+
+def test_logdet_inverse_diagonally_dominant_no_fallback():
+    """A matrix whose multipliers are all below 1 (no exchanges) stays on the fast path."""
+    n = 300
+    om = OracleModel(n, 1, 1)
+    ctx = gpu_context(om)
+    A = np.random.default_rng(5).standard_normal((n, n)) * 0.01 + np.eye(n) * 2
+    ld, sg, inv = ctx.logdet_inv(A.T.copy().ravel())
+    s0, l0 = np.linalg.slogdet(A)
+    assert ld == pytest.approx(l0, rel=1e-12) and sg == int(s0)
+    assert rel(inv.reshape(n, n).T, np.linalg.inv(A)) < 1e-12
+    assert ctx.gj_fallbacks() == 0
 
 
 @pytest.mark.parametrize("l1,l2,nt", [(6, 6, 32), (16, 16, 64)])
@@ -70,6 +90,7 @@ def test_logdet_inverse_M_hat(l1, l2, nt):
     ldg, sg, invg = ctx.logdet_inv(M)
     assert ldg == pytest.approx(ldo, rel=1e-12) and sg == so
     assert rel(invg, invo) < 1e-12
+    assert ctx.gj_fallbacks() == 0
 
 
 @pytest.mark.parametrize("l1,l2,nt,k", [(3, 3, 4, 16), (6, 6, 32, 16), (6, 6, 32, 5), (6, 6, 32, 1), (5, 7, 12, 4),
