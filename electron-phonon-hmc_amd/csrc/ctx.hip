@@ -143,8 +143,8 @@ int timer_end(ephmc_ctx *c, const char *group)
 	return EPHMC_OK;
 }
 
-// Columns per workgroup of the adjoint sweep: the widest that fits LDS, keeps at least one
-// workgroup per CU and (fp64) stays within the register file of a 16-wave workgroup.
+// Columns per workgroup of the adjoint sweep: the widest that fits LDS and keeps at least one
+// workgroup per CU (4 columns also at fp64: fewer partial sums outweigh a few spilled VGPRs).
 // EPHMC_ADJOINT_COLS overrides it for tuning sweeps.
 static unsigned adjoint_cols_auto(unsigned ns, size_t rsize)
 {
@@ -152,8 +152,7 @@ static unsigned adjoint_cols_auto(unsigned ns, size_t rsize)
 		const unsigned v = (unsigned)atoi(e);
 		if(v == 1 || v == 2 || v == 4) return v;
 	}
-	const unsigned widest = rsize == 8 ? 2u : 4u;
-	for(unsigned C = widest; C > 1; C >>= 1){
+	for(unsigned C = 4; C > 1; C >>= 1){
 		const size_t lds = (4 * (size_t)C * ns + 3 * (size_t)ns) * rsize;
 		if(lds <= 160 * 1024 && (ns + C - 1) / C >= 256) return C;
 	}

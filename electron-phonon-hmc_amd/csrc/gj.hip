@@ -10,10 +10,9 @@
 // 1e-2, so:
 //
 //   fast path: blocked Gauss-Jordan without row exchanges, 32 columns per block K = [k0, k0+nb):
-//     diag   one wave: LU of the diagonal block D = A[K,K]; multipliers checked |l| <= 1
-//     panel  every row i: E[i,K] = -R_i D^{-1} (i not in K), D^{-1} rows (i in K), through the
-//            triangular solves y = R_i U^{-1}, e = y L^{-1}; rows below the block also check
-//            |y| <= 1 (y is row i of LU's L for those rows)
+//     panel  D^{-1} of the diagonal block D = A[K,K] (one wave), then every row i:
+//            E[i,K] = -A[i,K] D^{-1} (i not in K) or D^{-1} rows (i in K); the LU multipliers
+//            of D and of the rows below K are checked |l| <= 1
 //     update every column c outside K: A'[:,c] = [i not in K] A[:,c] + E[:,K] A[K,c]
 //   check:   any multiplier > 1 sets a device flag; the single-workgroup fallback kernel, always
 //            launched, then inverts the untouched input with partial pivoting (in-place
@@ -30,149 +29,139 @@ __global__ void gj_clear_kernel(double *scal)
 	scal[SCAL_GJ_FLAG] = 0;
 }
 
-// LU (no row exchanges) of the nbe x nbe diagonal block, one wave; lane c holds column c.
-// dlu (NB x NB, column-major) receives L (strict lower, unit diagonal implied) and U, padded
-// with the identity beyond nbe.
-template <int NB>
-__global__ __launch_bounds__(64) void gj_diag_kernel(const double *__restrict__ A, unsigned n, unsigned k0, unsigned nbe,
-		double *__restrict__ dlu, double *__restrict__ diag, int *__restrict__ piv, double *__restrict__ scal)
-{
-	__shared__ double lb[NB];
-	const int lane = threadIdx.x;
-	double col[NB];
-#pragma unroll
-	for(int r = 0; r < NB; r++)
-		col[r] = (lane < (int)nbe && r < (int)nbe) ? A[(size_t)(k0 + lane) * n + k0 + r] : (r == lane ? 1.0 : 0.0);
-	bool bad = false;
-	for(int g = 0; g < (int)nbe; g++){
-		if(lane == g){
-			double d = 0;
-#pragma unroll
-			for(int r = 0; r < NB; r++) if(r == g) d = col[r];
-			const double inv = 1.0 / d;
-#pragma unroll
-			for(int r = 0; r < NB; r++)
-				if(r > g){
-					col[r] *= inv;
-					lb[r] = col[r];
-					bad |= !(fabs(col[r]) <= 1.0);
-				}
-			diag[k0 + g] = d;
-			piv[k0 + g] = (int)(k0 + g);
-		}
-		__syncthreads();
-		if(lane > g && lane < NB){
-			double u = 0;
-#pragma unroll
-			for(int r = 0; r < NB; r++) if(r == g) u = col[r];
-#pragma unroll
-			for(int r = 0; r < NB; r++) if(r > g) col[r] -= lb[r] * u;
-		}
-		__syncthreads();
-	}
-	if(lane < NB){
-#pragma unroll
-		for(int r = 0; r < NB; r++) dlu[lane * NB + r] = col[r];
-	}
-	if(bad) scal[SCAL_GJ_FLAG] = 1;
-}
-
-// Transform columns E[:, K] for every row (see header); one thread per row.
+// Block step, part 1 (grid over rows, 256 threads per workgroup).
+// Every workgroup first inverts the diagonal block D = A[K,K] by Gauss-Jordan without row
+// exchanges: thread t owns the four entries (r, 4q..4q+3), r = t/8, q = t%8, in registers; at
+// step g the owners of row g and of column g publish them to LDS (double-buffered by step
+// parity, one barrier per step).  D^{-1} and the LU multipliers (rows below the pivot) end in
+// LDS; workgroup 0 records the pivots and checks |L| <= 1.  Then every thread forms the
+// transform row of one matrix row i:
+//   i in K:      E[i,:] = D^{-1}[i-k0,:]
+//   i not in K:  E[i,:] = -A[i,K] D^{-1};  rows below K check their LU multipliers
+//                l = A[i,K] U^{-1} = -E[i,:] L  (|l| <= 1).
 template <int NB>
 __global__ __launch_bounds__(256) void gj_panel_np_kernel(const double *__restrict__ A, double *__restrict__ An,
-		unsigned n, unsigned k0, unsigned nbe, const double *__restrict__ dlu, double *__restrict__ scal)
+		unsigned n, unsigned k0, unsigned nbe, double *__restrict__ diag, int *__restrict__ piv, double *__restrict__ scal)
 {
-	__shared__ double lu[NB * NB];
-	for(int e = threadIdx.x; e < NB * NB; e += 256) lu[e] = dlu[e];
+	static_assert(NB == 32, "one 32 x 32 block per 256 threads");
+	__shared__ double dinv[NB][NB + 1];
+	__shared__ double lm[NB][NB + 1];        // lm[r][g]: multiplier of row r at step g (r > g)
+	__shared__ double rowb[2][NB], colb[2][NB];
+	const int tid = threadIdx.x;
+	const int r = tid >> 3, c0 = (tid & 7) * 4;
+	double v[4];
+#pragma unroll
+	for(int q = 0; q < 4; q++){
+		const int cc = c0 + q;
+		v[q] = (r < (int)nbe && cc < (int)nbe) ? A[(size_t)(k0 + cc) * n + k0 + r] : (r == cc ? 1.0 : 0.0);
+		lm[r][cc] = 0;
+	}
+	bool bad = false;
+	for(int g = 0; g < (int)nbe; g++){
+		const int b = g & 1;
+		if(r == g){
+#pragma unroll
+			for(int q = 0; q < 4; q++) rowb[b][c0 + q] = v[q];
+		}
+		if(g >= c0 && g < c0 + 4){
+#pragma unroll
+			for(int q = 0; q < 4; q++) if(c0 + q == g) colb[b][r] = v[q];
+		}
+		__syncthreads();
+		const double d = rowb[b][g];
+		const double inv = 1.0 / d;
+		if(r == g){
+#pragma unroll
+			for(int q = 0; q < 4; q++) v[q] = (c0 + q == g) ? inv : v[q] * inv;
+		}else{
+			const double fi = colb[b][r] * inv;
+#pragma unroll
+			for(int q = 0; q < 4; q++) v[q] = (c0 + q == g) ? -fi : v[q] - fi * rowb[b][c0 + q];
+			if(r > g && g >= c0 && g < c0 + 4){
+				lm[r][g] = fi;
+				bad |= !(fabs(fi) <= 1.0);
+			}
+		}
+		if(tid == 0 && blockIdx.x == 0){ diag[k0 + g] = d; piv[k0 + g] = (int)(k0 + g); }
+	}
+#pragma unroll
+	for(int q = 0; q < 4; q++) dinv[r][c0 + q] = v[q];
+	if(bad && blockIdx.x == 0) scal[SCAL_GJ_FLAG] = 1;
 	__syncthreads();
-	const unsigned i = blockIdx.x * 256 + threadIdx.x;
+	const unsigned i = blockIdx.x * 256 + tid;
 	if(i >= n) return;
 	const bool inK = i >= k0 && i < k0 + nbe;
-	double y[NB];
+	double e[NB];
+	if(inK){
 #pragma unroll
-	for(int j = 0; j < NB; j++){
-		double r = 0;
-		if(j < (int)nbe) r = inK ? ((int)(i - k0) == j ? 1.0 : 0.0) : A[(size_t)(k0 + j) * n + i];
-		y[j] = r;
-	}
-	// y <- y U^{-1}  (U[m][j] = lu[j*NB + m], m <= j)
+		for(int j = 0; j < NB; j++) e[j] = dinv[i - k0][j];
+	}else{
+		double a[NB];
 #pragma unroll
-	for(int j = 0; j < NB; j++){
-		double s = y[j];
+		for(int m = 0; m < NB; m++) a[m] = m < (int)nbe ? A[(size_t)(k0 + m) * n + i] : 0.0;
 #pragma unroll
-		for(int m = 0; m < j; m++) s -= y[m] * lu[j * NB + m];
-		y[j] = s / lu[j * NB + j];
-	}
-	if(i >= k0 + nbe){
-		bool bad = false;
+		for(int j = 0; j < NB; j++){
+			double s = 0;
 #pragma unroll
-		for(int j = 0; j < NB; j++) bad |= !(fabs(y[j]) <= 1.0);
-		if(bad) scal[SCAL_GJ_FLAG] = 1;
-	}
-	// e <- y L^{-1}  (L[m][j] = lu[j*NB + m], m > j, unit diagonal)
+			for(int m = 0; m < NB; m++) s += a[m] * dinv[m][j];
+			e[j] = -s;
+		}
+		if(i >= k0 + nbe){
+			bool bad = false;
 #pragma unroll
-	for(int j = NB - 1; j >= 0; j--){
-		double s = y[j];
+			for(int j = 0; j < NB; j++){
+				double y = e[j];
 #pragma unroll
-		for(int m = j + 1; m < NB; m++) s -= y[m] * lu[j * NB + m];
-		y[j] = s;
+				for(int m = j + 1; m < NB; m++) y += e[m] * lm[m][j];
+				bad |= !(fabs(y) <= 1.0);
+			}
+			if(bad) scal[SCAL_GJ_FLAG] = 1;
+		}
 	}
 #pragma unroll
 	for(int j = 0; j < NB; j++)
-		if(j < (int)nbe) An[(size_t)(k0 + j) * n + i] = inK ? y[j] : -y[j];
+		if(j < (int)nbe) An[(size_t)(k0 + j) * n + i] = e[j];
 }
 
-// Rank-nb update of every column outside K (64 x 64 tiles, 4 x 4 outputs per thread).
+// Block step, part 2: rank-nb update of every column outside K (32 x 32 tiles, 2 x 2 per thread).
 template <int NB>
 __global__ __launch_bounds__(256) void gj_update_kernel(const double *__restrict__ A, double *__restrict__ An,
 		unsigned n, unsigned k0, unsigned nbe)
 {
-	__shared__ double Y[NB][64 + 1];      // rows K of this column tile
-	__shared__ double E[NB][64 + 1];      // transform columns of this row tile (transposed)
-	const unsigned row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
+	__shared__ double Y[NB][32 + 1];      // rows K of this column tile
+	__shared__ double E[NB][32 + 1];      // transform columns of this row tile (transposed)
+	const unsigned row0 = blockIdx.x * 32, col0 = blockIdx.y * 32;
 	const int tid = threadIdx.x;
-	if(col0 >= k0 && col0 + 64 <= k0 + nbe) return;   // tile entirely inside K
-	for(int e = tid; e < NB * 64; e += 256){
-		const int kk = e / 64, cc = e % 64;
+	if(col0 >= k0 && col0 + 32 <= k0 + nbe) return;   // tile entirely inside K
+	for(int q = tid; q < NB * 32; q += 256){
+		const int kk = q / 32, cc = q % 32;
 		const unsigned col = col0 + cc;
 		Y[kk][cc] = (kk < (int)nbe && col < n) ? A[(size_t)col * n + k0 + kk] : 0.0;
 		const unsigned row = row0 + cc;
 		E[kk][cc] = (kk < (int)nbe && row < n) ? An[(size_t)(k0 + kk) * n + row] : 0.0;
 	}
 	__syncthreads();
-	const int tr = tid & 15, tc = tid >> 4;           // rows tr + 16 i, columns tc + 16 j
-	double acc[4][4];
+	const int tr = tid & 31, tc = tid >> 5;           // row tr, columns tc + 8 j
+	const unsigned row = row0 + tr;
+	const bool rowK = row >= k0 && row < k0 + nbe;
+	double acc[4];
 #pragma unroll
-	for(int i = 0; i < 4; i++){
-		const unsigned row = row0 + tr + 16 * i;
-		const bool inK = row >= k0 && row < k0 + nbe;
-#pragma unroll
-		for(int j = 0; j < 4; j++){
-			const unsigned col = col0 + tc + 16 * j;
-			acc[i][j] = (row < n && col < n && !inK) ? A[(size_t)col * n + row] : 0.0;
-		}
+	for(int j = 0; j < 4; j++){
+		const unsigned col = col0 + tc + 8 * j;
+		acc[j] = (row < n && col < n && !rowK) ? A[(size_t)col * n + row] : 0.0;
 	}
 #pragma unroll 8
 	for(int kk = 0; kk < NB; kk++){
-		double e[4], y[4];
+		const double ev = E[kk][tr];
 #pragma unroll
-		for(int i = 0; i < 4; i++) e[i] = E[kk][tr + 16 * i];
-#pragma unroll
-		for(int j = 0; j < 4; j++) y[j] = Y[kk][tc + 16 * j];
-#pragma unroll
-		for(int i = 0; i < 4; i++)
-#pragma unroll
-			for(int j = 0; j < 4; j++) acc[i][j] += e[i] * y[j];
+		for(int j = 0; j < 4; j++) acc[j] += ev * Y[kk][tc + 8 * j];
 	}
+	if(row >= n) return;
 #pragma unroll
 	for(int j = 0; j < 4; j++){
-		const unsigned col = col0 + tc + 16 * j;
+		const unsigned col = col0 + tc + 8 * j;
 		if(col >= n || (col >= k0 && col < k0 + nbe)) continue;
-#pragma unroll
-		for(int i = 0; i < 4; i++){
-			const unsigned row = row0 + tr + 16 * i;
-			if(row < n) An[(size_t)col * n + row] = acc[i][j];
-		}
+		An[(size_t)col * n + row] = acc[j];
 	}
 }
 
@@ -294,13 +283,11 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M)
 	const double *cur = M;
 	double *buf[2] = {c->d_gj[0], c->d_gj[1]};
 	int which = 0;
-	const dim3 ugrid((n + 63) / 64, (n + 63) / 64);
+	const dim3 ugrid((n + 31) / 32, (n + 31) / 32);
 	for(unsigned k0 = 0; k0 < n; k0 += GJ_NB){
 		const unsigned nbe = std::min<unsigned>(GJ_NB, n - k0);
 		double *next = buf[which];
-		gj_diag_kernel<GJ_NB><<<1, 64, 0, c->stream>>>(cur, n, k0, nbe, c->d_dlu, c->d_diag, c->d_piv, c->d_scal);
-		EPHMC_LAUNCHED();
-		gj_panel_np_kernel<GJ_NB><<<(n + 255) / 256, 256, 0, c->stream>>>(cur, next, n, k0, nbe, c->d_dlu, c->d_scal);
+		gj_panel_np_kernel<GJ_NB><<<(n + 255) / 256, 256, 0, c->stream>>>(cur, next, n, k0, nbe, c->d_diag, c->d_piv, c->d_scal);
 		EPHMC_LAUNCHED();
 		gj_update_kernel<GJ_NB><<<ugrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe);
 		EPHMC_LAUNCHED();
