@@ -337,7 +337,7 @@ int ephmc_destroy(ephmc_ctx *c)
 	if(c->stream) hipStreamSynchronize(c->stream);
 	void *bufs[] = {c->d_nb, c->d_link, c->d_coef64, c->d_coef32, c->d_ck, c->d_seg, c->d_partial[0], c->d_partial[1], c->d_Tfin,
 		c->d_Tfin32, c->d_gj[0], c->d_gj[1], c->d_piv, c->d_src, c->d_diag, c->d_dlu, c->d_scal, c->d_twiddle, c->d_f2, c->d_red,
-		c->d_x, c->d_p, c->d_pdot, c->d_xnew, c->d_normals, c->d_minv, c->d_mhat};
+		c->d_x, c->d_p, c->d_pdot, c->d_xnew, c->d_normals, c->d_minv, c->d_mhat, c->d_Rt0, c->d_R0t, c->d_ccg};
 	for(void *b : bufs) if(b) hipFree(b);
 	for(int s = 0; s < 2; s++){
 		if(c->h_stage[s]) hipHostFree(c->h_stage[s]);

@@ -107,6 +107,9 @@ struct ephmc_ctx {
 	double *d_red;                    // reduction partials
 	unsigned red_blocks;
 
+	// correlator (allocated on first use): R_t0, R_0t for every slice, g_ab(tau)
+	double *d_Rt0, *d_R0t, *d_ccg;
+
 	// chain state
 	double *d_x, *d_p, *d_pdot, *d_xnew, *d_minv, *d_mhat, *d_normals;
 	double *h_stage[2];               // pinned staging for normals

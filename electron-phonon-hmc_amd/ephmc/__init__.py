@@ -72,6 +72,7 @@ def _load():
         "ephmc_chain_get_minv": [vp, dp],
         "ephmc_chain_get_logdet": [vp, dp],
         "ephmc_chain_buffers": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)],
+        "ephmc_cc_correlator": [vp, vp, vp, C.c_double, dp],
         "ephmc_tl_hist": [vp, dp, C.c_uint, dp, C.c_double, C.c_uint, dp, dp],
         "ephmc_gj_fallbacks": [vp, C.POINTER(C.c_ulong)],
         "ephmc_timing": [vp, C.c_int],
@@ -276,6 +277,14 @@ class Context:
         v = C.c_double()
         check(LIB.ephmc_chain_get_logdet(self.h, C.byref(v)))
         return v.value
+
+    # -- measurement ---------------------------------------------------------------------
+    def cc_correlator(self, x, Minv, logdet=0.0):
+        """g_ab(tau), shape (nt, 3, 3), of field x with M_inv = (1 + B..B)^{-1}."""
+        dx, dm = self.array(self.dim, x), self.array(self.mat, Minv)
+        out = np.zeros(self.nt * 9)
+        check(LIB.ephmc_cc_correlator(self.h, dx.ptr, dm.ptr, logdet, _dp(out)))
+        return out.reshape(self.nt, 3, 3)
 
     def gj_fallbacks(self):
         v = C.c_ulong()

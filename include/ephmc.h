@@ -138,6 +138,12 @@ int ephmc_chain_get_logdet(ephmc_ctx *ctx, double *logdet_host);
 /* device pointers of the chain state (valid until ephmc_destroy) */
 int ephmc_chain_buffers(ephmc_ctx *ctx, double **x_dev, double **p_dev, double **minv_dev);
 
+/* ---- measurement -------------------------------------------------------------------------- */
+/* Current-current correlator g_ab(tau) (nt x 3 x 3, tau-major) of field x with
+ * M_inv = (1 + B..B)^{-1}; replaces cc_correlator (organic_dense_matrix.c:451-524).  FP64.
+ * Allocates 2 nt N^2 doubles of device memory on first use.  Waits. */
+int ephmc_cc_correlator(ephmc_ctx *ctx, const double *x_dev, const double *M_inv_dev, double logdet, double *greens_host);
+
 /* ---- (e) transient localisation ---------------------------------------------------------- */
 /* Batched disorder samples; replaces hist_trans_loc x meas (organic_trans_loc.c:25-83,
  * :101-105) with the normals of each sample given (3N per sample, sample-major).
