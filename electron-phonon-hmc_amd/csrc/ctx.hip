@@ -309,7 +309,7 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 			return fail(EPHMC_ERR_ALLOC);
 		}
 		if(hipEventCreateWithFlags(&c->stage_ev[s], hipEventDisableTiming) != hipSuccess){ set_error("event"); return fail(EPHMC_ERR_HIP); }
-		hipEventRecord(c->stage_ev[s], c->stream);
+		(void)hipEventRecord(c->stage_ev[s], c->stream);
 	}
 	std::vector<double> tw(2 * (size_t)c->nt), f2(c->nt);
 	const double pi = 3.141592653589793;
@@ -334,17 +334,17 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 int ephmc_destroy(ephmc_ctx *c)
 {
 	if(!c) return EPHMC_OK;
-	if(c->stream) hipStreamSynchronize(c->stream);
+	if(c->stream) (void)hipStreamSynchronize(c->stream);
 	void *bufs[] = {c->d_nb, c->d_link, c->d_coef64, c->d_coef32, c->d_ck, c->d_seg, c->d_partial[0], c->d_partial[1], c->d_Tfin,
 		c->d_Tfin32, c->d_gj[0], c->d_gj[1], c->d_piv, c->d_src, c->d_diag, c->d_dlu, c->d_scal, c->d_twiddle, c->d_f2, c->d_red,
 		c->d_x, c->d_p, c->d_pdot, c->d_xnew, c->d_normals, c->d_minv, c->d_mhat, c->d_Rt0, c->d_R0t, c->d_ccg};
-	for(void *b : bufs) if(b) hipFree(b);
+	for(void *b : bufs) if(b) (void)hipFree(b);
 	for(int s = 0; s < 2; s++){
-		if(c->h_stage[s]) hipHostFree(c->h_stage[s]);
-		if(c->stage_ev[s]) hipEventDestroy(c->stage_ev[s]);
+		if(c->h_stage[s]) (void)hipHostFree(c->h_stage[s]);
+		if(c->stage_ev[s]) (void)hipEventDestroy(c->stage_ev[s]);
 	}
-	for(auto &kv : c->timers) for(hipEvent_t e : kv.second.ev) hipEventDestroy(e);
-	if(c->own_stream && c->stream) hipStreamDestroy(c->stream);
+	for(auto &kv : c->timers) for(hipEvent_t e : kv.second.ev) (void)hipEventDestroy(e);
+	if(c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
 	delete c;
 	return EPHMC_OK;
 }
@@ -587,7 +587,7 @@ int ephmc_timing(ephmc_ctx *c, int enable)
 {
 	if(!c){ set_error("null context"); return EPHMC_ERR_ARG; }
 	EPHMC_HIP(hipStreamSynchronize(c->stream));
-	for(auto &kv : c->timers) for(hipEvent_t e : kv.second.ev) hipEventDestroy(e);
+	for(auto &kv : c->timers) for(hipEvent_t e : kv.second.ev) (void)hipEventDestroy(e);
 	c->timers.clear();
 	c->timing = enable != 0;
 	return EPHMC_OK;
