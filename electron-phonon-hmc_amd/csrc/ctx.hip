@@ -339,6 +339,7 @@ int ephmc_destroy(ephmc_ctx *c)
 		c->d_Tfin32, c->d_gj[0], c->d_gj[1], c->d_piv, c->d_src, c->d_diag, c->d_dlu, c->d_scal, c->d_twiddle, c->d_f2, c->d_red,
 		c->d_x, c->d_p, c->d_pdot, c->d_xnew, c->d_normals, c->d_minv, c->d_mhat, c->d_Rt0, c->d_R0t, c->d_ccg};
 	for(void *b : bufs) if(b) (void)hipFree(b);
+	tl_release(c);
 	for(int s = 0; s < 2; s++){
 		if(c->h_stage[s]) (void)hipHostFree(c->h_stage[s]);
 		if(c->stage_ev[s]) (void)hipEventDestroy(c->stage_ev[s]);
@@ -564,14 +565,6 @@ int ephmc_chain_buffers(ephmc_ctx *c, double **x, double **p, double **minv)
 	if(p) *p = c->d_p;
 	if(minv) *minv = c->d_minv;
 	return EPHMC_OK;
-}
-
-int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamples, const double *contr_mat, double width,
-		unsigned nbins, double *greens, double *results)
-{
-	(void)c; (void)normals; (void)nsamples; (void)contr_mat; (void)width; (void)nbins; (void)greens; (void)results;
-	set_error("transient localisation is not provided by this build yet");
-	return EPHMC_ERR_UNSUPPORTED;
 }
 
 int ephmc_gj_fallbacks(ephmc_ctx *c, unsigned long *count)

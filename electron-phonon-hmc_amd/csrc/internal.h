@@ -63,6 +63,20 @@ struct TimerGroup {
 	unsigned long calls;
 };
 
+// Transient-localisation workspace for a batch of cap_b disorder samples (allocated on first use).
+struct TlWork {
+	unsigned cap_b = 0, cap_nbins = 0;
+	size_t rot_cap = 0;               // rotations recorded per sample
+	unsigned sw_cap = 0;              // sweeps recorded per sample
+	double *A = nullptr, *Z = nullptr, *W0 = nullptr, *W1 = nullptr, *V = nullptr;   // [B][N][N]
+	double2 *rot = nullptr;           // [B][rot_cap] (c, s)
+	int4 *sw = nullptr;               // [B][sw_cap] (first rotation, lowest, highest position)
+	unsigned *nsw = nullptr;
+	int *status = nullptr, *perm = nullptr;
+	double *x = nullptr, *d = nullptr, *e = nullptr, *tau = nullptr, *ev = nullptr, *evs = nullptr, *ems = nullptr;
+	double *res = nullptr, *part = nullptr, *acc = nullptr;
+};
+
 } // namespace ephmc
 
 struct ephmc_ctx {
@@ -109,6 +123,9 @@ struct ephmc_ctx {
 
 	// correlator (allocated on first use): R_t0, R_0t for every slice, g_ab(tau)
 	double *d_Rt0, *d_R0t, *d_ccg;
+
+	// transient localisation
+	ephmc::TlWork tl;
 
 	// chain state
 	double *d_x, *d_p, *d_pdot, *d_xnew, *d_minv, *d_mhat, *d_normals;
@@ -168,5 +185,8 @@ int fa_energy(ephmc_ctx *c, const double *p, double *scal_out);
 int boson_action(ephmc_ctx *c, const double *x, double *scal_out);
 int hamilton_dev(ephmc_ctx *c, const double *x, const double *p, int logdet_slot, int out_slot);
 int metropolis(ephmc_ctx *c, double uniform);
+
+// tl.hip
+void tl_release(ephmc_ctx *c);
 
 } // namespace ephmc

@@ -1,0 +1,754 @@
+// tl.hip -- transient localisation with static disorder on the GPU (FP64), replacing
+// hist_trans_loc x meas (organic_trans_loc.c:25-83, :101-105).
+//
+// Per disorder sample s (3N normals x_s, drawn by the host in the reference's order):
+//   H = -mu + K(x_s)                     (construct_hamilton with -mu, organic_dense_matrix.c:218-224)
+//   H = V diag(E) V^T                    (LAPACKE_dsyev, organic_trans_loc.c:40)
+//   for all pairs E_j >= E_i:            (organic_trans_loc.c:45-78)
+//     proj_a = v_j . I_a v_i,  I_nm = sum_c (sum_a C[c,a] proj_a)^2 = sum_c (v_j . G_c v_i)^2,
+//     G_c = sum_a C[c,a] I_a (antisymmetric bond currents, apply_current_to_v :226-249),
+//     greens[(unsigned)((E_j - E_i)/width + .5)] += (e^{-E_i} + e^{-E_j}) I_nm / (width N)
+//   results[s] = sum_i e^{-E_i} / N
+//
+// Kernels, a batch of B samples at a time (one workgroup per sample unless stated):
+//   tl_build        dense H (column c contiguous; H is symmetric)
+//   tl_tridiag      Householder reduction to tridiagonal form (lower, one column at a time) on
+//                   the full symmetric matrix; the rank-2 update of step j is fused with the
+//                   matrix-vector product of step j+1, so the trailing block is read and written
+//                   once per step.  Reflectors v_j overwrite column j below the diagonal.
+//   tl_ql_scalar    implicit QL with Wilkinson shifts on (d, e) (the scalar recurrence of the
+//                   EISPACK tql2 method, one lane per sample), recording every rotation (c, s)
+//                   and the position range of every sweep
+//   tl_ql_apply     the recorded rotations applied to Z = I, thread = row of Z: G sweeps per
+//                   pass with a register window of G+1 columns (a wavefront: sweep g runs one
+//                   position behind sweep g-1), so Z is read and written once per G sweeps
+//   tl_back         V = H_0 H_1 ... H_{N-3} Z: a wave holds CW columns of Z in registers and
+//                   applies every reflector (last first)
+//   tl_sort         eigenvalue ranks, e^{-E}, the partition sum
+//   tl_current      V sorted by eigenvalue and W_c = G_c V (gather form of the bond currents)
+//   tl_pairs        64 x 64 tiles of J_c = V^T W_c (VALU FP64) with the binning in the
+//                   epilogue; tiles that cannot hold a pair E_j >= E_i are skipped.  Each tile
+//                   bins its 4096 pairs in a fixed order (no atomics), so bins are
+//                   reproducible run to run
+//   tl_reduce       fixed-order sum of the tile histograms into the accumulated bins
+// Pairs whose bin index is >= nbins are dropped (the reference writes them past the end of its
+// greens array).
+#include "internal.h"
+#include <algorithm>
+#include <cstring>
+
+namespace ephmc {
+
+constexpr int TL_BT = 1024;     // tridiagonalisation / sort workgroup
+constexpr int TL_G = 16;        // sweeps per wavefront pass
+constexpr int TL_TILE = 64;
+constexpr int TL_KC = 16;       // rows of V / W per LDS stage of the pair tiles
+constexpr unsigned TL_NONE = 0xffffffffu;
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+	for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	return v;          // identical in every lane (a + b == b + a)
+}
+
+// deterministic block sum; every thread returns the same value
+template <int BT>
+__device__ double block_sum(double v, double *red)
+{
+	v = wave_sum(v);
+	if((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+	__syncthreads();
+	double t = 0;
+#pragma unroll
+	for(int q = 0; q < BT / 64; q++) t += red[q];
+	__syncthreads();
+	return t;
+}
+
+// ---- H ----------------------------------------------------------------------------------
+__global__ void tl_build_kernel(const double *__restrict__ xs, const int *__restrict__ nb, const unsigned *__restrict__ link,
+		double t0, double t1, double t2, double k0, double k1, double k2, double diag, unsigned n, double *__restrict__ Aall)
+{
+	const unsigned s = blockIdx.y;
+	const double *x = xs + (size_t)s * 3 * n;
+	double *A = Aall + (size_t)s * n * n;
+	const size_t nn = (size_t)n * n;
+	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x){
+		const unsigned c = (unsigned)(e / n), r = (unsigned)(e - (size_t)c * n);
+		double h = r == c ? diag : 0.0;
+#pragma unroll
+		for(int d = 0; d < 6; d++){
+			if((unsigned)nb[d * n + r] == c){
+				const int k = d >> 1;
+				const double tk = k == 0 ? t0 : (k == 1 ? t1 : t2), kk = k == 0 ? k0 : (k == 1 ? k1 : k2);
+				h += tk - kk * x[link[d * n + r]];
+			}
+		}
+		A[e] = h;
+	}
+}
+
+// ---- Householder tridiagonalisation --------------------------------------------------------
+// x = A[j][j+1 .. n-1] in vb; makes v (v[j+1] = 1) in vb and in column j of A, returns tau.
+template <int BT, int RPT>
+__device__ double tl_reflector(double *A, unsigned n, unsigned j, double *vb, double *red, double *e, double *tau)
+{
+	double ss = 0;
+#pragma unroll
+	for(int q = 0; q < RPT; q++){
+		const unsigned r = threadIdx.x + q * BT;
+		if(r >= j + 2 && r < n) ss += vb[r] * vb[r];
+	}
+	ss = block_sum<BT>(ss, red);
+	const double alpha = vb[j + 1];
+	double tj = 0, beta = alpha, scale = 0;
+	if(ss != 0){
+		beta = -copysign(hypot(alpha, sqrt(ss)), alpha);
+		tj = (beta - alpha) / beta;
+		scale = 1.0 / (alpha - beta);
+	}
+#pragma unroll
+	for(int q = 0; q < RPT; q++){
+		const unsigned r = threadIdx.x + q * BT;
+		if(r >= j + 1 && r < n){
+			const double v = r == j + 1 ? 1.0 : vb[r] * scale;
+			vb[r] = v;
+			A[(size_t)j * n + r] = v;
+		}
+	}
+	if(threadIdx.x == 0){
+		e[j] = beta;
+		tau[j] = tj;
+	}
+	__syncthreads();
+	return tj;
+}
+
+template <int BT, int RPT>
+__global__ __launch_bounds__(BT) void tl_tridiag_kernel(double *__restrict__ Aall, unsigned n, double *__restrict__ dall,
+		double *__restrict__ eall, double *__restrict__ tall)
+{
+	extern __shared__ double sh[];
+	double *vA = sh, *vB = sh + n, *w = sh + 2 * n, *red = sh + 3 * n;
+	const unsigned s = blockIdx.x, tid = threadIdx.x;
+	double *A = Aall + (size_t)s * n * n, *d = dall + (size_t)s * n, *e = eall + (size_t)s * n, *tau = tall + (size_t)s * n;
+	if(n <= 2){
+		if(tid == 0){
+			d[0] = A[0];
+			e[0] = 0;
+			tau[0] = 0;
+			if(n == 2){
+				d[1] = A[3];
+				e[0] = A[1];
+				e[1] = 0;
+				tau[1] = 0;
+			}
+		}
+		return;
+	}
+	double p[RPT];
+	// step 0: reflector of column 0 and p = tau A22 v
+#pragma unroll
+	for(int q = 0; q < RPT; q++){
+		const unsigned r = tid + q * BT;
+		if(r >= 1 && r < n) vA[r] = A[r];
+	}
+	if(tid == 0) d[0] = A[0];
+	double tj = tl_reflector<BT, RPT>(A, n, 0, vA, red, e, tau);
+	double kk = 0;
+#pragma unroll
+	for(int q = 0; q < RPT; q++){
+		const unsigned r = tid + q * BT;
+		p[q] = 0;
+		if(r >= 1 && r < n){
+			double acc = 0;
+			for(unsigned c = 1; c < n; c++) acc += A[(size_t)c * n + r] * vA[c];
+			p[q] = tj * acc;
+			kk += p[q] * vA[r];
+		}
+	}
+	kk = 0.5 * tj * block_sum<BT>(kk, red);
+#pragma unroll
+	for(int q = 0; q < RPT; q++){
+		const unsigned r = tid + q * BT;
+		if(r >= 1 && r < n) w[r] = p[q] - kk * vA[r];
+	}
+	__syncthreads();
+	double *v = vA, *vn = vB;
+	for(unsigned j = 0; j + 3 < n; j++){
+		// (v, w) of step j act on rows/columns j+1 .. n-1.  Column j+1 first:
+		const double wj1 = w[j + 1], vj1 = v[j + 1];
+#pragma unroll
+		for(int q = 0; q < RPT; q++){
+			const unsigned r = tid + q * BT;
+			if(r >= j + 1 && r < n){
+				const size_t o = (size_t)(j + 1) * n + r;
+				const double a = A[o] - (v[r] * wj1 + w[r] * vj1);
+				A[o] = a;
+				if(r == j + 1) d[j + 1] = a;
+				else vn[r] = a;
+			}
+		}
+		tj = tl_reflector<BT, RPT>(A, n, j + 1, vn, red, e, tau);
+		// rank-2 update of step j on rows/columns j+2 .. n-1, fused with p = tau A22 vn
+		kk = 0;
+#pragma unroll
+		for(int q = 0; q < RPT; q++){
+			const unsigned r = tid + q * BT;
+			p[q] = 0;
+			if(r >= j + 2 && r < n){
+				const double vr = v[r], wr = w[r];
+				double acc = 0;
+				unsigned c = j + 2;
+				for(; c + 4 <= n; c += 4){
+					const size_t o = (size_t)c * n + r;
+					const double a0 = A[o] - (vr * w[c] + wr * v[c]);
+					const double a1 = A[o + n] - (vr * w[c + 1] + wr * v[c + 1]);
+					const double a2 = A[o + 2 * (size_t)n] - (vr * w[c + 2] + wr * v[c + 2]);
+					const double a3 = A[o + 3 * (size_t)n] - (vr * w[c + 3] + wr * v[c + 3]);
+					A[o] = a0;
+					A[o + n] = a1;
+					A[o + 2 * (size_t)n] = a2;
+					A[o + 3 * (size_t)n] = a3;
+					acc += a0 * vn[c];
+					acc += a1 * vn[c + 1];
+					acc += a2 * vn[c + 2];
+					acc += a3 * vn[c + 3];
+				}
+				for(; c < n; c++){
+					const size_t o = (size_t)c * n + r;
+					const double a = A[o] - (vr * w[c] + wr * v[c]);
+					A[o] = a;
+					acc += a * vn[c];
+				}
+				p[q] = tj * acc;
+				kk += p[q] * vn[r];
+			}
+		}
+		kk = 0.5 * tj * block_sum<BT>(kk, red);
+#pragma unroll
+		for(int q = 0; q < RPT; q++){
+			const unsigned r = tid + q * BT;
+			if(r >= j + 2 && r < n) w[r] = p[q] - kk * vn[r];
+		}
+		double *t = v;
+		v = vn;
+		vn = t;
+		__syncthreads();
+	}
+	// (v, w) of step n-3 on the last 2 x 2 block
+	if(tid == 0){
+		const unsigned a = n - 2, b = n - 1;
+		d[a] = A[(size_t)a * n + a] - 2 * v[a] * w[a];
+		e[a] = A[(size_t)a * n + b] - (v[b] * w[a] + w[b] * v[a]);
+		d[b] = A[(size_t)b * n + b] - 2 * v[b] * w[b];
+		e[b] = 0;
+		tau[a] = 0;
+		tau[b] = 0;
+	}
+}
+
+// ---- implicit QL: scalar recurrence ----------------------------------------------------------
+// e[i] couples i and i+1.  Rotation (c, s) at position i acts on columns (i, i+1) of Z:
+// Z(:,i+1) <- s Z(:,i) + c Z(:,i+1),  Z(:,i) <- c Z(:,i) - s Z(:,i+1).  Sweep records: (first
+// rotation index, lowest position, highest position); positions run from high to low.
+__global__ __launch_bounds__(64) void tl_ql_scalar_kernel(const double *__restrict__ dall, const double *__restrict__ eall,
+		unsigned n, double *__restrict__ evall, double2 *__restrict__ rotall, size_t rot_cap, int4 *__restrict__ swall,
+		unsigned sw_cap, unsigned *__restrict__ nsw_all, int *__restrict__ status)
+{
+	extern __shared__ double sh[];
+	double *d = sh, *e = sh + n;
+	const unsigned s = blockIdx.x;
+	for(unsigned i = threadIdx.x; i < n; i += 64){
+		d[i] = dall[(size_t)s * n + i];
+		e[i] = eall[(size_t)s * n + i];
+	}
+	__syncthreads();
+	if(threadIdx.x != 0) return;
+	double2 *rot = rotall + (size_t)s * rot_cap;
+	int4 *sw = swall + (size_t)s * sw_cap;
+	const double eps = 2.220446049250313e-16;
+	size_t nrot = 0;
+	unsigned nsw = 0;
+	int st = 0;
+	e[n - 1] = 0;
+	for(unsigned l = 0; l < n && !st; l++){
+		int it = 0;
+		for(;;){
+			unsigned m = l;
+			for(; m + 1 < n; m++){
+				const double dd = fabs(d[m]) + fabs(d[m + 1]);
+				if(fabs(e[m]) <= eps * dd) break;
+			}
+			if(m == l) break;
+			if(++it > 60){ st = 1; break; }
+			if(nrot + (m - l) > rot_cap || nsw >= sw_cap){ st = 2; break; }
+			double g = (d[l + 1] - d[l]) / (2 * e[l]);
+			double r = hypot(g, 1.0);
+			g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+			double sn = 1, cs = 1, p = 0;
+			const size_t base = nrot;
+			int i = (int)m - 1;
+			bool broke = false;
+			for(; i >= (int)l; i--){
+				const double f = sn * e[i], b = cs * e[i];
+				r = hypot(f, g);
+				e[i + 1] = r;
+				if(r == 0){
+					d[i + 1] -= p;
+					e[m] = 0;
+					broke = true;
+					break;
+				}
+				sn = f / r;
+				cs = g / r;
+				g = d[i + 1] - p;
+				r = (d[i] - g) * sn + 2 * cs * b;
+				p = sn * r;
+				d[i + 1] = g + p;
+				g = cs * r - b;
+				rot[nrot++] = make_double2(cs, sn);
+			}
+			const int hi = (int)m - 1, lo = broke ? i + 1 : (int)l;
+			if(lo <= hi) sw[nsw++] = make_int4((int)base, lo, hi, 0);
+			if(broke) continue;
+			d[l] -= p;
+			e[l] = g;
+			e[m] = 0;
+		}
+	}
+	for(unsigned i = 0; i < n; i++) evall[(size_t)s * n + i] = d[i];
+	nsw_all[s] = nsw;
+	status[s] = st;
+}
+
+// ---- rotations applied to Z = I (thread = row, column-major Z) ------------------------------
+template <int G>
+__global__ __launch_bounds__(256) void tl_ql_apply_kernel(double *__restrict__ Zall, unsigned n, const double2 *__restrict__ rotall,
+		size_t rot_cap, const int4 *__restrict__ swall, unsigned sw_cap, const unsigned *__restrict__ nsw_all)
+{
+	const unsigned s = blockIdx.y;
+	const unsigned r = blockIdx.x * 256 + threadIdx.x;
+	const bool act = r < n;
+	double *Z = Zall + (size_t)s * n * n;
+	const double2 *rot = rotall + (size_t)s * rot_cap;
+	const int4 *sw = swall + (size_t)s * sw_cap;
+	if(act)
+		for(unsigned c = 0; c < n; c++) Z[(size_t)c * n + r] = c == r ? 1.0 : 0.0;
+	const unsigned nsw = nsw_all[s];
+	for(unsigned g0 = 0; g0 < nsw; g0 += G){
+		int lo[G], hi[G], base[G];
+		int P = -0x7fffffff, qend = 0x7fffffff;
+#pragma unroll
+		for(int k = 0; k < G; k++){
+			if(g0 + k < nsw){
+				const int4 q = sw[g0 + k];
+				base[k] = q.x;
+				lo[k] = q.y;
+				hi[k] = q.z;
+				P = max(P, hi[k] - k);
+				qend = min(qend, lo[k] - k);
+			}else{
+				base[k] = 0;
+				lo[k] = 1;
+				hi[k] = 0;
+			}
+		}
+		double W[G + 1];
+#pragma unroll
+		for(int k = 0; k <= G; k++){
+			const int col = P + k;
+			W[k] = (act && col < (int)n) ? Z[(size_t)col * n + r] : 0.0;
+		}
+		int q = P;
+		for(;;){
+#pragma unroll
+			for(int k = 0; k < G; k++){
+				const int pos = q + k;
+				if(pos >= lo[k] && pos <= hi[k]){
+					const double2 cs = rot[base[k] + (hi[k] - pos)];
+					const double a = W[k], f = W[k + 1];
+					W[k + 1] = cs.y * a + cs.x * f;
+					W[k] = cs.x * a - cs.y * f;
+				}
+			}
+			if(q <= qend) break;
+			const int col = q + G;
+			if(act && col < (int)n) Z[(size_t)col * n + r] = W[G];
+#pragma unroll
+			for(int k = G; k > 0; k--) W[k] = W[k - 1];
+			q--;
+			W[0] = (act && q >= 0) ? Z[(size_t)q * n + r] : 0.0;
+		}
+#pragma unroll
+		for(int k = 0; k <= G; k++){
+			const int col = q + k;
+			if(act && col >= 0 && col < (int)n) Z[(size_t)col * n + r] = W[k];
+		}
+	}
+}
+
+// ---- back-transformation V = H_0 ... H_{n-3} Z --------------------------------------------
+// 256 threads = 4 waves; wave w owns columns (blockIdx.x * 4 + w) * CW + c, c < CW; lane l
+// holds rows l + 64 q, q < RPL.
+template <int RPL, int CW>
+__global__ __launch_bounds__(256) void tl_back_kernel(const double *__restrict__ Aall, const double *__restrict__ tall,
+		double *__restrict__ Zall, unsigned n)
+{
+	const unsigned s = blockIdx.y, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const unsigned col0 = (blockIdx.x * 4 + wave) * CW;
+	const double *A = Aall + (size_t)s * n * n, *tau = tall + (size_t)s * n;
+	double *Z = Zall + (size_t)s * n * n;
+	double z[CW][RPL];
+#pragma unroll
+	for(int c = 0; c < CW; c++)
+#pragma unroll
+		for(int q = 0; q < RPL; q++){
+			const unsigned r = lane + 64 * q, col = col0 + c;
+			z[c][q] = (col < n && r < n) ? Z[(size_t)col * n + r] : 0.0;
+		}
+	for(int j = (int)n - 3; j >= 0; j--){
+		const double tj = tau[j];
+		if(tj == 0) continue;
+		const double *vj = A + (size_t)j * n;
+		const int qlo = (j + 1) >> 6;
+		double v[RPL], part[CW];
+#pragma unroll
+		for(int c = 0; c < CW; c++) part[c] = 0;
+#pragma unroll
+		for(int q = 0; q < RPL; q++){
+			const unsigned r = lane + 64 * q;
+			v[q] = 0;
+			if(q >= qlo){
+				v[q] = (r >= (unsigned)j + 1 && r < n) ? vj[r] : 0.0;
+#pragma unroll
+				for(int c = 0; c < CW; c++) part[c] += v[q] * z[c][q];
+			}
+		}
+#pragma unroll
+		for(int c = 0; c < CW; c++){
+			const double sc = tj * wave_sum(part[c]);
+#pragma unroll
+			for(int q = 0; q < RPL; q++)
+				if(q >= qlo) z[c][q] -= sc * v[q];
+		}
+	}
+#pragma unroll
+	for(int c = 0; c < CW; c++)
+#pragma unroll
+		for(int q = 0; q < RPL; q++){
+			const unsigned r = lane + 64 * q, col = col0 + c;
+			if(col < n && r < n) Z[(size_t)col * n + r] = z[c][q];
+		}
+}
+
+// ---- order, Boltzmann factors, partition sum -------------------------------------------------
+template <int BT>
+__global__ __launch_bounds__(BT) void tl_sort_kernel(const double *__restrict__ evall, unsigned n, double *__restrict__ evs_all,
+		double *__restrict__ ems_all, int *__restrict__ perm_all, double *__restrict__ res)
+{
+	extern __shared__ double sh[];
+	double *ev = sh, *red = sh + n;
+	const unsigned s = blockIdx.x;
+	for(unsigned i = threadIdx.x; i < n; i += BT) ev[i] = evall[(size_t)s * n + i];
+	__syncthreads();
+	for(unsigned i = threadIdx.x; i < n; i += BT){
+		const double ei = ev[i];
+		unsigned rank = 0;
+		for(unsigned j = 0; j < n; j++){
+			const double ej = ev[j];
+			rank += (ej < ei || (ej == ei && j < i)) ? 1u : 0u;
+		}
+		evs_all[(size_t)s * n + rank] = ei;
+		ems_all[(size_t)s * n + rank] = exp(-ei);
+		perm_all[(size_t)s * n + rank] = (int)i;
+	}
+	__syncthreads();
+	double z = 0;
+	for(unsigned i = threadIdx.x; i < n; i += BT) z += ems_all[(size_t)s * n + i];
+	z = block_sum<BT>(z, red);
+	if(threadIdx.x == 0) res[s] = z / n;
+}
+
+// ---- sorted eigenvectors and W_c = G_c V ---------------------------------------------------
+// (I_a v)[r] = f_a(r) v[nb(r,2a)] - f_a(nb(r,2a+1)) v[nb(r,2a+1)],  f_a(i) = t_a - kappa_a x[3i+a]
+__global__ __launch_bounds__(256) void tl_current_kernel(const double *__restrict__ Zall, const int *__restrict__ perm_all,
+		const double *__restrict__ xs, const int *__restrict__ nb, unsigned n, double t0, double t1, double t2,
+		double k0, double k1, double k2, double c00, double c01, double c02, double c10, double c11, double c12,
+		double *__restrict__ Vall, double *__restrict__ W0all, double *__restrict__ W1all)
+{
+	extern __shared__ double v[];
+	const unsigned s = blockIdx.y, jj = blockIdx.x;
+	const int src = perm_all[(size_t)s * n + jj];
+	const double *z = Zall + (size_t)s * n * n + (size_t)src * n;
+	const double *x = xs + (size_t)s * 3 * n;
+	for(unsigned r = threadIdx.x; r < n; r += blockDim.x){
+		const double a = z[r];
+		v[r] = a;
+		Vall[(size_t)s * n * n + (size_t)jj * n + r] = a;
+	}
+	__syncthreads();
+	const double tk[3] = {t0, t1, t2}, kk[3] = {k0, k1, k2};
+	const double C0[3] = {c00, c01, c02}, C1[3] = {c10, c11, c12};
+	for(unsigned r = threadIdx.x; r < n; r += blockDim.x){
+		double w0 = 0, w1 = 0;
+#pragma unroll
+		for(int a = 0; a < 3; a++){
+			const unsigned fw = (unsigned)nb[(2 * a) * n + r], bw = (unsigned)nb[(2 * a + 1) * n + r];
+			const double cur = (tk[a] - kk[a] * x[(size_t)r * 3 + a]) * v[fw] - (tk[a] - kk[a] * x[(size_t)bw * 3 + a]) * v[bw];
+			w0 += C0[a] * cur;
+			w1 += C1[a] * cur;
+		}
+		W0all[(size_t)s * n * n + (size_t)jj * n + r] = w0;
+		W1all[(size_t)s * n * n + (size_t)jj * n + r] = w1;
+	}
+}
+
+// ---- pair tiles: J_c(j, i) = v_j . W_c[:, i], binned --------------------------------------
+// grid (i-block, j-block, sample); 256 threads, thread (tj, ti) = (tid / 16, tid % 16) owns
+// j = j0 + 4 tj + a, i = i0 + 4 ti + b.
+__global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict__ Vall, const double *__restrict__ W0all,
+		const double *__restrict__ W1all, const double *__restrict__ evs_all, const double *__restrict__ ems_all,
+		unsigned n, double width, unsigned nbins, double *__restrict__ part)
+{
+	__shared__ double Vs[TL_KC][TL_TILE + 1], W0s[TL_KC][TL_TILE + 1], W1s[TL_KC][TL_TILE + 1];
+	__shared__ unsigned ebin[TL_TILE * TL_TILE];
+	__shared__ double eval[TL_TILE * TL_TILE];
+	__shared__ unsigned brange[2];
+	const unsigned ib = blockIdx.x, jb = blockIdx.y, s = blockIdx.z, nblk = gridDim.x;
+	const unsigned tid = threadIdx.x, tj = tid >> 4, ti = tid & 15;
+	const unsigned i0 = ib * TL_TILE, j0 = jb * TL_TILE;
+	const double *evs = evs_all + (size_t)s * n, *ems = ems_all + (size_t)s * n;
+	double *out = part + (((size_t)s * nblk + jb) * nblk + ib) * nbins;
+	// sorted order: E_j >= E_i needs j >= i except for equal eigenvalues across the blocks
+	const unsigned jlast = min(j0 + TL_TILE, n) - 1;
+	if(jb < ib && !(evs[jlast] >= evs[i0])){
+		for(unsigned b = tid; b < nbins; b += 256) out[b] = 0;
+		return;
+	}
+	const double *V = Vall + (size_t)s * n * n, *W0 = W0all + (size_t)s * n * n, *W1 = W1all + (size_t)s * n * n;
+	double acc0[4][4], acc1[4][4];
+#pragma unroll
+	for(int a = 0; a < 4; a++)
+#pragma unroll
+		for(int b = 0; b < 4; b++){
+			acc0[a][b] = 0;
+			acc1[a][b] = 0;
+		}
+	if(tid == 0){
+		brange[0] = TL_NONE;
+		brange[1] = 0;
+	}
+	for(unsigned r0 = 0; r0 < n; r0 += TL_KC){
+#pragma unroll
+		for(int q = 0; q < 4; q++){
+			const unsigned e = tid + q * 256, cc = e / TL_KC, rr = e % TL_KC;
+			const unsigned r = r0 + rr;
+			const unsigned jc = j0 + cc, ic = i0 + cc;
+			Vs[rr][cc] = (jc < n && r < n) ? V[(size_t)jc * n + r] : 0.0;
+			W0s[rr][cc] = (ic < n && r < n) ? W0[(size_t)ic * n + r] : 0.0;
+			W1s[rr][cc] = (ic < n && r < n) ? W1[(size_t)ic * n + r] : 0.0;
+		}
+		__syncthreads();
+#pragma unroll 4
+		for(int rr = 0; rr < TL_KC; rr++){
+			double va[4], w0[4], w1[4];
+#pragma unroll
+			for(int a = 0; a < 4; a++){
+				va[a] = Vs[rr][4 * tj + a];
+				w0[a] = W0s[rr][4 * ti + a];
+				w1[a] = W1s[rr][4 * ti + a];
+			}
+#pragma unroll
+			for(int a = 0; a < 4; a++)
+#pragma unroll
+				for(int b = 0; b < 4; b++){
+					acc0[a][b] += va[a] * w0[b];
+					acc1[a][b] += va[a] * w1[b];
+				}
+		}
+		__syncthreads();
+	}
+	const double wn = width * n;
+	unsigned bmin = TL_NONE, bmax = 0;
+#pragma unroll
+	for(int a = 0; a < 4; a++)
+#pragma unroll
+		for(int b = 0; b < 4; b++){
+			const unsigned j = j0 + 4 * tj + a, i = i0 + 4 * ti + b;
+			unsigned bin = TL_NONE;
+			double val = 0;
+			if(j < n && i < n && evs[j] >= evs[i]){
+				const double fr = (evs[j] - evs[i]) / width + 0.5;
+				if(fr < (double)nbins){
+					bin = (unsigned)fr;
+					const double inm = acc0[a][b] * acc0[a][b] + acc1[a][b] * acc1[a][b];
+					val = (ems[i] + ems[j]) * inm / wn;
+					bmin = min(bmin, bin);
+					bmax = max(bmax, bin);
+				}
+			}
+			ebin[tid * 16 + a * 4 + b] = bin;
+			eval[tid * 16 + a * 4 + b] = val;
+		}
+	if(bmin != TL_NONE){
+		atomicMin(&brange[0], bmin);
+		atomicMax(&brange[1], bmax);
+	}
+	__syncthreads();
+	const unsigned blo = brange[0], bhi = brange[1];
+	for(unsigned b = tid; b < nbins; b += 256){
+		double sum = 0;
+		if(blo != TL_NONE && b >= blo && b <= bhi)
+			for(unsigned e = 0; e < TL_TILE * TL_TILE; e++)
+				if(ebin[e] == b) sum += eval[e];
+		out[b] = sum;
+	}
+}
+
+__global__ void tl_reduce_kernel(const double *__restrict__ part, unsigned nparts, unsigned nbins, double *__restrict__ acc)
+{
+	const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
+	if(b >= nbins) return;
+	double s = 0;
+	for(unsigned t = 0; t < nparts; t++) s += part[(size_t)t * nbins + b];
+	acc[b] += s;
+}
+
+__global__ void tl_zero_kernel(double *a, unsigned n)
+{
+	const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < n) a[i] = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+static int tl_reserve(ephmc_ctx *c, unsigned B, unsigned nbins)
+{
+	TlWork &w = c->tl;
+	const unsigned n = c->ns, nblk = (n + TL_TILE - 1) / TL_TILE;
+	if(w.cap_b >= B && w.cap_nbins >= nbins) return EPHMC_OK;
+	tl_release(c);
+	const size_t mat = (size_t)n * n;
+	w.rot_cap = std::max<size_t>(3 * mat / 2, 64);
+	w.sw_cap = 16 * n + 64;
+	auto al = [&](void **p, size_t bytes) -> int {
+		if(hipMalloc(p, bytes) != hipSuccess){
+			(void)hipGetLastError();
+			set_error("transient-localisation workspace: hipMalloc(%zu) failed", bytes);
+			return EPHMC_ERR_ALLOC;
+		}
+		return EPHMC_OK;
+	};
+	int rc;
+	if((rc = al((void **)&w.A, B * mat * 8)) || (rc = al((void **)&w.Z, B * mat * 8)) || (rc = al((void **)&w.W0, B * mat * 8))
+		|| (rc = al((void **)&w.W1, B * mat * 8)) || (rc = al((void **)&w.V, B * mat * 8))
+		|| (rc = al((void **)&w.rot, B * w.rot_cap * sizeof(double2))) || (rc = al((void **)&w.sw, B * (size_t)w.sw_cap * sizeof(int4)))
+		|| (rc = al((void **)&w.nsw, B * sizeof(unsigned))) || (rc = al((void **)&w.status, B * sizeof(int)))
+		|| (rc = al((void **)&w.x, B * 3 * (size_t)n * 8)) || (rc = al((void **)&w.d, B * (size_t)n * 8))
+		|| (rc = al((void **)&w.e, B * (size_t)n * 8)) || (rc = al((void **)&w.tau, B * (size_t)n * 8))
+		|| (rc = al((void **)&w.ev, B * (size_t)n * 8)) || (rc = al((void **)&w.evs, B * (size_t)n * 8))
+		|| (rc = al((void **)&w.ems, B * (size_t)n * 8)) || (rc = al((void **)&w.perm, B * (size_t)n * sizeof(int)))
+		|| (rc = al((void **)&w.res, B * 8)) || (rc = al((void **)&w.part, B * (size_t)nblk * nblk * nbins * 8))
+		|| (rc = al((void **)&w.acc, (size_t)nbins * 8))){
+		tl_release(c);
+		return rc;
+	}
+	w.cap_b = B;
+	w.cap_nbins = nbins;
+	return EPHMC_OK;
+}
+
+void tl_release(ephmc_ctx *c)
+{
+	TlWork &w = c->tl;
+	void *bufs[] = {w.A, w.Z, w.W0, w.W1, w.V, w.rot, w.sw, w.nsw, w.status, w.x, w.d, w.e, w.tau, w.ev, w.evs, w.ems, w.perm,
+		w.res, w.part, w.acc};
+	for(void *b : bufs) if(b) (void)hipFree(b);
+	w = TlWork();
+}
+
+} // namespace ephmc
+
+using namespace ephmc;
+
+extern "C" int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamples, const double *contr_mat, double width,
+		unsigned nbins, double *greens, double *results)
+{
+	if(!c || !normals || !contr_mat || !greens || !results){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	if(nsamples == 0) return EPHMC_OK;
+	if(nbins == 0 || nbins > 65536){ set_error("nbins must be in [1, 65536], got %u", nbins); return EPHMC_ERR_ARG; }
+	if(!(width > 0)){ set_error("bin width must be positive, got %g", width); return EPHMC_ERR_ARG; }
+	const unsigned n = c->ns;
+	if(n > 2 * TL_BT){ set_error("transient localisation supports N <= %d sites", 2 * TL_BT); return EPHMC_ERR_ARG; }
+	const size_t mat = (size_t)n * n;
+	const unsigned nblk = (n + TL_TILE - 1) / TL_TILE;
+	// samples per batch: about 8 GB of workspace (8.5 N^2 doubles per sample), at most 256
+	const size_t per = (size_t)(8.5 * mat * 8) + (size_t)nblk * nblk * nbins * 8;
+	unsigned B = (unsigned)std::max<size_t>(1, ((size_t)8 << 30) / per);
+	B = std::min(B, 256u);
+	B = std::min(B, nsamples);
+	EPHMC_TRY(tl_reserve(c, B, nbins));
+	TlWork &w = c->tl;
+	EPHMC_TRY(timer_begin(c, "tl"));
+	tl_zero_kernel<<<(nbins + 255) / 256, 256, 0, c->stream>>>(w.acc, nbins);
+	EPHMC_LAUNCHED();
+	std::vector<int> status(B);
+	for(unsigned s0 = 0; s0 < nsamples; s0 += B){
+		const unsigned nb_ = std::min(B, nsamples - s0);
+		EPHMC_HIP(hipMemcpyAsync(w.x, normals + (size_t)s0 * 3 * n, (size_t)nb_ * 3 * n * 8, hipMemcpyHostToDevice, c->stream));
+		{
+			const dim3 grid((unsigned)std::min<size_t>((mat + 255) / 256, 1024), nb_);
+			tl_build_kernel<<<grid, 256, 0, c->stream>>>(w.x, c->d_nb, c->d_link, c->cfg.t[0], c->cfg.t[1], c->cfg.t[2],
+				c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], -c->cfg.mu, n, w.A);
+			EPHMC_LAUNCHED();
+		}
+		EPHMC_TRY(timer_begin(c, "tl_tridiag"));
+		{
+			const size_t lds = (3 * (size_t)n + TL_BT / 64) * 8;
+			if(n <= TL_BT) tl_tridiag_kernel<TL_BT, 1><<<nb_, TL_BT, lds, c->stream>>>(w.A, n, w.d, w.e, w.tau);
+			else tl_tridiag_kernel<TL_BT, 2><<<nb_, TL_BT, lds, c->stream>>>(w.A, n, w.d, w.e, w.tau);
+			EPHMC_LAUNCHED();
+		}
+		EPHMC_TRY(timer_end(c, "tl_tridiag"));
+		EPHMC_TRY(timer_begin(c, "tl_ql"));
+		tl_ql_scalar_kernel<<<nb_, 64, 2 * (size_t)n * 8, c->stream>>>(w.d, w.e, n, w.ev, w.rot, w.rot_cap, w.sw, w.sw_cap, w.nsw,
+			w.status);
+		EPHMC_LAUNCHED();
+		tl_ql_apply_kernel<TL_G><<<dim3((n + 255) / 256, nb_), 256, 0, c->stream>>>(w.Z, n, w.rot, w.rot_cap, w.sw, w.sw_cap, w.nsw);
+		EPHMC_LAUNCHED();
+		EPHMC_TRY(timer_end(c, "tl_ql"));
+		EPHMC_TRY(timer_begin(c, "tl_back"));
+		if(n <= 1024) tl_back_kernel<16, 4><<<dim3((n + 15) / 16, nb_), 256, 0, c->stream>>>(w.A, w.tau, w.Z, n);
+		else tl_back_kernel<32, 2><<<dim3((n + 7) / 8, nb_), 256, 0, c->stream>>>(w.A, w.tau, w.Z, n);
+		EPHMC_LAUNCHED();
+		EPHMC_TRY(timer_end(c, "tl_back"));
+		tl_sort_kernel<TL_BT><<<nb_, TL_BT, ((size_t)n + TL_BT / 64) * 8, c->stream>>>(w.ev, n, w.evs, w.ems, w.perm, w.res);
+		EPHMC_LAUNCHED();
+		tl_current_kernel<<<dim3(n, nb_), 256, (size_t)n * 8, c->stream>>>(w.Z, w.perm, w.x, c->d_nb, n, c->cfg.t[0], c->cfg.t[1],
+			c->cfg.t[2], c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], contr_mat[0], contr_mat[1], contr_mat[2],
+			contr_mat[3], contr_mat[4], contr_mat[5], w.V, w.W0, w.W1);
+		EPHMC_LAUNCHED();
+		EPHMC_TRY(timer_begin(c, "tl_pairs"));
+		tl_pairs_kernel<<<dim3(nblk, nblk, nb_), 256, 0, c->stream>>>(w.V, w.W0, w.W1, w.evs, w.ems, n, width, nbins, w.part);
+		EPHMC_LAUNCHED();
+		tl_reduce_kernel<<<(nbins + 255) / 256, 256, 0, c->stream>>>(w.part, nb_ * nblk * nblk, nbins, w.acc);
+		EPHMC_LAUNCHED();
+		EPHMC_TRY(timer_end(c, "tl_pairs"));
+		EPHMC_HIP(hipMemcpyAsync(results + s0, w.res, (size_t)nb_ * 8, hipMemcpyDeviceToHost, c->stream));
+		EPHMC_HIP(hipMemcpyAsync(status.data(), w.status, (size_t)nb_ * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+		EPHMC_HIP(hipStreamSynchronize(c->stream));
+		for(unsigned k = 0; k < nb_; k++)
+			if(status[k]){
+				set_error("eigensolver of disorder sample %u: %s", s0 + k,
+					status[k] == 1 ? "QL iteration did not converge in 60 sweeps" : "rotation record capacity exceeded");
+				return EPHMC_ERR_NUMERIC;
+			}
+	}
+	EPHMC_TRY(timer_end(c, "tl"));
+	std::vector<double> acc(nbins);
+	EPHMC_HIP(hipMemcpyAsync(acc.data(), w.acc, (size_t)nbins * 8, hipMemcpyDeviceToHost, c->stream));
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	for(unsigned b = 0; b < nbins; b++) greens[b] += acc[b];
+	return EPHMC_OK;
+}

@@ -286,6 +286,17 @@ class Context:
         check(LIB.ephmc_cc_correlator(self.h, dx.ptr, dm.ptr, logdet, _dp(out)))
         return out.reshape(self.nt, 3, 3)
 
+    # -- transient localisation ------------------------------------------------------------
+    def tl_hist(self, normals, contr_mat, width, nbins):
+        """Disorder samples (rows of `normals`, 3N each): bins accumulated over the samples (not
+        divided by their number) and sum_i exp(-E_i)/N per sample."""
+        normals = np.ascontiguousarray(normals, np.float64).reshape(-1, 3 * self.ns)
+        cm = np.ascontiguousarray(contr_mat, np.float64).reshape(6)
+        greens = np.zeros(nbins)
+        res = np.zeros(normals.shape[0])
+        check(LIB.ephmc_tl_hist(self.h, _dp(normals), normals.shape[0], _dp(cm), width, nbins, _dp(greens), _dp(res)))
+        return greens, res
+
     def gj_fallbacks(self):
         v = C.c_ulong()
         check(LIB.ephmc_gj_fallbacks(self.h, C.byref(v)))

@@ -165,3 +165,58 @@ def test_organic_main_end_to_end():
     np.testing.assert_allclose(got, exp, rtol=1e-9, atol=1e-12)
     assert [int(c[0]) for c in corr_rows] == list(range(nt)) * (meas // freq)
     np.testing.assert_allclose([float(c[1]) for c in corr_rows], expect_corr, rtol=1e-9, atol=1e-14)
+
+
+TL_INPUT = """{l1} {l2}
+{nbins}
+40
+0.1 -0.03 0.02
+0.3 0.3 0.3
+0
+0.006
+1 0
+0
+{meas} 1
+1 0
+0.5 0.8660254037844386
+1
+STATIC_DISORDER
+SEED {seed}
+"""
+
+
+def test_organic_main_transient_localisation():
+    """organic_main with STATIC_DISORDER (run_trans_loc, organic_trans_loc.c:85-114): per-sample
+    results.csv lines and the greens.bin histogram against the oracle with the same RNG stream."""
+    l1, l2, nbins, meas, seed = 6, 6, 40, 12, 99
+    om = OracleModel(l1, l2, nbins, beta=40.0, static_disorder=True)
+    a, b = np.array([1.0, 0.0]), np.array([0.5, 0.8660254037844386])
+    cm = np.ctypeslib.as_array(oracle().orc_construct_contraction_mat(ptr(a), ptr(b), 3), (6,)).copy()
+    with tempfile.TemporaryDirectory() as d:
+        inp, res, gbin = (os.path.join(d, f) for f in ("input.txt", "results.csv", "greens.bin"))
+        open(inp, "w").write(TL_INPUT.format(l1=l1, l2=l2, nbins=nbins, meas=meas, seed=seed))
+        say("organic_main STATIC_DISORDER", l1, l2, nbins)
+        out = subprocess.run([MAIN, inp, res, gbin], capture_output=True, text=True, timeout=300)
+        say("organic_main rc", out.returncode, out.stdout.strip()[-200:], out.stderr.strip()[-200:])
+        assert out.returncode == 0 and "ERROR" not in out.stdout
+        z = np.array([float(line.split("\t")[0]) for line in open(res).read().splitlines()])
+        raw = open(gbin, "rb").read()
+    n_read = int(np.frombuffer(raw[:4], np.uint32)[0])
+    width_read = float(np.frombuffer(raw[4:12], np.float64)[0])
+    g = np.frombuffer(raw[12:], np.float64)
+    assert n_read == nbins and g.size == nbins
+    lib = oracle()
+    width = lib.orc_tl_width(om.ref, nbins)
+    assert width_read == width
+    from _oracle import MT
+    rng = MT()
+    lib.orc_mt_seed(C.byref(rng), seed)
+    go = np.zeros(nbins)
+    zo = []
+    x = np.zeros(3 * om.ns)
+    for _ in range(meas):
+        lib.orc_random_vector(C.byref(rng), ptr(x), 3 * om.ns)
+        zo.append(lib.orc_hist_trans_loc(om.ref, ptr(x), ptr(cm), width, ptr(go), nbins))
+    go /= meas
+    np.testing.assert_allclose(z, zo, rtol=1e-12)
+    assert rel(g, go) < 1e-9
