@@ -120,19 +120,61 @@ def cpu_baseline(sample_threads):
                       % (t_mlu, t_force, sample_threads, t_traj, platform.processor() or platform.machine())}
 
 
+def copy_calibration():
+    """A plain device-to-device copy of 512 MiB, five times: 512 MiB read + 512 MiB written per
+    launch, the reference point for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy calibration: 5 x %d bytes" % (a.numel() * 8))
+
+
+def tl_leg(E, nsamples):
+    """Transient localisation at 32x32 (N=1024), 512 bins, beta=40, seed-7 disorder normals:
+    disorder samples per second of the batched eigensolve + current matrix elements + binning."""
+    nbins = 512
+    t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, nbins, static_disorder=True)
+    ctx = E.Context(L, L, 1, t, kappa, mu, omega, flavors=1, steps=1)
+    width = -2 * mu / nbins + sum(8 * abs(k) / nbins for k in kappa)
+    a, b = (1.0, 0.0), (0.5, 0.866)
+    cm = [a[0], 0.5 * (a[0] + b[0]), 0.5 * (b[0] - a[0]), a[1], 0.5 * (a[1] + b[1]), 0.5 * (b[1] - a[1])]
+    normals = np.random.default_rng(7).standard_normal((nsamples, 3 * L * L))
+    ctx.tl_hist(normals[:min(4, nsamples)], cm, width, nbins)       # warm-up (workspace, code load)
+    ctx.sync()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    g, z = ctx.tl_hist(normals, cm, width, nbins)
+    el = time.perf_counter() - t0
+    groups = {k: ctx.timing_get(k) for k in ("tl", "tl_tridiag", "tl_ql", "tl_back", "tl_pairs")}
+    ctx.close()
+    return {"samples_per_s": nsamples / el, "samples": nsamples, "seconds": el,
+            "config": "32x32 (N=1024), 512 bins, beta=40, FP64",
+            "kernel_ms_per_batch": {k: round(v[0], 3) for k, v in groups.items()},
+            "greens_sum": float(g.sum())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-tl", action="store_true", help="skip the transient-localisation leg")
+    ap.add_argument("--tl-samples", type=int, default=256)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 512 MiB device copy (known bytes) for HBM counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; a single chain does not span GPUs here "
               "(independent chains and disorder samples shard trivially, see the design notes)", file=sys.stderr)
         sys.exit(2)
     import ephmc as E
+    if args.copy_calibration:
+        copy_calibration()
     rng = np.random.default_rng(SEED)
     dim = NT * 3 * L * L
     nbuf = min(args.steps + args.warmup, 8)
@@ -176,6 +218,9 @@ def main():
         "kernel_ms": {p: {g: round(v[0], 4) for g, v in res[p]["groups"].items()} for p in res},
         "roofline": roof,
     }
+    if not args.no_tl:
+        out["tl"] = tl_leg(E, args.tl_samples)
+        log("tl", out["tl"])
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(threads)
