@@ -166,6 +166,176 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 	}
 }
 
+// ---------------------------------------------------------------------------------------------
+// Strip form of the same chain.  A thread owns M consecutive sites s0 .. s0+M-1 of one lattice
+// row (l1 % M == 0).  Inside the strip the left/right neighbours are the thread's own
+// registers, and neighbouring sites share their neighbours in the rows above and below:
+//   up[k], k = 0..M: TL(s0), TR(s0), TR(s0+1), ..., TR(s0+M-1)   (row above, a0-1 .. a0+M-1)
+//   dn[k], k = 0..M: BL(s0), ..., BL(s0+M-1), BR(s0+M-1)         (row below, a0 .. a0+M)
+//   lf = L(s0), rt = R(s0+M-1)
+// so site k has R = own[k+1] (or rt), L = own[k-1] (or lf), TR = up[k+1], BL = dn[k],
+// TL = up[k], BR = dn[k+1]: 2M+4 LDS reads per strip instead of 6M.  The sum is taken in the
+// same order (d = R, L, TR, BL, TL, BR) as stencil_gather, so both forms give identical bits.
+template <int M>
+struct StripNb {
+	int up[M + 1], dn[M + 1], lf, rt;
+};
+
+template <int M>
+__device__ __forceinline__ void strip_neighbours(const int *__restrict__ nb, unsigned ns, unsigned s0, StripNb<M> &sn)
+{
+	sn.up[0] = nb[4 * ns + s0];
+#pragma unroll
+	for(int k = 0; k < M; k++){
+		sn.up[k + 1] = nb[2 * ns + s0 + k];
+		sn.dn[k] = nb[3 * ns + s0 + k];
+	}
+	sn.dn[M] = nb[5 * ns + s0 + M - 1];
+	sn.lf = nb[1 * ns + s0];
+	sn.rt = nb[0 * ns + s0 + M - 1];
+}
+
+template <typename R, int C, int M>
+__device__ __forceinline__ void strip_gather(const R *__restrict__ buf, const StripNb<M> &sn, const R (&cf)[M][6], R mu,
+		const R (&own)[M][C], R (&out)[M][C])
+{
+	R up[M + 1][C], dn[M + 1][C], lf[C], rt[C];
+#pragma unroll
+	for(int k = 0; k <= M; k++)
+#pragma unroll
+		for(int c = 0; c < C; c++){
+			up[k][c] = buf[(size_t)sn.up[k] * C + c];
+			dn[k][c] = buf[(size_t)sn.dn[k] * C + c];
+		}
+#pragma unroll
+	for(int c = 0; c < C; c++){
+		lf[c] = buf[(size_t)sn.lf * C + c];
+		rt[c] = buf[(size_t)sn.rt * C + c];
+	}
+#pragma unroll
+	for(int k = 0; k < M; k++)
+#pragma unroll
+		for(int c = 0; c < C; c++){
+			const R r = k < M - 1 ? own[k + 1][c] : rt[c];
+			const R l = k > 0 ? own[k - 1][c] : lf[c];
+			R o = mu * own[k][c];
+			o += cf[k][0] * r;
+			o += cf[k][1] * l;
+			o += cf[k][2] * up[k + 1][c];
+			o += cf[k][3] * dn[k][c];
+			o += cf[k][4] * up[k][c];
+			o += cf[k][5] * dn[k + 1][c];
+			out[k][c] = o;
+		}
+}
+
+template <typename R, int C, int M, int LB>
+__global__ __launch_bounds__(LB) void chain_strip_kernel(ChainArgs<R> a)
+{
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	R *V = reinterpret_cast<R *>(smem_raw);
+	R *W = V + (size_t)C * a.ns;
+	const unsigned ns = a.ns;
+	const unsigned col0 = blockIdx.x * C;
+	const size_t mat = (size_t)ns * ns;
+	const unsigned s0 = threadIdx.x * M;      // blockDim.x * M == ns
+	StripNb<M> sn;
+	strip_neighbours<M>(a.nb, ns, s0, sn);
+	R v[M][C], w[M][C], cf[M][6];
+#pragma unroll
+	for(int k = 0; k < M; k++)
+#pragma unroll
+		for(int c = 0; c < C; c++){
+			const unsigned col = col0 + c, site = s0 + k;
+			R val = 0;
+			if(col < ns) val = a.src ? a.src[(size_t)col * ns + site] : (R)(site == col ? 1 : 0);
+			v[k][c] = val;
+		}
+	for(int i = 0; i < a.ntau; i++){
+		const int tau = a.tau0 + a.dir * i;
+		if(a.ck && (i % a.ck_every) == 0){
+			R *ck = a.ck + (size_t)(i / a.ck_every) * mat;
+#pragma unroll
+			for(int k = 0; k < M; k++)
+#pragma unroll
+				for(int c = 0; c < C; c++)
+					if(col0 + c < ns) ck[(size_t)(col0 + c) * ns + s0 + k] = v[k][c];
+		}
+		const R *cft = a.coef + (size_t)tau * 6 * ns;
+#pragma unroll
+		for(int k = 0; k < M; k++)
+#pragma unroll
+			for(int d = 0; d < 6; d++) cf[k][d] = cft[d * ns + s0 + k];
+		for(int p = 0; p < a.np; p++){
+#pragma unroll
+			for(int k = 0; k < M; k++)
+#pragma unroll
+				for(int c = 0; c < C; c++) V[(size_t)(s0 + k) * C + c] = v[k][c];
+			__syncthreads();
+			strip_gather<R, C, M>(V, sn, cf, a.mu, v, w);
+			if(a.cplx[p]){
+#pragma unroll
+				for(int k = 0; k < M; k++)
+#pragma unroll
+					for(int c = 0; c < C; c++) W[(size_t)(s0 + k) * C + c] = w[k][c];
+				__syncthreads();
+				R z[M][C];
+				strip_gather<R, C, M>(W, sn, cf, a.mu, w, z);
+#pragma unroll
+				for(int k = 0; k < M; k++)
+#pragma unroll
+					for(int c = 0; c < C; c++) v[k][c] += a.a_lin[p] * w[k][c] + a.a_quad[p] * z[k][c];
+			}else{
+				__syncthreads();
+#pragma unroll
+				for(int k = 0; k < M; k++)
+#pragma unroll
+					for(int c = 0; c < C; c++) v[k][c] += a.a_lin[p] * w[k][c];
+			}
+		}
+	}
+	if(a.dst){
+#pragma unroll
+		for(int k = 0; k < M; k++)
+#pragma unroll
+			for(int c = 0; c < C; c++)
+				if(col0 + c < ns) a.dst[(size_t)(col0 + c) * ns + s0 + k] = v[k][c];
+	}
+}
+
+// Strip width for this lattice (0: the strip form does not apply).  EPHMC_CHAIN_STRIP=0/2/4
+// overrides it for tuning sweeps.
+unsigned strip_width(const ephmc_ctx *c)
+{
+	unsigned m = 4;
+	if(const char *e = getenv("EPHMC_CHAIN_STRIP")) m = (unsigned)atoi(e);
+	if(m != 2 && m != 4) return 0;
+	if(c->l1 % m != 0) m = (m == 4 && c->l1 % 2 == 0) ? 2 : 0;
+	if(m == 0 || c->ns / m > 1024) return 0;
+	return m;
+}
+
+template <typename R, int C, int M>
+static int launch_strip(ephmc_ctx *c, const ChainArgs<R> &a)
+{
+	const unsigned ns = a.ns;
+	const size_t lds = 2 * (size_t)C * ns * sizeof(R);
+	if(lds > 160 * 1024){ set_error("chain: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
+	const unsigned bt = ns / M, blocks = (ns + C - 1) / C;
+#define EPHMC_STRIP_LAUNCH(LB)                                                                          \
+	do {                                                                                                \
+		(void)hipFuncSetAttribute((const void *)chain_strip_kernel<R, C, M, LB>,                        \
+			hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                       \
+		chain_strip_kernel<R, C, M, LB><<<blocks, bt, lds, c->stream>>>(a);                              \
+	} while(0)
+	if(bt <= 256) EPHMC_STRIP_LAUNCH(256);
+	else if(bt <= 512) EPHMC_STRIP_LAUNCH(512);
+	else EPHMC_STRIP_LAUNCH(1024);
+#undef EPHMC_STRIP_LAUNCH
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
 // Launch shape: threads per workgroup and sites per thread from the lattice size; columns
 // per workgroup from the tuning table (ctx->cols) capped by LDS.
 template <typename R, int C>
@@ -197,6 +367,23 @@ static int launch_chain(ephmc_ctx *c, const ChainArgs<R> &a, unsigned cols)
 {
 	// LDS budget: 2 * C * ns * sizeof(R) <= 160 KiB
 	while(cols > 1 && 2 * (size_t)cols * a.ns * sizeof(R) > 160 * 1024) cols >>= 1;
+	const unsigned m = strip_width(c);
+	if(m == 4){
+		switch(cols){
+			case 1: return launch_strip<R, 1, 4>(c, a);
+			case 2: return launch_strip<R, 2, 4>(c, a);
+			case 4: return launch_strip<R, 4, 4>(c, a);
+			default: return launch_strip<R, 8, 4>(c, a);
+		}
+	}
+	if(m == 2){
+		switch(cols){
+			case 1: return launch_strip<R, 1, 2>(c, a);
+			case 2: return launch_strip<R, 2, 2>(c, a);
+			case 4: return launch_strip<R, 4, 2>(c, a);
+			default: return launch_strip<R, 8, 2>(c, a);
+		}
+	}
 	switch(cols){
 		case 1: return launch_chain_c<R, 1>(c, a);
 		case 2: return launch_chain_c<R, 2>(c, a);

@@ -17,7 +17,7 @@ def field(om, seed, scale=1.0):
     return np.random.default_rng(seed).standard_normal(om.dim) * scale
 
 
-@pytest.mark.parametrize("l1,l2,nt", [(1, 1, 4), (2, 2, 4), (2, 3, 8), (6, 6, 32), (8, 12, 16), (16, 16, 8)])
+@pytest.mark.parametrize("l1,l2,nt", [(1, 1, 4), (2, 2, 4), (2, 3, 8), (5, 4, 8), (6, 6, 32), (8, 12, 16), (16, 16, 8)])
 def test_M_hat(l1, l2, nt):
     say("M_hat", l1, l2, nt)
     om = OracleModel(l1, l2, nt, beta=nt * 40 / 32)
@@ -28,6 +28,21 @@ def test_M_hat(l1, l2, nt):
     Mg = ctx.M_hat(x)
     Mo = om.M_hat(x)
     assert rel(Mg, Mo) < 1e-13
+
+
+@pytest.mark.parametrize("l1,l2,nt", [(2, 3, 4), (6, 6, 8), (16, 16, 8), (32, 32, 4)])
+def test_strip_chain_bit_identical(monkeypatch, l1, l2, nt):
+    """The strip form of the slice stencil sums in the same order as the site-per-thread form:
+    M_hat must agree to the last bit (EPHMC_CHAIN_STRIP=0 selects the site-per-thread form)."""
+    say("strip vs site chain", l1, l2, nt)
+    om = OracleModel(l1, l2, nt, beta=nt * 40 / 32)
+    ctx = gpu_context(om)
+    x = field(om, 5)
+    monkeypatch.setenv("EPHMC_CHAIN_STRIP", "4")
+    Ms = ctx.M_hat(x)
+    monkeypatch.setenv("EPHMC_CHAIN_STRIP", "0")
+    Mg = ctx.M_hat(x)
+    np.testing.assert_array_equal(Ms, Mg)
 
 
 @pytest.mark.parametrize("revert,stop", [(0, 2), (0, 1), (0, 0), (1, 0), (1, -1)])
