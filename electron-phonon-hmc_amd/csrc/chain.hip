@@ -12,6 +12,7 @@
 // between i and nb(i,d) (k = d/2).  This gather is exactly the reference's scatter over
 // forward bonds, including lattices of extent 1 or 2 where bonds alias.
 #include "internal.h"
+#include "stencil.h"
 #include <cstdlib>
 
 namespace ephmc {
@@ -167,68 +168,10 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Strip form of the same chain.  A thread owns M consecutive sites s0 .. s0+M-1 of one lattice
-// row (l1 % M == 0).  Inside the strip the left/right neighbours are the thread's own
-// registers, and neighbouring sites share their neighbours in the rows above and below:
-//   up[k], k = 0..M: TL(s0), TR(s0), TR(s0+1), ..., TR(s0+M-1)   (row above, a0-1 .. a0+M-1)
-//   dn[k], k = 0..M: BL(s0), ..., BL(s0+M-1), BR(s0+M-1)         (row below, a0 .. a0+M)
-//   lf = L(s0), rt = R(s0+M-1)
-// so site k has R = own[k+1] (or rt), L = own[k-1] (or lf), TR = up[k+1], BL = dn[k],
-// TL = up[k], BR = dn[k+1]: 2M+4 LDS reads per strip instead of 6M.  The sum is taken in the
-// same order (d = R, L, TR, BL, TL, BR) as stencil_gather, so both forms give identical bits.
-template <int M>
-struct StripNb {
-	int up[M + 1], dn[M + 1], lf, rt;
-};
-
-template <int M>
-__device__ __forceinline__ void strip_neighbours(const int *__restrict__ nb, unsigned ns, unsigned s0, StripNb<M> &sn)
-{
-	sn.up[0] = nb[4 * ns + s0];
-#pragma unroll
-	for(int k = 0; k < M; k++){
-		sn.up[k + 1] = nb[2 * ns + s0 + k];
-		sn.dn[k] = nb[3 * ns + s0 + k];
-	}
-	sn.dn[M] = nb[5 * ns + s0 + M - 1];
-	sn.lf = nb[1 * ns + s0];
-	sn.rt = nb[0 * ns + s0 + M - 1];
-}
-
-template <typename R, int C, int M>
-__device__ __forceinline__ void strip_gather(const R *__restrict__ buf, const StripNb<M> &sn, const R (&cf)[M][6], R mu,
-		const R (&own)[M][C], R (&out)[M][C])
-{
-	R up[M + 1][C], dn[M + 1][C], lf[C], rt[C];
-#pragma unroll
-	for(int k = 0; k <= M; k++)
-#pragma unroll
-		for(int c = 0; c < C; c++){
-			up[k][c] = buf[(size_t)sn.up[k] * C + c];
-			dn[k][c] = buf[(size_t)sn.dn[k] * C + c];
-		}
-#pragma unroll
-	for(int c = 0; c < C; c++){
-		lf[c] = buf[(size_t)sn.lf * C + c];
-		rt[c] = buf[(size_t)sn.rt * C + c];
-	}
-#pragma unroll
-	for(int k = 0; k < M; k++)
-#pragma unroll
-		for(int c = 0; c < C; c++){
-			const R r = k < M - 1 ? own[k + 1][c] : rt[c];
-			const R l = k > 0 ? own[k - 1][c] : lf[c];
-			R o = mu * own[k][c];
-			o += cf[k][0] * r;
-			o += cf[k][1] * l;
-			o += cf[k][2] * up[k + 1][c];
-			o += cf[k][3] * dn[k][c];
-			o += cf[k][4] * up[k][c];
-			o += cf[k][5] * dn[k + 1][c];
-			out[k][c] = o;
-		}
-}
-
+// ---------------------------------------------------------------------------------------------
+// Strip form of the same chain (stencil.h): a thread owns M consecutive sites of a lattice row.
+// The sum is taken in the same order (d = R, L, TR, BL, TL, BR) as stencil_gather, so both forms
+// give identical bits.
 template <typename R, int C, int M, int LB>
 __global__ __launch_bounds__(LB) void chain_strip_kernel(ChainArgs<R> a)
 {

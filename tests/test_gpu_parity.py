@@ -45,6 +45,24 @@ def test_strip_chain_bit_identical(monkeypatch, l1, l2, nt):
     np.testing.assert_array_equal(Ms, Mg)
 
 
+@pytest.mark.parametrize("l1,l2,nt,k", [(4, 6, 8, 4), (16, 16, 16, 4), (32, 32, 8, 8)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_strip_adjoint_bit_identical(monkeypatch, l1, l2, nt, k, precision):
+    """Strip forms (2 and 4 sites per thread) of the adjoint sweep sum in the same order as the
+    site-per-thread form: the force must agree to the last bit."""
+    say("strip vs site adjoint", l1, l2, nt, k, precision)
+    om = OracleModel(l1, l2, nt, beta=nt * 40 / 32)
+    ctx = gpu_context(om, precision=precision, checkpoint=k)
+    x = field(om, 6)
+    _, _, Minv = om.logdet_inv(om.M_hat(x))
+    out = {}
+    for m in ("0", "2", "4"):
+        monkeypatch.setenv("EPHMC_ADJOINT_STRIP", m)
+        out[m] = ctx.force(x, Minv, 0.0)
+    np.testing.assert_array_equal(out["2"], out["0"])
+    np.testing.assert_array_equal(out["4"], out["0"])
+
+
 @pytest.mark.parametrize("revert,stop", [(0, 2), (0, 1), (0, 0), (1, 0), (1, -1)])
 def test_apply_exp_partial(revert, stop):
     say("apply_exp_M", revert, stop)
