@@ -435,7 +435,9 @@ int ephmc_force(ephmc_ctx *c, const double *x, const double *M_inv, double logde
 	const int prec = c->cfg.precision;
 	EPHMC_TRY(coefficients(c, x, prec != EPHMC_FP64));
 	void *tfin = prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32;
+	EPHMC_TRY(timer_begin(c, "chain"));
 	EPHMC_TRY(chain_forward(c, prec, nullptr, tfin, c->d_ck, c->ckpt, 0, true));
+	EPHMC_TRY(timer_end(c, "chain"));
 	EPHMC_TRY(timer_begin(c, "adjoint"));
 	EPHMC_TRY(force_adjoint(c, prec, M_inv, p_dot, 0, nullptr));
 	EPHMC_TRY(timer_end(c, "adjoint"));
