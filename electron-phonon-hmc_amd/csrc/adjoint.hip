@@ -45,6 +45,7 @@ struct AdjArgs {
 	const double *red_src;  // partials of the segment reduced here, or null
 	int red_tau_b, red_ntau;
 	double f[3];            // -kappa_k flavors
+	const double *fscale;   // device scalar multiplying the force (SINGLE_PARTICLE: exp(-logdet)), or null
 	double *pdot, *p;       // outputs of the reduction
 	double kick;            // p += kick F when p != null
 };
@@ -54,9 +55,10 @@ struct AdjArgs {
 // `nthreads` threads: thread t handles link t % P and the workgroup chunk t / P (P = min(nl,
 // nthreads)); chunk sums are combined in chunk order through `scratch` (nthreads doubles).
 __device__ void reduce_links(const double *__restrict__ part, unsigned nwg, unsigned loc, unsigned l0, unsigned nl,
-		unsigned nlinks_total, int tau_b, const double (&f)[3], double *__restrict__ pdot, double *__restrict__ p,
-		double kick, double *scratch, unsigned nthreads)
+		unsigned nlinks_total, int tau_b, const double (&f)[3], const double *__restrict__ fscale, double *__restrict__ pdot,
+		double *__restrict__ p, double kick, double *scratch, unsigned nthreads)
 {
+	const double fs = fscale ? *fscale : 1.0;
 	const unsigned t = threadIdx.x;
 	const unsigned P = nl < nthreads ? nl : nthreads;
 	if(P == 0) return;
@@ -79,7 +81,7 @@ __device__ void reduce_links(const double *__restrict__ part, unsigned nwg, unsi
 			for(unsigned q = 0; q < CH; q++) tot += scratch[q * P + t];
 			const unsigned lk2 = l0 + g0 + t;
 			const unsigned tl = lk2 / loc, link = lk2 - tl * loc;
-			const double F = f[link % 3] * tot;
+			const double F = f[link % 3] * tot * fs;
 			const size_t pos = (size_t)(tau_b + tl) * loc + link;
 			if(pdot) pdot[pos] = F;
 			if(p) p[pos] += kick * F;
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 	if(a.red_src){
 		const unsigned total = (unsigned)a.red_ntau * 3 * ns;
 		const unsigned per = (total + a.nwg - 1) / a.nwg;
-		reduce_links(a.red_src, a.nwg, 3 * ns, blockIdx.x * per, per, total, a.red_tau_b, a.f, a.pdot, a.p, a.kick,
+		reduce_links(a.red_src, a.nwg, 3 * ns, blockIdx.x * per, per, total, a.red_tau_b, a.f, a.fscale, a.pdot, a.p, a.kick,
 			rscratch, BT);
 	}
 	R *wbs[2] = {wb0, wb1};
@@ -360,7 +362,7 @@ __global__ __launch_bounds__(LB) void adjoint_strip_kernel(AdjArgs<R> a)
 	if(a.red_src){
 		const unsigned total = (unsigned)a.red_ntau * 3 * ns;
 		const unsigned per = (total + a.nwg - 1) / a.nwg;
-		reduce_links(a.red_src, a.nwg, 3 * ns, blockIdx.x * per, per, total, a.red_tau_b, a.f, a.pdot, a.p, a.kick,
+		reduce_links(a.red_src, a.nwg, 3 * ns, blockIdx.x * per, per, total, a.red_tau_b, a.f, a.fscale, a.pdot, a.p, a.kick,
 			rscratch, blockDim.x);
 	}
 	const unsigned col0 = blockIdx.x * C;
@@ -572,12 +574,21 @@ __global__ __launch_bounds__(LB) void adjoint_strip_kernel(AdjArgs<R> a)
 
 // Stand-alone form of reduce_links for the last segment: 64 links x 4 chunks per workgroup.
 __global__ __launch_bounds__(256) void force_reduce_kernel(const double *__restrict__ partial, unsigned nwg, unsigned loc,
-		int tau_b, int ntau, double f0, double f1, double f2, double *__restrict__ pdot, double *__restrict__ p, double kick)
+		int tau_b, int ntau, double f0, double f1, double f2, const double *__restrict__ fscale, double *__restrict__ pdot,
+		double *__restrict__ p, double kick)
 {
 	__shared__ double scratch[256];
 	const double f[3] = {f0, f1, f2};
 	const unsigned total = (unsigned)ntau * loc;
-	reduce_links(partial, nwg, loc, blockIdx.x * 64, 64, total, tau_b, f, pdot, p, kick, scratch, 256);
+	reduce_links(partial, nwg, loc, blockIdx.x * 64, 64, total, tau_b, f, fscale, pdot, p, kick, scratch, 256);
+}
+
+// CONSTANT_FORCE: every slice gets the force of slice 0 (organic_hmc.c:92-98,128)
+__global__ void broadcast_slice_kernel(double *__restrict__ pdot, unsigned loc, unsigned nt)
+{
+	const size_t n = (size_t)loc * nt;
+	for(size_t e = loc + blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+		pdot[e] = pdot[e % loc];
 }
 
 unsigned adjoint_cols(const ephmc_ctx *c, int prec)
@@ -647,10 +658,14 @@ static unsigned adjoint_strip(const ephmc_ctx *c, int prec)
 	return m;
 }
 
+// Segments s = nseg-1 .. 0 of the reverse sweep; W starts from minv (fp64) or, when minv is null,
+// from the W buffer (already in the precision R).  nt_eff < nt restricts the sweep to slices
+// [0, nt_eff) (CONSTANT_FORCE: nt_eff = 1 with W_0 prepared by the caller).
 template <typename R>
-static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p)
+static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p,
+		const double *fscale, unsigned nt_eff)
 {
-	const unsigned ns = c->ns, nt = c->nt, k = c->ckpt, C = adjoint_cols(c, prec);
+	const unsigned ns = c->ns, nt = nt_eff, k = c->ckpt, C = adjoint_cols(c, prec);
 	const size_t mat = c->mat;
 	AdjArgs<R> a{};
 	a.coef = prec == EPHMC_FP64 ? (const R *)c->d_coef64 : (const R *)c->d_coef32;
@@ -675,15 +690,17 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 	a.pdot = pdot;
 	a.p = p;
 	a.kick = kick;
+	a.fscale = fscale;
+	const unsigned nseg = (nt + k - 1) / k;
 	int prev_b = 0, prev_n = 0;
 	const double *prev = nullptr;
 	const unsigned m = adjoint_strip(c, prec);
-	for(int s = (int)c->nseg - 1; s >= 0; s--){
+	for(int s = (int)nseg - 1; s >= 0; s--){
 		a.tau_b = s * (int)k;
 		a.tau_e = std::min<int>((s + 1) * (int)k, (int)nt);
 		a.ck = ck + (size_t)s * mat;
 		a.seg = (R *)c->d_seg;
-		a.w_init = (s == (int)c->nseg - 1) ? minv : nullptr;
+		a.w_init = (s == (int)nseg - 1) ? minv : nullptr;
 		a.partial = c->d_partial[s & 1];
 		a.red_src = prev;
 		a.red_tau_b = prev_b;
@@ -718,7 +735,7 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 	const size_t n = (size_t)prev_n * c->loc;
 	EPHMC_TRY(timer_begin(c, "force_reduce"));
 	force_reduce_kernel<<<(unsigned)((n + 63) / 64), 256, 0, c->stream>>>(prev, a.nwg, c->loc, prev_b, prev_n,
-		f[0], f[1], f[2], pdot, p, kick);
+		f[0], f[1], f[2], fscale, pdot, p, kick);
 	EPHMC_LAUNCHED();
 	EPHMC_TRY(timer_end(c, "force_reduce"));
 	return EPHMC_OK;
@@ -726,10 +743,27 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 
 // Force for the field whose coefficients and checkpoints are current in the context
 // (chain_forward with checkpoints must have run for this field).
-int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p)
+int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p, const double *fscale)
 {
-	if(prec == EPHMC_FP64) return run_adjoint<double>(c, prec, minv, pdot, kick, p);
-	return run_adjoint<float>(c, prec, minv, pdot, kick, p);
+	if(prec == EPHMC_FP64) return run_adjoint<double>(c, prec, minv, pdot, kick, p, fscale, c->nt);
+	return run_adjoint<float>(c, prec, minv, pdot, kick, p, fscale, c->nt);
+}
+
+// CONSTANT_FORCE (organic_hmc.c:92-98 with construct_QM1P at time 0, organic_dense_matrix.c:267-309):
+// W_0 = B_1 ... B_{nt-1} M_hat^{-1} (or the identity chain for SINGLE_PARTICLE), the exact
+// slice-0 derivative with T_0 = 1 (checkpoint 0 of the forward chain), the same force on every
+// slice.  minv: M_hat^{-1} (fp64), or null for SINGLE_PARTICLE.
+int force_constant(ephmc_ctx *c, int prec, const double *minv, double *pdot, const double *fscale)
+{
+	void *w0 = prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32;
+	EPHMC_TRY(chain_range(c, prec, minv, w0, (int)c->nt - 1, -1, (int)c->nt - 1));
+	if(prec == EPHMC_FP64) EPHMC_TRY(run_adjoint<double>(c, prec, nullptr, pdot, 0, nullptr, fscale, 1));
+	else EPHMC_TRY(run_adjoint<float>(c, prec, nullptr, pdot, 0, nullptr, fscale, 1));
+	if(c->nt > 1){
+		broadcast_slice_kernel<<<1024, 256, 0, c->stream>>>(pdot, c->loc, c->nt);
+		EPHMC_LAUNCHED();
+	}
+	return EPHMC_OK;
 }
 
 } // namespace ephmc

@@ -33,7 +33,8 @@ struct FftArgs {
 	const double *twid;    // [nt] e^{-2 pi i j/nt} as (re, im)
 	const double *f2;      // [nt] f_n
 	double c, s;           // evolve: cos h / nt, sin h / nt
-	int gain_inv;          // filter: 0 -> sqrt(f)/nt, 1 -> 1/(sqrt(f) nt)
+	int gain;              // filter: 0 -> sqrt(f)/nt, 1 -> 1/(sqrt(f) nt), 2 -> h/(f nt) added to out0
+	double h;
 	double *partial;       // energy: one value per workgroup
 	unsigned loc, nt, logn;
 	int pow2;
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(FFT_BT) void fa_fft_kernel(FftArgs a)
 		for(unsigned q = tid; q < nt * FFT_LW; q += FFT_BT){
 			const unsigned l = q % FFT_LW, n = q / FFT_LW;
 			const double f = a.f2[n];
-			const double g = a.gain_inv ? 1.0 / (sqrt(f) * nt) : sqrt(f) / nt;
+			const double g = a.gain == 0 ? sqrt(f) / nt : (a.gain == 1 ? 1.0 / (sqrt(f) * nt) : a.h / (f * nt));
 			double2 &v = Z[(size_t)mode_pos(n, a) * FFT_LW + l];
 			v = make_double2(g * v.x, g * v.y);
 		}
@@ -193,8 +194,13 @@ __global__ __launch_bounds__(FFT_BT) void fa_fft_kernel(FftArgs a)
 			}
 		}else{
 			const unsigned la = l0 + l, lb = l0 + FFT_LW + l;
-			if(la < a.loc) a.out0[(size_t)t * a.loc + la] = v.x;
-			if(lb < a.loc) a.out0[(size_t)t * a.loc + lb] = v.y;
+			if(a.gain == 2){
+				if(la < a.loc) a.out0[(size_t)t * a.loc + la] += v.x;
+				if(lb < a.loc) a.out0[(size_t)t * a.loc + lb] += v.y;
+			}else{
+				if(la < a.loc) a.out0[(size_t)t * a.loc + la] = v.x;
+				if(lb < a.loc) a.out0[(size_t)t * a.loc + lb] = v.y;
+			}
 		}
 	}
 }
@@ -203,7 +209,7 @@ static FftArgs fft_args(ephmc_ctx *c, int mode)
 {
 	FftArgs a{};
 	a.twid = c->d_twiddle;
-	a.f2 = c->d_f2;
+	a.f2 = c->d_f2mom;
 	a.loc = c->loc;
 	a.nt = c->nt;
 	a.pow2 = (c->nt & (c->nt - 1)) == 0;
@@ -229,6 +235,7 @@ static int launch_fft(ephmc_ctx *c, const FftArgs &a)
 int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick)
 {
 	FftArgs a = fft_args(c, FFT_EVOLVE);
+	a.f2 = c->d_f2;
 	a.in0 = x; a.in1 = p; a.out0 = x; a.out1 = p;
 	a.kickF = pdot; a.kick = kick;
 	a.c = cos(h) / c->nt;
@@ -236,12 +243,99 @@ int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, 
 	return launch_fft(c, a);
 }
 
+// inverse_sqrt = 0: momentum refresh with the kinetic frequencies (organic_bosons.c:76-100,
+// omega -> sqrt(cfa_mass^2 + omega^2) under CLASSICAL_FOURIER_ACC); 1: free-boson draw with the
+// bare frequencies (:55-74)
 int fa_filter(ephmc_ctx *c, const double *in, double *out, int inverse_sqrt)
 {
 	FftArgs a = fft_args(c, FFT_FILTER);
 	a.in0 = in; a.out0 = out;
-	a.gain_inv = inverse_sqrt;
+	a.gain = inverse_sqrt ? 1 : 0;
+	if(inverse_sqrt) a.f2 = c->d_f2;
 	return launch_fft(c, a);
+}
+
+// x += c2r((h / f_n) r2c(p)) / nt with the kinetic frequencies: the drift of
+// evolve_bosons_approx_step (organic_bosons.c:152-177)
+int fa_drift(ephmc_ctx *c, const double *p, double *x, double h)
+{
+	FftArgs a = fft_args(c, FFT_FILTER);
+	a.in0 = p; a.out0 = x;
+	a.gain = 2;
+	a.h = h;
+	return launch_fft(c, a);
+}
+
+// boson_force (organic_bosons.c:179-194): pdot (+)= -omega x + (x_{t+1} - 2 x_t + x_{t-1}) / omega
+__global__ void boson_force_kernel(const double *__restrict__ x, double *__restrict__ pdot, double omega, unsigned loc,
+		size_t dim, int overwrite)
+{
+	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < dim; e += (size_t)gridDim.x * blockDim.x){
+		const size_t nxt = (e + loc < dim) ? e + loc : e + loc - dim;
+		const size_t prv = (e >= loc) ? e - loc : e + dim - loc;
+		const double v = x[e];
+		double f = overwrite ? 0.0 : pdot[e];
+		f -= omega * v;
+		f += (x[nxt] - 2 * v + x[prv]) / omega;
+		pdot[e] = f;
+	}
+}
+
+int boson_force(ephmc_ctx *c, const double *x, double *pdot, int overwrite)
+{
+	boson_force_kernel<<<1024, 256, 0, c->stream>>>(x, pdot, c->cfg.omega, c->loc, c->dim, overwrite);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+__global__ void axpy_kernel(double *__restrict__ y, double a, const double *__restrict__ x, size_t n)
+{
+	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) y[e] += a * x[e];
+}
+
+int axpy(ephmc_ctx *c, double *y, double a, const double *x, size_t n)
+{
+	axpy_kernel<<<1024, 256, 0, c->stream>>>(y, a, x, n);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+// NO_FOURIER_ACC kinetic energy .5 * dim * average_sq(p, dim) (organic_hmc.c:57-59)
+__global__ __launch_bounds__(256) void sumsq_kernel(const double *__restrict__ p, size_t n, double *__restrict__ part)
+{
+	__shared__ double red[256];
+	double s = 0;
+	for(size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) s += p[e] * p[e];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1){
+		if((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void plain_energy_final(const double *__restrict__ part, unsigned nb, double dim, double *__restrict__ out)
+{
+	__shared__ double red[256];
+	double s = 0;
+	for(unsigned i = threadIdx.x; i < nb; i += 256) s += part[i];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1){
+		if((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) *out = .5 * dim * (red[0] / dim);
+}
+
+int plain_energy(ephmc_ctx *c, const double *p, double *scal_out)
+{
+	sumsq_kernel<<<c->red_blocks, 256, 0, c->stream>>>(p, c->dim, c->d_red);
+	EPHMC_LAUNCHED();
+	plain_energy_final<<<1, 256, 0, c->stream>>>(c->d_red, c->red_blocks, (double)c->dim, scal_out);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
 }
 
 __global__ void sum_partials_kernel(const double *__restrict__ part, unsigned n, double *__restrict__ out)
@@ -324,9 +418,18 @@ __global__ void hamilton_kernel(double *scal, int logdet_slot, int ep_slot, int 
 	scal[out_slot] = -scal[logdet_slot] * flavors + scal[ep_slot] + scal[sb_slot];
 }
 
+// energy_momenta (organic_hmc.c:56-65)
+int kinetic_energy(ephmc_ctx *c, const double *p, double *scal_out)
+{
+	if(c->cfg.no_fourier_acc) return plain_energy(c, p, scal_out);
+	return fa_energy(c, p, scal_out);
+}
+
+// H = -flavors logdet + E_p + S_B(x); p null: E_p = 0 (ZERO_FILLING, organic_hmc.c:252)
 int hamilton_dev(ephmc_ctx *c, const double *x, const double *p, int logdet_slot, int out_slot)
 {
-	EPHMC_TRY(fa_energy(c, p, c->d_scal + SCAL_EP));
+	if(p) EPHMC_TRY(kinetic_energy(c, p, c->d_scal + SCAL_EP));
+	else EPHMC_TRY(set_scalar(c, SCAL_EP, 0.0));
 	EPHMC_TRY(boson_action(c, x, c->d_scal + SCAL_BOSON));
 	hamilton_kernel<<<1, 1, 0, c->stream>>>(c->d_scal, logdet_slot, SCAL_EP, SCAL_BOSON, out_slot, (double)c->cfg.flavors);
 	EPHMC_LAUNCHED();

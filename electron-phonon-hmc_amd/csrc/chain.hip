@@ -395,6 +395,33 @@ int chain_forward(ephmc_ctx *c, int prec, const void *src, void *dst, void *ck, 
 	}
 }
 
+// dst = B_{tau_last} ... B_{tau0} src over the slices tau0, tau0+dir, ... (ntau of them) in the
+// precision prec; src is fp64 (converted for fp32) or null for the identity.
+int chain_range(ephmc_ctx *c, int prec, const double *src, void *dst, int tau0, int dir, int ntau)
+{
+	if(prec == EPHMC_FP64){
+		ChainArgs<double> a = make_args<double>(c, c->d_coef64);
+		a.src = src;
+		a.dst = (double *)dst;
+		a.ck = nullptr;
+		a.ck_every = 1;
+		a.tau0 = tau0; a.dir = dir; a.ntau = ntau;
+		return launch_chain<double>(c, a, chain_cols(c, prec));
+	}
+	float *s32 = nullptr;
+	if(src){
+		s32 = (float *)c->d_seg;        // scratch: the segment buffer holds at least N^2 values
+		EPHMC_TRY(convert_f64_to_f32(c, src, s32, c->mat));
+	}
+	ChainArgs<float> a = make_args<float>(c, c->d_coef32);
+	a.src = s32;
+	a.dst = (float *)dst;
+	a.ck = nullptr;
+	a.ck_every = 1;
+	a.tau0 = tau0; a.dir = dir; a.ntau = ntau;
+	return launch_chain<float>(c, a, chain_cols(c, prec));
+}
+
 // apply_exp_mu_K_to_M with partial pair products (revert/stop, organic_dense_matrix.c:186-194)
 // for a single slice whose 3N couplings are x_tau.  Uses d_coef64 slot 0 as scratch.
 int apply_exp_partial(ephmc_ctx *c, const double *x_tau, double *M, int revert, int stop)
@@ -460,6 +487,61 @@ __global__ void add_identity_kernel(double *__restrict__ M, unsigned n)
 int add_identity(ephmc_ctx *c, double *M)
 {
 	add_identity_kernel<<<(c->ns + 255) / 256, 256, 0, c->stream>>>(M, c->ns);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+// SINGLE_PARTICLE log det = log tr(B_0 ... B_{nt-1}) (organic_dense_matrix.c:550-551); the force
+// factor exp(-logdet) (organic_hmc.c:121) goes to fs_slot when >= 0.
+template <typename R>
+__global__ __launch_bounds__(256) void log_trace_kernel(const R *__restrict__ M, unsigned n, double *__restrict__ scal,
+		int ld_slot, int fs_slot)
+{
+	__shared__ double red[256];
+	double s = 0;
+	for(unsigned i = threadIdx.x; i < n; i += 256) s += (double)M[(size_t)i * (n + 1)];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1){
+		if((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0){
+		const double ld = log(red[0]);
+		scal[ld_slot] = ld;
+		if(fs_slot >= 0) scal[fs_slot] = exp(-ld);
+	}
+}
+
+int log_trace(ephmc_ctx *c, int prec, const void *M, int ld_slot, int fs_slot)
+{
+	if(prec == EPHMC_FP64) log_trace_kernel<double><<<1, 256, 0, c->stream>>>((const double *)M, c->ns, c->d_scal, ld_slot, fs_slot);
+	else log_trace_kernel<float><<<1, 256, 0, c->stream>>>((const float *)M, c->ns, c->d_scal, ld_slot, fs_slot);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+template <typename R>
+__global__ void set_identity_kernel(R *__restrict__ M, unsigned n)
+{
+	const size_t nn = (size_t)n * n;
+	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x)
+		M[e] = (e % (n + 1) == 0) ? (R)1 : (R)0;
+}
+
+int set_identity(ephmc_ctx *c, int prec, void *M)
+{
+	if(prec == EPHMC_FP64) set_identity_kernel<double><<<1024, 256, 0, c->stream>>>((double *)M, c->ns);
+	else set_identity_kernel<float><<<1024, 256, 0, c->stream>>>((float *)M, c->ns);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+__global__ void set_scalar_kernel(double *scal, int slot, double v){ scal[slot] = v; }
+
+int set_scalar(ephmc_ctx *c, int slot, double v)
+{
+	set_scalar_kernel<<<1, 1, 0, c->stream>>>(c->d_scal, slot, v);
 	EPHMC_LAUNCHED();
 	return EPHMC_OK;
 }

@@ -86,8 +86,10 @@ using namespace ephmc;
 
 extern "C" int ephmc_cc_correlator(ephmc_ctx *c, const double *x, const double *minv, double logdet, double *greens_host)
 {
-	(void)logdet;
-	if(!c || !x || !minv || !greens_host){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	if(!c || !x || !greens_host || (!minv && !c->cfg.zero_filling && !c->cfg.single_particle)){
+		set_error("null argument");
+		return EPHMC_ERR_ARG;
+	}
 	const unsigned ns = c->ns, nt = c->nt;
 	const size_t mat = c->mat;
 	if(!c->d_Rt0){
@@ -101,12 +103,15 @@ extern "C" int ephmc_cc_correlator(ephmc_ctx *c, const double *x, const double *
 	}
 	EPHMC_TRY(timer_begin(c, "correlator"));
 	EPHMC_TRY(coefficients(c, x, false));
+	// ZERO_FILLING / SINGLE_PARTICLE start both propagators from the identity
+	// (organic_dense_matrix.c:458-470)
+	const bool ident = c->cfg.zero_filling || c->cfg.single_particle;
 	// R_t0: slices nt-1 .. 0 applied to M_hat^{-1}, the state stored before every slice: step i
 	// (slice nt-1-i) stores slot nt-i, so slot s >= 1 is stored entry nt-s; slot 0 is the result
-	EPHMC_TRY(chain_forward(c, EPHMC_FP64, minv, c->d_Tfin, c->d_Rt0, 1, 1, false));
+	EPHMC_TRY(chain_forward(c, EPHMC_FP64, minv, c->d_Tfin, c->d_Rt0, 1, 1, ident));
 	// R_0t: transposed inverse, then slices 0 .. nt-1 ascending; stored entry tau = state at tau
-	EPHMC_TRY(transpose_plus_identity(c, EPHMC_FP64, minv, c->d_gj[0], 0));
-	EPHMC_TRY(chain_forward(c, EPHMC_FP64, c->d_gj[0], c->d_gj[1], c->d_R0t, 1, 0, false));
+	if(!ident) EPHMC_TRY(transpose_plus_identity(c, EPHMC_FP64, minv, c->d_gj[0], 0));
+	EPHMC_TRY(chain_forward(c, EPHMC_FP64, c->d_gj[0], c->d_gj[1], c->d_R0t, 1, 0, ident));
 	const unsigned bx = (ns + CC_BT * CC_ROWS - 1) / (CC_BT * CC_ROWS);
 	const dim3 grid(bx, ns);
 	double *partial = c->d_gj[1];                   // ns * bx * 9 doubles <= N^2
@@ -115,7 +120,8 @@ extern "C" int ephmc_cc_correlator(ephmc_ctx *c, const double *x, const double *
 		cc_contract_kernel<<<grid, CC_BT, 0, c->stream>>>(rt, c->d_R0t + (size_t)tau * mat, x, c->d_nb,
 			c->cfg.t[0], c->cfg.t[1], c->cfg.t[2], c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], ns, tau, partial);
 		EPHMC_LAUNCHED();
-		cc_sum_kernel<<<1, 64, 0, c->stream>>>(partial, bx * ns, 1.0 / ns, c->d_ccg + (size_t)tau * 9);
+		cc_sum_kernel<<<1, 64, 0, c->stream>>>(partial, bx * ns, c->cfg.single_particle ? exp(-logdet) : 1.0 / ns,
+			c->d_ccg + (size_t)tau * 9);
 		EPHMC_LAUNCHED();
 	}
 	EPHMC_TRY(timer_end(c, "correlator"));

@@ -159,34 +159,88 @@ static unsigned adjoint_cols_auto(unsigned ns, size_t rsize)
 	return 1;
 }
 
+// Adjoint force for the current coefficients and checkpoints.  minv: M_hat^{-1} (fp64), or
+// null for SINGLE_PARTICLE (the sweep starts from the identity).
+static int force_dispatch(ephmc_ctx *c, int prec, const double *minv, double *pdot, const double *fscale)
+{
+	if(c->cfg.constant_force) return force_constant(c, prec, minv, pdot, fscale);
+	if(!minv) EPHMC_TRY(set_identity(c, prec, prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32));
+	return force_adjoint(c, prec, minv, pdot, 0, nullptr, fscale);
+}
+
 // ---- one MD force evaluation at field x: coefficients, forward chain with checkpoints,
-// M_hat, Gauss-Jordan inverse, adjoint sweep; p += kick * F when p != null.
-static int md_force(ephmc_ctx *c, const double *x, double *pdot, double kick, double *p)
+// M_hat, Gauss-Jordan inverse, adjoint sweep (organic_hmc.c:178-183 + force :86-134).
+// SINGLE_PARTICLE: no +1 and no inverse; the sweep starts from the identity and the force is
+// scaled by exp(-log tr M_hat).  CONSTANT_FORCE: the slice-0 force on every slice.
+// NO_FOURIER_ACC: the boson force is added (organic_hmc.c:131).
+static int md_force(ephmc_ctx *c, const double *x, double *pdot)
 {
 	const int prec = c->cfg.precision;
+	const bool sp = c->cfg.single_particle != 0;
 	EPHMC_TRY(timer_begin(c, "chain"));
 	EPHMC_TRY(coefficients(c, x, prec != EPHMC_FP64));
 	void *tfin = prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32;
 	EPHMC_TRY(chain_forward(c, prec, nullptr, tfin, c->d_ck, c->ckpt, 0, true));
-	EPHMC_TRY(transpose_plus_identity(c, prec, tfin, c->d_mhat, 1));
+	if(!sp) EPHMC_TRY(transpose_plus_identity(c, prec, tfin, c->d_mhat, 1));
 	EPHMC_TRY(timer_end(c, "chain"));
-	EPHMC_TRY(timer_begin(c, "gj"));
-	EPHMC_TRY(gj_logdet_inverse(c, c->d_mhat));
-	EPHMC_TRY(timer_end(c, "gj"));
+	const double *minv = c->d_mhat, *fscale = nullptr;
+	if(sp){
+		EPHMC_TRY(log_trace(c, prec, tfin, SCAL_LOGDET, SCAL_FSCALE));
+		fscale = c->d_scal + SCAL_FSCALE;
+		minv = nullptr;
+	}else{
+		EPHMC_TRY(timer_begin(c, "gj"));
+		EPHMC_TRY(gj_logdet_inverse(c, c->d_mhat));
+		EPHMC_TRY(timer_end(c, "gj"));
+	}
 	EPHMC_TRY(timer_begin(c, "adjoint"));
-	EPHMC_TRY(force_adjoint(c, prec, c->d_mhat, pdot, kick, p));
+	EPHMC_TRY(force_dispatch(c, prec, minv, pdot, fscale));
 	EPHMC_TRY(timer_end(c, "adjoint"));
+	if(c->cfg.no_fourier_acc) EPHMC_TRY(boson_force(c, x, pdot, 0));
 	return EPHMC_OK;
 }
 
-// FP64 M_hat^{-1} and log det of field x into d_mhat / scal[SCAL_LOGDET] (Metropolis quality)
-// (slices applied from nt-1 down to 0 as in construct_M_hat, organic_dense_matrix.c:256-258)
+// FP64 M_hat and log det of field x (Metropolis quality), slices applied from nt-1 down to 0 as
+// in construct_M_hat (organic_dense_matrix.c:256-258): d_mhat = M_hat^{-1} and the log det in
+// scal[SCAL_LOGDET]; SINGLE_PARTICLE: d_mhat = B..B and log tr; ZERO_FILLING: d_mhat = B..B.
 static int exact_logdet(ephmc_ctx *c, const double *x)
 {
 	EPHMC_TRY(coefficients(c, x, false));
 	EPHMC_TRY(chain_forward(c, EPHMC_FP64, nullptr, c->d_mhat, nullptr, 0, 1, true));
+	if(c->cfg.zero_filling) return set_scalar(c, SCAL_LOGDET, 0.0);
+	if(c->cfg.single_particle) return log_trace(c, EPHMC_FP64, c->d_mhat, SCAL_LOGDET, -1);
 	EPHMC_TRY(add_identity(c, c->d_mhat));
 	EPHMC_TRY(gj_logdet_inverse(c, c->d_mhat));
+	return EPHMC_OK;
+}
+
+// update_x (organic_hmc.c:136-165) for the modes without the fused Fourier-accelerated drift
+static int update_x_dev(ephmc_ctx *c, double *x, double *p, double h, double half)
+{
+	if(c->cfg.no_fourier_acc) return axpy(c, x, h * fabs(half), p, c->dim);
+	// CLASSICAL_FOURIER_ACC: cfa_steps approximate sub-steps with the boson force
+	const unsigned cs = c->cfg.cfa_steps;
+	const double hh = h / cs;
+	if(half != .5){
+		EPHMC_TRY(boson_force(c, x, c->d_pdot, 1));
+		EPHMC_TRY(axpy(c, p, .5 * hh, c->d_pdot, c->dim));
+		for(unsigned k = 0; k < cs / 2; k++){
+			EPHMC_TRY(fa_drift(c, p, x, hh));
+			EPHMC_TRY(boson_force(c, x, c->d_pdot, 1));
+			EPHMC_TRY(axpy(c, p, hh, c->d_pdot, c->dim));
+		}
+		EPHMC_TRY(fa_drift(c, p, x, .5 * hh));
+	}
+	if(half != -.5){
+		EPHMC_TRY(fa_drift(c, p, x, .5 * hh));
+		for(unsigned k = 0; k < cs / 2; k++){
+			EPHMC_TRY(boson_force(c, x, c->d_pdot, 1));
+			EPHMC_TRY(axpy(c, p, hh, c->d_pdot, c->dim));
+			EPHMC_TRY(fa_drift(c, p, x, hh));
+		}
+		EPHMC_TRY(boson_force(c, x, c->d_pdot, 1));
+		EPHMC_TRY(axpy(c, p, .5 * hh, c->d_pdot, c->dim));
+	}
 	return EPHMC_OK;
 }
 
@@ -195,11 +249,20 @@ static int leap_frog_dev(ephmc_ctx *c, double *x, double *p)
 {
 	const unsigned steps = c->cfg.steps;
 	const double h = steps ? c->traj_length / steps : 2 * c->traj_length;
+	if(c->cfg.no_fourier_acc || c->cfg.classical_fourier_acc){
+		EPHMC_TRY(update_x_dev(c, x, p, h, .5));
+		for(unsigned s = 1; s <= steps; s++){
+			EPHMC_TRY(md_force(c, x, c->d_pdot));
+			EPHMC_TRY(axpy(c, p, h, c->d_pdot, c->dim));
+			EPHMC_TRY(update_x_dev(c, x, p, h, s == steps ? -.5 : 1.0));
+		}
+		return EPHMC_OK;
+	}
 	EPHMC_TRY(timer_begin(c, "fft"));
 	EPHMC_TRY(fa_evolve(c, x, p, 0.5 * h, nullptr, 0));
 	EPHMC_TRY(timer_end(c, "fft"));
 	for(unsigned s = 1; s <= steps; s++){
-		EPHMC_TRY(md_force(c, x, c->d_pdot, 0, nullptr));
+		EPHMC_TRY(md_force(c, x, c->d_pdot));
 		const double drift = (s == steps) ? 0.5 * h : h;
 		// kick p += h F fused into the drift's load
 		EPHMC_TRY(timer_begin(c, "fft"));
@@ -207,6 +270,16 @@ static int leap_frog_dev(ephmc_ctx *c, double *x, double *p)
 		EPHMC_TRY(timer_end(c, "fft"));
 	}
 	return EPHMC_OK;
+}
+
+// momenta from dim normals (sample_fourier_momenta, organic_bosons.c:76-100)
+static int momenta(ephmc_ctx *c, const double *normals, double *p)
+{
+	if(c->cfg.no_fourier_acc){
+		EPHMC_HIP(hipMemcpyAsync(p, normals, c->dim * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+		return EPHMC_OK;
+	}
+	return fa_filter(c, normals, p, 0);
 }
 
 static int upload_normals(ephmc_ctx *c, const double *normals)
@@ -243,10 +316,8 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 	*out = nullptr;
 	if(cfg->l1 == 0 || cfg->l2 == 0 || cfg->nt == 0){ set_error("empty lattice"); return EPHMC_ERR_ARG; }
 	if(cfg->precision != EPHMC_FP64 && cfg->precision != EPHMC_MIXED){ set_error("precision"); return EPHMC_ERR_ARG; }
-	if(cfg->zero_filling || cfg->single_particle || cfg->constant_force || cfg->no_fourier_acc || cfg->classical_fourier_acc){
-		set_error("flags ZERO_FILLING, SINGLE_PARTICLE, CONSTANT_FORCE, NO_FOURIER_ACC and CLASSICAL_FOURIER_ACC are not provided by this build");
-		return EPHMC_ERR_UNSUPPORTED;
-	}
+	if(cfg->classical_fourier_acc && cfg->cfa_steps == 0){ set_error("CLASSICAL_FOURIER_ACC needs cfa_steps >= 1"); return EPHMC_ERR_ARG; }
+	if(cfg->no_fourier_acc && cfg->classical_fourier_acc){ set_error("NO_FOURIER_ACC and CLASSICAL_FOURIER_ACC exclude each other"); return EPHMC_ERR_ARG; }
 	const unsigned ns = cfg->l1 * cfg->l2;
 	if(ns > 2048){ set_error("lattice of %u sites exceeds 2048", ns); return EPHMC_ERR_UNSUPPORTED; }
 	ephmc_ctx *c = new ephmc_ctx();
@@ -297,7 +368,7 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 		|| (rc = dalloc(&c->d_piv, ns)) || (rc = dalloc(&c->d_src, ns)) || (rc = dalloc(&c->d_diag, ns))
 		|| (rc = dalloc(&c->d_dlu, 32 * 32))
 		|| (rc = dalloc(&c->d_scal, SCAL_COUNT))
-		|| (rc = dalloc(&c->d_twiddle, 2 * (size_t)c->nt)) || (rc = dalloc(&c->d_f2, c->nt))
+		|| (rc = dalloc(&c->d_twiddle, 2 * (size_t)c->nt)) || (rc = dalloc(&c->d_f2, c->nt)) || (rc = dalloc(&c->d_f2mom, c->nt))
 		|| (rc = dalloc(&c->d_red, std::max<size_t>(2 * c->red_blocks, fft_blocks)))
 		|| (rc = dalloc(&c->d_x, c->dim)) || (rc = dalloc(&c->d_p, c->dim)) || (rc = dalloc(&c->d_pdot, c->dim))
 		|| (rc = dalloc(&c->d_xnew, c->dim)) || (rc = dalloc(&c->d_normals, c->dim))
@@ -311,18 +382,23 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 		if(hipEventCreateWithFlags(&c->stage_ev[s], hipEventDisableTiming) != hipSuccess){ set_error("event"); return fail(EPHMC_ERR_HIP); }
 		(void)hipEventRecord(c->stage_ev[s], c->stream);
 	}
-	std::vector<double> tw(2 * (size_t)c->nt), f2(c->nt);
+	std::vector<double> tw(2 * (size_t)c->nt), f2(c->nt), f2m(c->nt);
 	const double pi = 3.141592653589793;
+	// kinetic frequencies: omega -> sqrt(cfa_mass^2 + omega^2) under CLASSICAL_FOURIER_ACC
+	// (organic_hmc.c:61-62, organic_bosons.c:83-84,162-163)
+	const double om_k = cfg->classical_fourier_acc ? sqrt(pow(cfg->cfa_mass, 2) + pow(cfg->omega, 2)) : cfg->omega;
 	for(unsigned j = 0; j < c->nt; j++){
 		tw[2 * j] = cos(2 * pi * j / c->nt);
 		tw[2 * j + 1] = -sin(2 * pi * j / c->nt);
 		const double s = 2 * sin(pi / c->nt * j);
 		f2[j] = cfg->omega + s * s / cfg->omega;   // organic_bosons.c:137
+		f2m[j] = om_k + s * s / om_k;
 	}
 	if(hipMemcpy(c->d_nb, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess
 		|| hipMemcpy(c->d_link, link.data(), link.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess
 		|| hipMemcpy(c->d_twiddle, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
 		|| hipMemcpy(c->d_f2, f2.data(), f2.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+		|| hipMemcpy(c->d_f2mom, f2m.data(), f2m.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
 		|| hipMemset(c->d_scal, 0, SCAL_COUNT * sizeof(double)) != hipSuccess){
 		set_error("initial uploads failed");
 		return fail(EPHMC_ERR_HIP);
@@ -336,7 +412,7 @@ int ephmc_destroy(ephmc_ctx *c)
 	if(!c) return EPHMC_OK;
 	if(c->stream) (void)hipStreamSynchronize(c->stream);
 	void *bufs[] = {c->d_nb, c->d_link, c->d_coef64, c->d_coef32, c->d_ck, c->d_seg, c->d_partial[0], c->d_partial[1], c->d_Tfin,
-		c->d_Tfin32, c->d_gj[0], c->d_gj[1], c->d_piv, c->d_src, c->d_diag, c->d_dlu, c->d_scal, c->d_twiddle, c->d_f2, c->d_red,
+		c->d_Tfin32, c->d_gj[0], c->d_gj[1], c->d_piv, c->d_src, c->d_diag, c->d_dlu, c->d_scal, c->d_twiddle, c->d_f2, c->d_f2mom, c->d_red,
 		c->d_x, c->d_p, c->d_pdot, c->d_xnew, c->d_normals, c->d_minv, c->d_mhat, c->d_Rt0, c->d_R0t, c->d_ccg};
 	for(void *b : bufs) if(b) (void)hipFree(b);
 	tl_release(c);
@@ -412,14 +488,23 @@ int ephmc_construct_M_hat(ephmc_ctx *c, const double *x, double *M_hat)
 {
 	if(!c || !x || !M_hat){ set_error("null argument"); return EPHMC_ERR_ARG; }
 	EPHMC_TRY(coefficients(c, x, false));
-	// B_0 (B_1 ( ... B_{nt-1} I)) in the reference's order (slices applied from nt-1 down to 0)
+	// B_0 (B_1 ( ... B_{nt-1} I)) in the reference's order (slices applied from nt-1 down to 0);
+	// no identity for ZERO_FILLING / SINGLE_PARTICLE (organic_dense_matrix.c:260-264)
 	EPHMC_TRY(chain_forward(c, EPHMC_FP64, nullptr, M_hat, nullptr, 0, 1, true));
+	if(c->cfg.zero_filling || c->cfg.single_particle) return EPHMC_OK;
 	return add_identity(c, M_hat);
 }
 
 int ephmc_logdet_inv(ephmc_ctx *c, double *M_hat, double *logdet, int *sign)
 {
 	if(!c || !M_hat){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	if(c->cfg.single_particle){
+		// log tr M_hat, M_hat unchanged (organic_dense_matrix.c:550-551)
+		EPHMC_TRY(log_trace(c, EPHMC_FP64, M_hat, SCAL_LOGDET, -1));
+		if(logdet) EPHMC_TRY(read_scal(c, SCAL_LOGDET, logdet));
+		if(sign) *sign = 1;
+		return EPHMC_OK;
+	}
 	EPHMC_TRY(timer_begin(c, "gj"));
 	EPHMC_TRY(gj_logdet_inverse(c, M_hat));
 	EPHMC_TRY(timer_end(c, "gj"));
@@ -430,17 +515,23 @@ int ephmc_logdet_inv(ephmc_ctx *c, double *M_hat, double *logdet, int *sign)
 
 int ephmc_force(ephmc_ctx *c, const double *x, const double *M_inv, double logdet, double *p_dot)
 {
-	(void)logdet;
-	if(!c || !x || !M_inv || !p_dot){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	if(!c || !x || !p_dot || (!M_inv && !c->cfg.single_particle)){ set_error("null argument"); return EPHMC_ERR_ARG; }
 	const int prec = c->cfg.precision;
 	EPHMC_TRY(coefficients(c, x, prec != EPHMC_FP64));
 	void *tfin = prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32;
 	EPHMC_TRY(timer_begin(c, "chain"));
 	EPHMC_TRY(chain_forward(c, prec, nullptr, tfin, c->d_ck, c->ckpt, 0, true));
 	EPHMC_TRY(timer_end(c, "chain"));
+	const double *fscale = nullptr;
+	if(c->cfg.single_particle){
+		EPHMC_TRY(set_scalar(c, SCAL_FSCALE, exp(-logdet)));
+		fscale = c->d_scal + SCAL_FSCALE;
+		M_inv = nullptr;
+	}
 	EPHMC_TRY(timer_begin(c, "adjoint"));
-	EPHMC_TRY(force_adjoint(c, prec, M_inv, p_dot, 0, nullptr));
+	EPHMC_TRY(force_dispatch(c, prec, M_inv, p_dot, fscale));
 	EPHMC_TRY(timer_end(c, "adjoint"));
+	if(c->cfg.no_fourier_acc) EPHMC_TRY(boson_force(c, x, p_dot, 0));
 	return EPHMC_OK;
 }
 
@@ -453,13 +544,13 @@ int ephmc_sample_free_bosons(ephmc_ctx *c, const double *normals, double *x)
 int ephmc_sample_fourier_momenta(ephmc_ctx *c, const double *normals, double *p)
 {
 	if(!c || !normals || !p){ set_error("null argument"); return EPHMC_ERR_ARG; }
-	return fa_filter(c, normals, p, 0);
+	return momenta(c, normals, p);
 }
 
 int ephmc_energy_momenta(ephmc_ctx *c, const double *p, double *energy)
 {
 	if(!c || !p || !energy){ set_error("null argument"); return EPHMC_ERR_ARG; }
-	EPHMC_TRY(fa_energy(c, p, c->d_scal + SCAL_EP));
+	EPHMC_TRY(kinetic_energy(c, p, c->d_scal + SCAL_EP));
 	return read_scal(c, SCAL_EP, energy);
 }
 
@@ -467,6 +558,13 @@ int ephmc_evolve_bosons(ephmc_ctx *c, double *x, double *p, double h)
 {
 	if(!c || !x || !p){ set_error("null argument"); return EPHMC_ERR_ARG; }
 	return fa_evolve(c, x, p, h, nullptr, 0);
+}
+
+int ephmc_update_x(ephmc_ctx *c, double *x, double *p, double h, double half)
+{
+	if(!c || !x || !p){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	if(c->cfg.no_fourier_acc || c->cfg.classical_fourier_acc) return update_x_dev(c, x, p, h, half);
+	return fa_evolve(c, x, p, fabs(half) * h, nullptr, 0);
 }
 
 int ephmc_hamilton(ephmc_ctx *c, const double *x, double logdet, double energyP, double *H)
@@ -513,10 +611,24 @@ int ephmc_trajectory(ephmc_ctx *c, const double *normals, double uniform, ephmc_
 {
 	if(!c || !normals){ set_error("null argument"); return EPHMC_ERR_ARG; }
 	if(!c->chain_ready){ set_error("ephmc_chain_init has not run"); return EPHMC_ERR_ARG; }
-	// organic_hmc.c:213-243
 	EPHMC_TRY(upload_normals(c, normals));
+	if(c->cfg.zero_filling){
+		// organic_hmc.c:246-254: a fresh free-boson field every trajectory, M_hat = B..B, no
+		// Metropolis step (the normals are sample_free_bosons' draw; no uniform is used)
+		EPHMC_TRY(fa_filter(c, c->d_normals, c->d_x, 1));
+		EPHMC_TRY(exact_logdet(c, c->d_x));
+		EPHMC_HIP(hipMemcpyAsync(c->d_minv, c->d_mhat, c->mat * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+		EPHMC_TRY(set_scalar(c, SCAL_LOGDET_CHAIN, 0.0));
+		EPHMC_TRY(hamilton_dev(c, c->d_x, nullptr, SCAL_LOGDET_CHAIN, SCAL_ENERGY));
+		EPHMC_TRY(set_scalar(c, SCAL_ACC, 1.0));
+		EPHMC_TRY(set_scalar(c, SCAL_BOLTZ, 1.0));
+		EPHMC_TRY(set_scalar(c, SCAL_DH, 0.0));
+		if(rec) return ephmc_last_record(c, rec);
+		return EPHMC_OK;
+	}
+	// organic_hmc.c:213-243
 	EPHMC_HIP(hipMemcpyAsync(c->d_xnew, c->d_x, c->dim * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-	EPHMC_TRY(fa_filter(c, c->d_normals, c->d_p, 0));
+	EPHMC_TRY(momenta(c, c->d_normals, c->d_p));
 	EPHMC_TRY(hamilton_dev(c, c->d_xnew, c->d_p, SCAL_LOGDET_CHAIN, SCAL_H_OLD));
 	EPHMC_TRY(leap_frog_dev(c, c->d_xnew, c->d_p));
 	EPHMC_TRY(timer_begin(c, "metropolis"));

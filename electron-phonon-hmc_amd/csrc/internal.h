@@ -117,7 +117,8 @@ struct ephmc_ctx {
 
 	// bosons
 	double *d_twiddle;                // [nt/2] complex e^{-2 pi i k/nt}
-	double *d_f2;                     // [nt] freq2(n)
+	double *d_f2;                     // [nt] freq2(n) with omega
+	double *d_f2mom;                  // [nt] kinetic frequencies: d_f2, or with sqrt(cfa_mass^2 + omega^2) (CFA)
 	double *d_red;                    // reduction partials
 	unsigned red_blocks;
 
@@ -156,6 +157,7 @@ enum Scalar {            // layout of d_scal
 	SCAL_BOSON = 11,      // S_B(x)
 	SCAL_GJ_FLAG = 12,    // a multiplier above 1 was seen by the Gauss-Jordan fast path
 	SCAL_GJ_FALLBACKS = 13, // number of pivoted fallback inversions run
+	SCAL_FSCALE = 14,     // force factor exp(-logdet) of SINGLE_PARTICLE
 	SCAL_COUNT = 16
 };
 
@@ -167,23 +169,33 @@ int timer_end(ephmc_ctx *c, const char *group);
 int coefficients(ephmc_ctx *c, const double *x, bool want32);
 int chain_forward(ephmc_ctx *c, int prec, const void *src, void *dst, void *ck, unsigned ck_every,
 		int descending, bool from_identity);
+int chain_range(ephmc_ctx *c, int prec, const double *src, void *dst, int tau0, int dir, int ntau);
 int apply_exp_partial(ephmc_ctx *c, const double *x_tau, double *M, int revert, int stop);
 int transpose_plus_identity(ephmc_ctx *c, int prec, const void *T, double *M, int add_identity);
 int convert_f64_to_f32(ephmc_ctx *c, const double *a, float *b, size_t n);
 int add_identity(ephmc_ctx *c, double *M);
+int log_trace(ephmc_ctx *c, int prec, const void *M, int ld_slot, int fs_slot);
+int set_identity(ephmc_ctx *c, int prec, void *M);
+int set_scalar(ephmc_ctx *c, int slot, double v);
 
 // gj.hip
 int gj_logdet_inverse(ephmc_ctx *c, double *M);
 
 // adjoint.hip
-int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p);
+int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p, const double *fscale);
+int force_constant(ephmc_ctx *c, int prec, const double *minv, double *pdot, const double *fscale);
 
 // bosons.hip
 int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick);
 int fa_filter(ephmc_ctx *c, const double *in, double *out, int inverse_sqrt);
 int fa_energy(ephmc_ctx *c, const double *p, double *scal_out);
+int fa_drift(ephmc_ctx *c, const double *p, double *x, double h);
+int boson_force(ephmc_ctx *c, const double *x, double *pdot, int overwrite);
+int axpy(ephmc_ctx *c, double *y, double a, const double *x, size_t n);
+int plain_energy(ephmc_ctx *c, const double *p, double *scal_out);
 int boson_action(ephmc_ctx *c, const double *x, double *scal_out);
 int hamilton_dev(ephmc_ctx *c, const double *x, const double *p, int logdet_slot, int out_slot);
+int kinetic_energy(ephmc_ctx *c, const double *p, double *scal_out);
 int metropolis(ephmc_ctx *c, double uniform);
 
 // tl.hip

@@ -64,6 +64,7 @@ def _load():
         "ephmc_evolve_bosons": [vp, vp, vp, C.c_double],
         "ephmc_hamilton": [vp, vp, C.c_double, C.c_double, dp],
         "ephmc_leap_frog": [vp, vp, vp],
+        "ephmc_update_x": [vp, vp, vp, C.c_double, C.c_double],
         "ephmc_chain_init": [vp, dp],
         "ephmc_trajectory": [vp, dp, C.c_double, C.POINTER(Record)],
         "ephmc_last_record": [vp, C.POINTER(Record)],
@@ -202,8 +203,9 @@ class Context:
         return ld.value, sg.value, dm.download()
 
     def force(self, x, Minv, logdet=0.0):
-        dx, dm, dp_ = self.array(self.dim, x), self.array(self.mat, Minv), self.array(self.dim)
-        check(LIB.ephmc_force(self.h, dx.ptr, dm.ptr, logdet, dp_.ptr))
+        dx, dp_ = self.array(self.dim, x), self.array(self.dim)
+        dm = self.array(self.mat, Minv) if Minv is not None else None
+        check(LIB.ephmc_force(self.h, dx.ptr, dm.ptr if dm is not None else None, logdet, dp_.ptr))
         return dp_.download()
 
     def apply_exp_M(self, x_tau, M, revert, stop):
@@ -241,6 +243,11 @@ class Context:
     def leap_frog(self, x, p):
         dx, dp_ = self.array(self.dim, x), self.array(self.dim, p)
         check(LIB.ephmc_leap_frog(self.h, dx.ptr, dp_.ptr))
+        return dx.download(), dp_.download()
+
+    def update_x(self, x, p, h, half):
+        dx, dp_ = self.array(self.dim, x), self.array(self.dim, p)
+        check(LIB.ephmc_update_x(self.h, dx.ptr, dp_.ptr, h, half))
         return dx.download(), dp_.download()
 
     # -- chain ------------------------------------------------------------------------

@@ -34,6 +34,7 @@ struct organic_engine {
 	unsigned l1, l2, ns, nt, flavors, steps;
 	double t[3], kappa[3], mu, omega, traj_length;
 	int precision;
+	organic_flags mode;
 	size_t dim, mat;
 	double *d_x, *d_p, *d_m, *d_v;          /* staging buffers for host-array calls */
 	double *x_host;                         /* buffers bound by organic_plan_create */
@@ -58,12 +59,12 @@ static unsigned env_checkpoint(void)
 	return e ? (unsigned)atoi(e) : 16;
 }
 
+static const organic_flags no_flags;
+
 static void check_flags(const organic_flags *mode, const char *who)
 {
-	if(mode && (mode->schur_form_inv || mode->constant_force || mode->no_fourier_acc || mode->classical_fourier_acc
-			|| mode->single_particle || mode->zero_filling)){
-		fprintf(stderr, "organic: %s: the flags SCHUR_FORM_INV, CONSTANT_FORCE, NO_FOURIER_ACC, "
-			"CLASSICAL_FOURIER_ACC, SINGLE_PARTICLE and ZERO_FILLING are not provided by this build\n", who);
+	if(mode && mode->schur_form_inv){
+		fprintf(stderr, "organic: %s: SCHUR_FORM_INV is not provided by this build\n", who);
 		exit(EXIT_FAILURE);
 	}
 }
@@ -86,13 +87,16 @@ static void lattice_extents(const unsigned *nnt, unsigned ns, unsigned *l1, unsi
 static int same(double a, double b){ return a == b; }
 
 static struct organic_engine *engine_get(unsigned l1, unsigned l2, unsigned nt, const double *t, const double *kappa,
-		double mu, double omega, unsigned flavors, unsigned steps, double traj_length)
+		double mu, double omega, unsigned flavors, unsigned steps, double traj_length, const organic_flags *mode)
 {
+	if(!mode) mode = &no_flags;
+	check_flags(mode, "engine");
 	for(int i = 0; i < MAX_ENGINES; i++){
 		struct organic_engine *e = engines[i];
 		if(e && e->l1 == l1 && e->l2 == l2 && e->nt == nt && same(e->mu, mu) && same(e->omega, omega)
 			&& e->flavors == flavors && e->steps == steps && same(e->traj_length, traj_length)
-			&& !memcmp(e->t, t, sizeof(e->t)) && !memcmp(e->kappa, kappa, sizeof(e->kappa)))
+			&& !memcmp(e->t, t, sizeof(e->t)) && !memcmp(e->kappa, kappa, sizeof(e->kappa))
+			&& !memcmp(&e->mode, mode, sizeof(*mode)))
 			return e;
 	}
 	struct organic_engine *e = calloc(1, sizeof(*e));
@@ -106,12 +110,20 @@ static struct organic_engine *engine_get(unsigned l1, unsigned l2, unsigned nt, 
 	cfg.taylor_order = 4;
 	cfg.precision = env_precision();
 	cfg.checkpoint = env_checkpoint();
+	cfg.zero_filling = mode->zero_filling;
+	cfg.single_particle = mode->single_particle;
+	cfg.constant_force = mode->constant_force;
+	cfg.no_fourier_acc = mode->no_fourier_acc;
+	cfg.classical_fourier_acc = mode->classical_fourier_acc;
+	cfg.cfa_steps = mode->cfa_steps;
+	cfg.cfa_mass = mode->cfa_mass;
 	GPU(ephmc_create(&cfg, &e->ctx));
 	e->l1 = l1; e->l2 = l2; e->nt = nt; e->ns = l1 * l2;
 	memcpy(e->t, t, sizeof(e->t));
 	memcpy(e->kappa, kappa, sizeof(e->kappa));
 	e->mu = mu; e->omega = omega; e->flavors = flavors; e->steps = steps; e->traj_length = traj_length;
 	e->precision = cfg.precision;
+	e->mode = *mode;
 	e->dim = (size_t)nt * 3 * e->ns;
 	e->mat = (size_t)e->ns * e->ns;
 	GPU(ephmc_malloc(e->dim * sizeof(double), (void **)&e->d_x));
@@ -133,12 +145,13 @@ static struct organic_engine *engine_get(unsigned l1, unsigned l2, unsigned nt, 
 	return e;
 }
 
-static struct organic_engine *engine_model(double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nt)
+static struct organic_engine *engine_model(double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nt,
+		const organic_flags *mode)
 {
 	unsigned l1, l2;
 	lattice_extents(nnt, ns, &l1, &l2);
 	const double omega = 1, traj = 0;
-	return engine_get(l1, l2, nt, t, kappa, mu, omega, 1, 1, traj);
+	return engine_get(l1, l2, nt, t, kappa, mu, omega, 1, 1, traj, mode);
 }
 
 static void up(struct organic_engine *e, double *dev, const double *host, size_t n)
@@ -155,8 +168,7 @@ static void down(struct organic_engine *e, double *host, const double *dev, size
 organic_plan organic_plan_create(double *x, double complex *xc, double *t, double *kappa, double mu, double omega,
 		unsigned flavors, unsigned l1, unsigned l2, unsigned nt, unsigned steps, double traj_length, organic_flags *mode)
 {
-	check_flags(mode, "organic_plan_create");
-	struct organic_engine *e = engine_get(l1, l2, nt, t, kappa, mu, omega, flavors, steps, traj_length);
+	struct organic_engine *e = engine_get(l1, l2, nt, t, kappa, mu, omega, flavors, steps, traj_length, mode);
 	e->x_host = x;
 	e->xc_host = xc;
 	e->chain_synced = 0;
@@ -204,7 +216,7 @@ void apply_exp_mu_K_to_M(double *x, double *M, double *w, double *t, double *kap
 		unsigned nn, int revert, int stop)
 {
 	(void)w; (void)nn;
-	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, 1);
+	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, 1, NULL);
 	up(e, e->d_x, x, 3 * (size_t)ns);
 	up(e, e->d_m, M, e->mat);
 	GPU(ephmc_apply_exp_M(e->ctx, e->d_x, e->d_m, revert, stop));
@@ -216,12 +228,10 @@ void construct_M_hat(double *x, double *M_hat, double *w, double *t, double *kap
 		unsigned nn, unsigned nt, organic_flags *mode)
 {
 	(void)w; (void)nn;
-	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, nt);
+	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, nt, mode);
 	up(e, e->d_x, x, e->dim);
 	GPU(ephmc_construct_M_hat(e->ctx, e->d_x, e->d_m));
 	down(e, M_hat, e->d_m, e->mat);
-	if(mode && (mode->zero_filling || mode->single_particle))
-		for(unsigned i = 0; i < ns; i++) M_hat[(size_t)i * (ns + 1)] -= 1;
 }
 
 /* organic_dense_matrix.c:547-607: log|det| and the inverse in place; p receives the (identity)
@@ -229,13 +239,14 @@ void construct_M_hat(double *x, double *M_hat, double *w, double *t, double *kap
 double logDetM_and_inv_M_hat(double *M_hat, double *w, int *p, unsigned ns, unsigned flavors, organic_flags *mode)
 {
 	(void)w;
-	if(mode && mode->single_particle) return log(trace(M_hat, ns));
-	if(mode && mode->schur_form_inv) check_flags(mode, "logDetM_and_inv_M_hat");
+	check_flags(mode, "logDetM_and_inv_M_hat");
+	if(mode && mode->single_particle) return log(trace(M_hat, ns));   /* M_hat unchanged */
 	struct organic_engine *e = NULL;
-	for(int i = 0; i < MAX_ENGINES && !e; i++) if(engines[i] && engines[i]->ns == ns) e = engines[i];
+	for(int i = 0; i < MAX_ENGINES && !e; i++)
+		if(engines[i] && engines[i]->ns == ns && !engines[i]->mode.single_particle) e = engines[i];
 	if(!e){
 		const double z[3] = {0, 0, 0};
-		e = engine_get(ns, 1, 1, z, z, 0, 1, flavors, 1, 0);
+		e = engine_get(ns, 1, 1, z, z, 0, 1, flavors, 1, 0, NULL);
 	}
 	up(e, e->d_m, M_hat, e->mat);
 	double logdet;
@@ -252,8 +263,7 @@ void cc_correlator(double *x, double *M_hat, double *R_t0, double *R_0t, double 
 		unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, double *greens, double logDetM, organic_flags *mode)
 {
 	(void)R_t0; (void)R_0t; (void)w; (void)nn;
-	check_flags(mode, "cc_correlator");
-	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, nt);
+	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, nt, mode);
 	up(e, e->d_x, x, e->dim);
 	up(e, e->d_m, M_hat, e->mat);
 	GPU(ephmc_cc_correlator(e->ctx, e->d_x, e->d_m, logDetM, greens));
@@ -284,10 +294,9 @@ void force(double *x, double *p_dot, double *M_hat, double *QM1P, double *w, int
 		double omega, unsigned flavors, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, double logDetM, organic_flags *mode)
 {
 	(void)QM1P; (void)w; (void)perm; (void)nn;
-	check_flags(mode, "force");
 	unsigned l1, l2;
 	lattice_extents(nnt, ns, &l1, &l2);
-	struct organic_engine *e = engine_get(l1, l2, nt, t, kappa, mu, omega, flavors, 1, 0);
+	struct organic_engine *e = engine_get(l1, l2, nt, t, kappa, mu, omega, flavors, 1, 0, mode);
 	up(e, e->d_x, x, e->dim);
 	up(e, e->d_m, M_hat, e->mat);
 	GPU(ephmc_force(e->ctx, e->d_x, e->d_m, logDetM, e->d_p));
@@ -305,7 +314,7 @@ double hamilton(double *x, double *p, double logDetM, double energyP, double ome
 			&& engines[i]->flavors == flavors) e = engines[i];
 	if(!e){
 		const double z[3] = {0, 0, 0};
-		e = engine_get(ns, 1, nt, z, z, 0, omega, flavors, 1, 0);
+		e = engine_get(ns, 1, nt, z, z, 0, omega, flavors, 1, 0, NULL);
 	}
 	up(e, e->d_x, x, e->dim);
 	double H;
@@ -331,7 +340,6 @@ void sample_fourier_momenta(double *p, double complex *pc, double omega, unsigne
 		const organic_plan *fft, organic_flags *mode)
 {
 	(void)pc; (void)omega; (void)ns; (void)nn; (void)nt;
-	check_flags(mode, "sample_fourier_momenta");
 	struct organic_engine *e = fft[0];
 	random_vector(p, (unsigned)e->dim);
 	up(e, e->d_v, p, e->dim);
@@ -351,13 +359,16 @@ double energy_fourier_momenta(double complex *pc, double omega, unsigned ns, uns
 	return en;
 }
 
-/* organic_hmc.c:56-65 */
+/* organic_hmc.c:56-65: p itself without Fourier acceleration, else the plan-bound momenta */
 double energy_momenta(double *p, double complex *pc, double omega, unsigned ns, unsigned nn, unsigned nt,
 		const organic_plan *fft, organic_flags *mode)
 {
-	(void)p;
-	check_flags(mode, "energy_momenta");
-	return energy_fourier_momenta(pc, omega, ns, nn, nt, fft);
+	if(!(mode && mode->no_fourier_acc)) return energy_fourier_momenta(pc, omega, ns, nn, nt, fft);
+	struct organic_engine *e = fft[0];
+	up(e, e->d_p, p, e->dim);
+	double en;
+	GPU(ephmc_energy_momenta(e->ctx, e->d_p, &en));
+	return en;
 }
 
 /* organic_bosons.c:126-150 on the plan-bound field and momenta */
@@ -374,13 +385,18 @@ void evolve_bosons(double complex *xc, double omega, unsigned ns, unsigned nn, u
 	e->chain_synced = 0;
 }
 
-/* organic_hmc.c:136-165 (Fourier acceleration) */
+/* organic_hmc.c:136-165 on x and p (p_dot is the engine's own scratch) */
 void update_x(double *x, double *p, double *p_dot, double complex *xc, double omega, unsigned ns, unsigned nn, unsigned nt,
 		const organic_plan *fft, double h, double half, organic_flags *mode)
 {
-	(void)x; (void)p; (void)p_dot;
-	check_flags(mode, "update_x");
-	evolve_bosons(xc, omega, ns, nn, nt, fft, fabs(half) * h);
+	(void)p_dot; (void)xc; (void)omega; (void)ns; (void)nn; (void)nt; (void)mode;
+	struct organic_engine *e = fft[0];
+	up(e, e->d_x, x, e->dim);
+	up(e, e->d_p, p, e->dim);
+	GPU(ephmc_update_x(e->ctx, e->d_x, e->d_p, h, half));
+	down(e, x, e->d_x, e->dim);
+	down(e, p, e->d_p, e->dim);
+	e->chain_synced = 0;
 }
 
 /* organic_hmc.c:167-190: the whole leapfrog runs on the GPU; x and p are updated on return */
@@ -389,10 +405,9 @@ void leap_frog(double *x, double *p, double *p_dot, double complex *xc, double *
 		unsigned nt, unsigned steps, double traj_length, const organic_plan *fft, organic_flags *mode)
 {
 	(void)p_dot; (void)xc; (void)M_hat; (void)QM1P; (void)w; (void)perm; (void)nnt; (void)nn; (void)fft;
-	check_flags(mode, "leap_frog");
 	unsigned l1, l2;
 	lattice_extents(nnt, ns, &l1, &l2);
-	struct organic_engine *e = engine_get(l1, l2, nt, t, kappa, mu, omega, flavors, steps, traj_length);
+	struct organic_engine *e = engine_get(l1, l2, nt, t, kappa, mu, omega, flavors, steps, traj_length, mode);
 	up(e, e->d_x, x, e->dim);
 	up(e, e->d_p, p, e->dim);
 	GPU(ephmc_leap_frog(e->ctx, e->d_x, e->d_p));
@@ -416,7 +431,8 @@ static void measure(struct organic_engine *e, double *x, double *M_hat, double e
 		results[2] = energy;
 		results[3] = -logDetM;
 		results[4] = higher_Matsubaras_norm(x, nnt, ns, 6, nt);
-		results[5] = 1 - trace(M_hat, ns) / ns;
+		if(!e->mode.zero_filling && !e->mode.single_particle) results[5] = 1 - trace(M_hat, ns) / ns;
+		else results[5] = trace(M_hat, ns) / ns;     /* M_hat = B..B here */
 		results[6] = acc;
 		results[7] = boltzmann;
 		average_links(x, e->kappa, ns, 3, nt, results + 8);
@@ -434,8 +450,7 @@ short trajectory(double *x, double complex *xc, int *perm, double *t, double *ka
 		double *old, FILE *res_out, FILE *corr_out, organic_flags *mode)
 {
 	(void)xc; (void)perm; (void)t; (void)kappa; (void)mu; (void)omega; (void)flavors; (void)nn; (void)steps;
-	(void)traj_length;
-	check_flags(mode, "trajectory");
+	(void)traj_length; (void)mode;
 	struct organic_engine *e = fft[0];
 	const size_t dim = e->dim;
 	double *p = x + dim, *M_hat = x + 4 * dim;
@@ -443,8 +458,9 @@ short trajectory(double *x, double complex *xc, int *perm, double *t, double *ka
 		GPU(ephmc_chain_set_x(e->ctx, x));
 		e->chain_synced = 1;
 	}
+	/* ZERO_FILLING draws a new free-boson field and no Metropolis uniform (organic_hmc.c:246-254) */
 	random_vector(p, (unsigned)dim);
-	const double u = genrand64_real2();
+	const double u = e->mode.zero_filling ? 0.0 : genrand64_real2();
 	ephmc_traj_record rec;
 	GPU(ephmc_trajectory(e->ctx, p, u, &rec));
 	GPU(ephmc_chain_get_x(e->ctx, x));
@@ -465,7 +481,6 @@ unsigned run_hmc(double *greens, unsigned l1, unsigned l2, unsigned nt, double *
 		unsigned flavors, double *contr_mat, unsigned steps, double traj_length, unsigned therm, unsigned meas,
 		unsigned meas_freq, const char *res_name, const char *corr_name, organic_flags *mode)
 {
-	check_flags(mode, "run_hmc");
 	unsigned ns, nn;
 	unsigned *nnt = construct_triangular(l1, l2, &ns, &nn);
 	const unsigned greens_dim = nt * 9;
@@ -479,14 +494,15 @@ unsigned run_hmc(double *greens, unsigned l1, unsigned l2, unsigned nt, double *
 	if(!organic_seed(&seed)) seed = (unsigned long)time(NULL);
 	init_genrand64(seed);
 
-	struct organic_engine *e = engine_get(l1, l2, nt, t, kappa, mu, omega, flavors, steps, traj_length);
+	struct organic_engine *e = engine_get(l1, l2, nt, t, kappa, mu, omega, flavors, steps, traj_length, mode);
+	const int zf = e->mode.zero_filling;
 	random_vector(normals, (unsigned)dim);             /* sample_free_bosons' draw */
 	GPU(ephmc_chain_init(e->ctx, normals));
 	double *dx, *dp, *dm;
 	GPU(ephmc_chain_buffers(e->ctx, &dx, &dp, &dm));
 	for(unsigned i = 0; i < therm + meas; i++){
 		random_vector(normals, (unsigned)dim);
-		const double u = genrand64_real2();
+		const double u = zf ? 0.0 : genrand64_real2();
 		const int measured = i >= therm && (i - therm) % meas_freq == 0;
 		if(!measured){
 			/* asynchronous: the next trajectory's normals are drawn while this one runs */
@@ -558,7 +574,7 @@ unsigned run_trans_loc(double *greens, unsigned l1, unsigned l2, unsigned nt, do
 	for(unsigned k = 0; k < nn2; k++) width += 8 * fabs(kappa[k]) / nt;
 
 	const double cm_default[6] = {0, 0, 0, 0, 0, 0};
-	struct organic_engine *e = engine_get(l1, l2, 1, t, kappa, mu, 1, 1, 1, 0);
+	struct organic_engine *e = engine_get(l1, l2, 1, t, kappa, mu, 1, 1, 1, 0, NULL);
 	/* samples are sent in batches; each batch's normals are drawn in sample order */
 	const unsigned batch = 64;
 	double *normals = malloc((size_t)batch * dim * sizeof(double));

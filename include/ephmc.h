@@ -92,13 +92,15 @@ int ephmc_construct_M_hat(ephmc_ctx *ctx, const double *x_dev, double *M_hat_dev
 
 /* ---- (c) log-det, inverse, force ------------------------------------------------------ */
 /* log|det M_hat| and M_hat <- M_hat^{-1} in place (FP64, partial pivoting); replaces
- * logDetM_and_inv_M_hat (organic_dense_matrix.c:547-607).  logdet_host / sign_host may be
- * NULL; when given, the call waits for the result. */
+ * logDetM_and_inv_M_hat (organic_dense_matrix.c:547-607).  SINGLE_PARTICLE: log tr M_hat,
+ * M_hat unchanged.  logdet_host / sign_host may be NULL; when given, the call waits. */
 int ephmc_logdet_inv(ephmc_ctx *ctx, double *M_hat_dev, double *logdet_host, int *sign_host);
 /* p_dot = fermion force for field x given M_inv = (1 + B..B)^{-1}; replaces force
  * (organic_hmc.c:86-134) with construct_all_QM1P + refine_QM1P
  * (organic_dense_matrix.c:311-442) computed as an adjoint sweep.  Uses the context's
- * precision; logdet is used by SINGLE_PARTICLE only. */
+ * precision and flags: CONSTANT_FORCE puts the slice-0 force (construct_QM1P at time 0,
+ * :267-309) on every slice; SINGLE_PARTICLE ignores M_inv (may be NULL) and scales by
+ * exp(-logdet); NO_FOURIER_ACC adds the boson force (organic_bosons.c:179-194). */
 int ephmc_force(ephmc_ctx *ctx, const double *x_dev, const double *M_inv_dev, double logdet, double *p_dot_dev);
 
 /* ---- (d) Fourier-accelerated bosons ---------------------------------------------------- */
@@ -114,6 +116,10 @@ int ephmc_energy_momenta(ephmc_ctx *ctx, const double *p_dev, double *energy_hos
 /* exact harmonic flow of every Matsubara mode for time h; replaces evolve_bosons
  * (organic_bosons.c:126-150) */
 int ephmc_evolve_bosons(ephmc_ctx *ctx, double *x_dev, double *p_dev, double h);
+/* drift of the position update with the context's momentum scheme: Fourier-accelerated flow
+ * for time |half| h, x += |half| h p (NO_FOURIER_ACC) or the CLASSICAL_FOURIER_ACC sub-steps
+ * with the boson force; replaces update_x (organic_hmc.c:136-165) */
+int ephmc_update_x(ephmc_ctx *ctx, double *x_dev, double *p_dev, double h, double half);
 /* H = -flavors logdet + energyP + S_B(x); replaces hamilton (organic_hmc.c:67-84).  Waits. */
 int ephmc_hamilton(ephmc_ctx *ctx, const double *x_dev, double logdet, double energyP, double *H_host);
 /* position-Verlet leapfrog of Nmd steps on x, p (the context's precision); replaces
