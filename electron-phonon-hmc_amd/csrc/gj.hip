@@ -19,6 +19,7 @@
 //            Gauss-Jordan with row exchanges and the columns un-permuted at the end); without
 //            the flag it returns at once.  No host round trip either way.
 #include "internal.h"
+#include <cstdlib>
 
 namespace ephmc {
 
@@ -165,6 +166,137 @@ __global__ __launch_bounds__(256) void gj_update_kernel(const double *__restrict
 	}
 }
 
+// Fused block step: one launch per 32-column block K, grid over 64 x 64 output tiles.
+// Every workgroup inverts D = A[K,K] in LDS (the same arithmetic as gj_panel_np_kernel), forms
+// the transform rows E[i,:] of its 64 rows (D^{-1} rows inside K, -A[i,K] D^{-1} outside), and
+// writes its tile of A' = [i not in K] A + E A[K,:] (columns inside K: E itself).  Workgroups
+// of the first column block check the LU multipliers of their rows below K; workgroup (0,0)
+// records the pivots and the multipliers inside D.
+constexpr int GJ_RB = 64, GJ_CB = 64;
+
+__global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__ A, double *__restrict__ An,
+		unsigned n, unsigned k0, unsigned nbe, double *__restrict__ diag, int *__restrict__ piv, double *__restrict__ scal)
+{
+	constexpr int NB = GJ_NB;
+	__shared__ double dinv[NB][NB + 1];
+	__shared__ double lm[NB][NB + 1];
+	__shared__ double rowb[2][NB], colb[2][NB];
+	__shared__ double Es[GJ_RB][NB + 1];
+	__shared__ double Ys[NB][GJ_CB + 1];
+	const int tid = threadIdx.x;
+	const unsigned r0 = blockIdx.x * GJ_RB, c0 = blockIdx.y * GJ_CB;
+	const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
+	// ---- D^{-1}: thread owns (r, 4q..4q+3) of the 32 x 32 block
+	{
+		const int r = tid >> 3, cq = (tid & 7) * 4;
+		double v[4];
+#pragma unroll
+		for(int q = 0; q < 4; q++){
+			const int cc = cq + q;
+			v[q] = (r < (int)nbe && cc < (int)nbe) ? A[(size_t)(k0 + cc) * n + k0 + r] : (r == cc ? 1.0 : 0.0);
+			lm[r][cc] = 0;
+		}
+		bool bad = false;
+		for(int g = 0; g < (int)nbe; g++){
+			const int b = g & 1;
+			if(r == g){
+#pragma unroll
+				for(int q = 0; q < 4; q++) rowb[b][cq + q] = v[q];
+			}
+			if(g >= cq && g < cq + 4){
+#pragma unroll
+				for(int q = 0; q < 4; q++) if(cq + q == g) colb[b][r] = v[q];
+			}
+			__syncthreads();
+			const double d = rowb[b][g];
+			const double inv = 1.0 / d;
+			if(r == g){
+#pragma unroll
+				for(int q = 0; q < 4; q++) v[q] = (cq + q == g) ? inv : v[q] * inv;
+			}else{
+				const double fi = colb[b][r] * inv;
+#pragma unroll
+				for(int q = 0; q < 4; q++) v[q] = (cq + q == g) ? -fi : v[q] - fi * rowb[b][cq + q];
+				if(r > g && g >= cq && g < cq + 4){
+					lm[r][g] = fi;
+					bad |= !(fabs(fi) <= 1.0);
+				}
+			}
+			if(tid == 0 && lead){ diag[k0 + g] = d; piv[k0 + g] = (int)(k0 + g); }
+		}
+#pragma unroll
+		for(int q = 0; q < 4; q++) dinv[r][cq + q] = v[q];
+		if(bad && lead) scal[SCAL_GJ_FLAG] = 1;
+	}
+	// ---- A[K, column block] (old values; the output goes to the other buffer)
+	for(int e = tid; e < NB * GJ_CB; e += 256){
+		const int m = e % NB, cc = e / NB;
+		const unsigned col = c0 + cc;
+		Ys[m][cc] = (m < (int)nbe && col < n) ? A[(size_t)col * n + k0 + m] : 0.0;
+	}
+	__syncthreads();
+	// ---- transform rows of this row block: thread (row tid/4, 8 columns)
+	{
+		const int rr = tid >> 2, j0 = (tid & 3) * 8;
+		const unsigned i = r0 + rr;
+		double e8[8];
+		if(i < n && i >= k0 && i < k0 + nbe){
+#pragma unroll
+			for(int j = 0; j < 8; j++) e8[j] = dinv[i - k0][j0 + j];
+		}else{
+#pragma unroll
+			for(int j = 0; j < 8; j++) e8[j] = 0;
+			if(i < n)
+				for(int m = 0; m < (int)nbe; m++){
+					const double a = A[(size_t)(k0 + m) * n + i];
+#pragma unroll
+					for(int j = 0; j < 8; j++) e8[j] += a * dinv[m][j0 + j];
+				}
+#pragma unroll
+			for(int j = 0; j < 8; j++) e8[j] = -e8[j];
+		}
+#pragma unroll
+		for(int j = 0; j < 8; j++) Es[rr][j0 + j] = e8[j];
+	}
+	__syncthreads();
+	// LU multipliers of the rows below K: l = -E L (first column block only)
+	if(blockIdx.y == 0 && tid < GJ_RB){
+		const unsigned i = r0 + tid;
+		if(i < n && i >= k0 + nbe){
+			bool bad = false;
+			for(int j = 0; j < (int)nbe; j++){
+				double y = Es[tid][j];
+				for(int m = j + 1; m < (int)nbe; m++) y += Es[tid][m] * lm[m][j];
+				bad |= !(fabs(y) <= 1.0);
+			}
+			if(bad) scal[SCAL_GJ_FLAG] = 1;
+		}
+	}
+	// ---- tile update: thread owns row tid % 64 and columns (tid / 64) + 4 j, j < 16
+	const int rr = tid & 63, cbase = tid >> 6;
+	const unsigned i = r0 + rr;
+	if(i >= n) return;
+	const bool rowK = i >= k0 && i < k0 + nbe;
+	double er[NB];
+#pragma unroll
+	for(int m = 0; m < NB; m++) er[m] = Es[rr][m];
+#pragma unroll 4
+	for(int j = 0; j < GJ_CB / 4; j++){
+		const int cc = cbase + 4 * j;
+		const unsigned col = c0 + cc;
+		if(col >= n) continue;
+		double out;
+		if(col >= k0 && col < k0 + nbe) out = Es[rr][col - k0];
+		else{
+			double acc = rowK ? 0.0 : A[(size_t)col * n + i];
+#pragma unroll
+			for(int m = 0; m < NB; m++) acc += er[m] * Ys[m][cc];
+			out = acc;
+		}
+		An[(size_t)col * n + i] = out;
+	}
+}
+
 // log|det| and sign from the pivots of the fast path (no row exchanges).
 __global__ __launch_bounds__(1024) void gj_logdet_kernel(const double *__restrict__ diag, unsigned n, double *__restrict__ scal)
 {
@@ -284,13 +416,21 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M)
 	double *buf[2] = {c->d_gj[0], c->d_gj[1]};
 	int which = 0;
 	const dim3 ugrid((n + 31) / 32, (n + 31) / 32);
+	const bool fused = !(getenv("EPHMC_GJ_SPLIT") && atoi(getenv("EPHMC_GJ_SPLIT")) != 0);
+	const dim3 sgrid((n + GJ_RB - 1) / GJ_RB, (n + GJ_CB - 1) / GJ_CB);
 	for(unsigned k0 = 0; k0 < n; k0 += GJ_NB){
 		const unsigned nbe = std::min<unsigned>(GJ_NB, n - k0);
 		double *next = buf[which];
-		gj_panel_np_kernel<GJ_NB><<<(n + 255) / 256, 256, 0, c->stream>>>(cur, next, n, k0, nbe, c->d_diag, c->d_piv, c->d_scal);
-		EPHMC_LAUNCHED();
-		gj_update_kernel<GJ_NB><<<ugrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe);
-		EPHMC_LAUNCHED();
+		if(fused){
+			gj_step_kernel<<<sgrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe, c->d_diag, c->d_piv, c->d_scal);
+			EPHMC_LAUNCHED();
+		}else{
+			gj_panel_np_kernel<GJ_NB><<<(n + 255) / 256, 256, 0, c->stream>>>(cur, next, n, k0, nbe, c->d_diag, c->d_piv,
+				c->d_scal);
+			EPHMC_LAUNCHED();
+			gj_update_kernel<GJ_NB><<<ugrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe);
+			EPHMC_LAUNCHED();
+		}
 		cur = next;
 		which ^= 1;
 	}

@@ -30,6 +30,24 @@ def test_M_hat(l1, l2, nt):
     assert rel(Mg, Mo) < 1e-13
 
 
+@pytest.mark.parametrize("n", [37, 100, 256, 1024])
+def test_gj_fused_step_matches_split(monkeypatch, n):
+    """One fused launch per block step (diagonal inversion, transform rows and tile update) gives
+    the same bits as the separate panel and update kernels (EPHMC_GJ_SPLIT=1)."""
+    say("gj fused vs split", n)
+    om = OracleModel(n, 1, 2)
+    ctx = gpu_context(om)
+    rng = np.random.default_rng(n)
+    M = (np.eye(n) * 3 + rng.standard_normal((n, n)) * 0.3 / np.sqrt(n)).ravel()
+    monkeypatch.setenv("EPHMC_GJ_SPLIT", "1")
+    ld1, s1, I1 = ctx.logdet_inv(M)
+    monkeypatch.setenv("EPHMC_GJ_SPLIT", "0")
+    ld2, s2, I2 = ctx.logdet_inv(M)
+    assert (ld1, s1) == (ld2, s2)
+    np.testing.assert_array_equal(I1, I2)
+    assert ctx.gj_fallbacks() == 0
+
+
 @pytest.mark.parametrize("l1,l2,nt", [(2, 3, 4), (6, 6, 8), (16, 16, 8), (32, 32, 4)])
 def test_strip_chain_bit_identical(monkeypatch, l1, l2, nt):
     """The strip form of the slice stencil sums in the same order as the site-per-thread form:
