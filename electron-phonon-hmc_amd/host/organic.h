@@ -62,6 +62,7 @@ void fprint_corr(FILE *out, double *corr, double *contr_mat, unsigned nn, unsign
 void fprint_results(FILE *out, double *res, unsigned n, unsigned r);
 void fwrite_sigma(const char *name, double *greens, double width, unsigned n);
 double *fread_greens(const char *in_name, double *h, unsigned *length);
+double *fscan_array(const char *in_name, unsigned *length, double *beta, unsigned *nt);
 double average(double *x, unsigned n);
 double average_sq(double *x, unsigned n);
 void average_links(double *x, double *scale, unsigned ns, unsigned nn, unsigned nt, double *res);

@@ -88,6 +88,28 @@ void fwrite_sigma(const char *name, double *greens, double width, unsigned n)
 	fclose(out);
 }
 
+/* whitespace-separated file: beta, nt, then values until the end (organic_aux.c:121-141) */
+double *fscan_array(const char *in_name, unsigned *length, double *beta, unsigned *nt)
+{
+	FILE *in = fopen(in_name, "r");
+	if(!in) return NULL;
+	if(beta && fscanf(in, "%lg ", beta) != 1) *beta = 0;
+	if(nt && fscanf(in, "%u ", nt) != 1) *nt = 0;
+	unsigned cap = 1024, k = 0;
+	double *x = malloc(cap * sizeof(double));
+	while(x && fscanf(in, "%lg ", x + k) > 0){
+		if(++k == cap){
+			cap *= 2;
+			double *y = realloc(x, cap * sizeof(double));
+			if(!y){ free(x); x = NULL; }
+			x = y;
+		}
+	}
+	*length = k;
+	fclose(in);
+	return x;
+}
+
 double *fread_greens(const char *in_name, double *h, unsigned *length)
 {
 	FILE *in = fopen(in_name, "r");
