@@ -41,6 +41,8 @@ typedef struct organic_engine *organic_plan;
 void set_flags(FILE *in, organic_flags *mode);
 int organic_seed(unsigned long *seed);          /* 1 and the seed when a SEED token was read */
 void organic_set_seed(unsigned long seed);
+const char *organic_state_file(void);           /* restart file of a STATE token, or NULL */
+void organic_set_state_file(const char *path);
 
 /* ---- organic_lattice.c */
 unsigned *construct_triangular(unsigned l1, unsigned l2, unsigned *ns, unsigned *nn);
@@ -54,6 +56,8 @@ void init_genrand64(unsigned long seed);
 unsigned long genrand64_int64(void);
 double genrand64_real2(void);
 void random_vector(double *x, unsigned N);
+void organic_rng_save(unsigned long long *words, int *pos);     /* 312 words */
+void organic_rng_load(const unsigned long long *words, int pos);
 
 /* ---- organic_aux.c */
 unsigned ceil_half(unsigned n);
@@ -62,6 +66,9 @@ void fprint_corr(FILE *out, double *corr, double *contr_mat, unsigned nn, unsign
 void fprint_results(FILE *out, double *res, unsigned n, unsigned r);
 void fwrite_sigma(const char *name, double *greens, double width, unsigned n);
 double *fread_greens(const char *in_name, double *h, unsigned *length);
+void print_config(double *x, unsigned ns, unsigned nn, unsigned nt);
+void fprint_config(const char *name, double *x, unsigned ns, unsigned nn, unsigned nt, const char *mode);
+void fwrite_config(const char *name, double *x, unsigned n, const char *mode);
 double *fscan_array(const char *in_name, unsigned *length, double *beta, unsigned *nt);
 double average(double *x, unsigned n);
 double average_sq(double *x, unsigned n);

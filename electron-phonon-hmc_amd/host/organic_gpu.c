@@ -475,8 +475,46 @@ short trajectory(double *x, double complex *xc, int *perm, double *t, double *ka
 	return (short)rec.accepted;
 }
 
+/* Restart file of a STATE token: "EPHMCST1", dim (u64), the field x, the MT19937-64 state. */
+static int state_load(const char *path, double *x, size_t dim)
+{
+	FILE *f = fopen(path, "rb");
+	if(!f) return 0;
+	char magic[8];
+	unsigned long long d = 0, words[312];
+	int pos = 0;
+	const int ok = fread(magic, 1, 8, f) == 8 && !memcmp(magic, "EPHMCST1", 8) && fread(&d, sizeof(d), 1, f) == 1 && d == dim
+		&& fread(x, sizeof(double), dim, f) == dim && fread(words, sizeof(words), 1, f) == 1 && fread(&pos, sizeof(pos), 1, f) == 1;
+	fclose(f);
+	if(!ok){
+		fprintf(stderr, "organic: restart file %s does not belong to this lattice\n", path);
+		exit(EXIT_FAILURE);
+	}
+	organic_rng_load(words, pos);
+	return 1;
+}
+
+static void state_save(const char *path, const double *x, size_t dim)
+{
+	FILE *f = fopen(path, "wb");
+	if(!f){
+		fprintf(stderr, "organic: cannot write restart file %s\n", path);
+		return;
+	}
+	unsigned long long d = dim, words[312];
+	int pos;
+	organic_rng_save(words, &pos);
+	fwrite("EPHMCST1", 1, 8, f);
+	fwrite(&d, sizeof(d), 1, f);
+	fwrite(x, sizeof(double), dim, f);
+	fwrite(words, sizeof(words), 1, f);
+	fwrite(&pos, sizeof(pos), 1, f);
+	fclose(f);
+}
+
 /* organic_hmc.c:296-353 with the chain resident on the GPU.  The seed is time(NULL) as in the
- * reference unless the input named one (SEED n). */
+ * reference unless the input named one (SEED n); with STATE <file> the run resumes from the
+ * file when it exists (field and random stream) and writes it at the end. */
 unsigned run_hmc(double *greens, unsigned l1, unsigned l2, unsigned nt, double *t, double *kappa, double mu, double omega,
 		unsigned flavors, double *contr_mat, unsigned steps, double traj_length, unsigned therm, unsigned meas,
 		unsigned meas_freq, const char *res_name, const char *corr_name, organic_flags *mode)
@@ -496,8 +534,13 @@ unsigned run_hmc(double *greens, unsigned l1, unsigned l2, unsigned nt, double *
 
 	struct organic_engine *e = engine_get(l1, l2, nt, t, kappa, mu, omega, flavors, steps, traj_length, mode);
 	const int zf = e->mode.zero_filling;
-	random_vector(normals, (unsigned)dim);             /* sample_free_bosons' draw */
-	GPU(ephmc_chain_init(e->ctx, normals));
+	const char *restart = organic_state_file();
+	if(restart && state_load(restart, x, dim)){
+		GPU(ephmc_chain_set_x(e->ctx, x));             /* resumes field and RNG stream */
+	}else{
+		random_vector(normals, (unsigned)dim);         /* sample_free_bosons' draw */
+		GPU(ephmc_chain_init(e->ctx, normals));
+	}
 	double *dx, *dp, *dm;
 	GPU(ephmc_chain_buffers(e->ctx, &dx, &dp, &dm));
 	for(unsigned i = 0; i < therm + meas; i++){
@@ -518,6 +561,10 @@ unsigned run_hmc(double *greens, unsigned l1, unsigned l2, unsigned nt, double *
 			results + (size_t)m * NUM_RES, greens + (size_t)m * greens_dim, res_out, corr_out, dx, dm);
 	}
 	GPU(ephmc_synchronize(e->ctx));
+	if(restart){
+		GPU(ephmc_chain_get_x(e->ctx, x));
+		state_save(restart, x, dim);
+	}
 	free(nnt); free(x); free(normals); free(minv);
 	if(res_out) fclose(res_out);
 	if(corr_out) fclose(corr_out);

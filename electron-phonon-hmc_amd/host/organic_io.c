@@ -8,6 +8,14 @@
 
 static int seed_given = 0;
 static unsigned long seed_value = 0;
+static char state_path[1024];
+
+const char *organic_state_file(void){ return state_path[0] ? state_path : NULL; }
+
+void organic_set_state_file(const char *path)
+{
+	snprintf(state_path, sizeof(state_path), "%s", path ? path : "");
+}
 
 int organic_seed(unsigned long *seed)
 {
@@ -22,8 +30,9 @@ void organic_set_seed(unsigned long seed)
 }
 
 /* Flag tokens are read until EOF; unknown tokens are ignored (organic_flags.c:18-39).  In
- * addition "SEED <n>" fixes the seed (the reference seeds with time(NULL)); the reference
- * parser skips it like any unknown token, so such input files stay valid for it. */
+ * addition "SEED <n>" fixes the seed (the reference seeds with time(NULL)) and "STATE <file>"
+ * names a restart file (run_hmc resumes from it when it exists and writes it at the end); the
+ * reference parser skips both like any unknown token, so such input files stay valid for it. */
 void set_flags(FILE *in, organic_flags *mode)
 {
 	memset(mode, 0, sizeof(*mode));
@@ -44,6 +53,9 @@ void set_flags(FILE *in, organic_flags *mode)
 		}else if(!strcmp(tok, "SEED")){
 			unsigned long s;
 			if(fscanf(in, "%lu\n", &s) == 1) organic_set_seed(s);
+		}else if(!strcmp(tok, "STATE")){
+			char path[1024];
+			if(fscanf(in, "%1023s\n", path) == 1) organic_set_state_file(path);
 		}
 	}
 }
@@ -85,6 +97,39 @@ void fwrite_sigma(const char *name, double *greens, double width, unsigned n)
 	fwrite(&n, sizeof(unsigned), 1, out);
 	fwrite(&width, sizeof(double), 1, out);
 	fwrite(greens, sizeof(double), n, out);
+	fclose(out);
+}
+
+/* configuration dumps (organic_aux.c:92-119) */
+void print_config(double *x, unsigned ns, unsigned nn, unsigned nt)
+{
+	for(unsigned tau = 0; tau < nt; tau++){
+		for(unsigned i = 0; i < ns; i++){
+			for(unsigned k = 0; k < nn / 2; k++, x++) printf("%g, ", *x);
+			printf("\t");
+		}
+		printf("\n");
+	}
+}
+
+void fprint_config(const char *name, double *x, unsigned ns, unsigned nn, unsigned nt, const char *mode)
+{
+	FILE *out = fopen(name, mode);
+	if(!out) return;
+	for(unsigned tau = 0; tau < nt; tau++)
+		for(unsigned i = 0; i < ns; i++){
+			for(unsigned k = 0; k < nn / 2; k++, x++) fprintf(out, "%.16g\t", *x);
+			fprintf(out, "\n");
+		}
+	fprintf(out, "\n");
+	fclose(out);
+}
+
+void fwrite_config(const char *name, double *x, unsigned n, const char *mode)
+{
+	FILE *out = fopen(name, mode);
+	if(!out) return;
+	fwrite(x, sizeof(double), n, out);
 	fclose(out);
 }
 

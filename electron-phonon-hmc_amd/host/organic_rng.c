@@ -37,6 +37,19 @@ unsigned long genrand64_int64(void)
 	return (unsigned long)y;
 }
 
+/* generator state for restart files: 312 words and the position */
+void organic_rng_save(unsigned long long *words, int *pos)
+{
+	for(int k = 0; k < MT_N; k++) words[k] = mt_state[k];
+	*pos = mt_pos;
+}
+
+void organic_rng_load(const unsigned long long *words, int pos)
+{
+	for(int k = 0; k < MT_N; k++) mt_state[k] = words[k];
+	mt_pos = pos;
+}
+
 /* [0, 1) with 53-bit resolution */
 double genrand64_real2(void)
 {

@@ -218,3 +218,22 @@ def test_organic_main_transient_localisation():
     go /= meas
     np.testing.assert_allclose(z, zo, rtol=1e-12)
     assert rel(g, go) < 1e-9
+
+
+def test_organic_main_restart():
+    """STATE <file>: 3 + 3 trajectories with a restart in between give the same results.csv lines
+    as 6 trajectories in one run (field and random stream are resumed exactly)."""
+    base = INPUT.format(l1=4, l2=4, nt=16, beta=20.0, steps=6, therm=0, meas="{meas}", freq=1, seed=77)
+    with tempfile.TemporaryDirectory() as d:
+        state = os.path.join(d, "chain.state")
+        runs = []
+        for tag, meas, extra in (("full", 6, ""), ("a", 3, "STATE %s\n" % state), ("b", 3, "STATE %s\n" % state)):
+            inp, res = os.path.join(d, tag + ".txt"), os.path.join(d, tag + ".csv")
+            open(inp, "w").write(base.format(meas=meas) + extra)
+            say("organic_main restart run", tag)
+            out = subprocess.run([MAIN, inp, res], capture_output=True, text=True, timeout=300)
+            assert out.returncode == 0, out.stderr
+            runs.append(open(res).read().splitlines())
+    full, a, b = runs
+    assert len(full) == 6 and len(a) == 3 and len(b) == 3
+    assert a + b == full

@@ -218,3 +218,36 @@ def test_main_usage_message():
         open(path, "w").write("4 4\n8\n")
         out = subprocess.run([MAIN, path, os.path.join(d, "r.csv")], capture_output=True, text=True, timeout=60)
         assert out.stdout.strip() == "Error: Need beta!"
+
+
+def test_config_writers_byte_identical(ref_available):
+    h, r = host(), ref()
+    for lib in (h, r):
+        lib.fprint_config.argtypes = [C.c_char_p, dp, C.c_uint, C.c_uint, C.c_uint, C.c_char_p]
+        lib.fwrite_config.argtypes = [C.c_char_p, dp, C.c_uint, C.c_char_p]
+    x = np.random.default_rng(9).standard_normal(3 * 6 * 4) * 1e3
+    outs = []
+    with tempfile.TemporaryDirectory() as d:
+        for lib, tag in ((h, "h"), (r, "r")):
+            a, b = os.path.join(d, tag + ".txt").encode(), os.path.join(d, tag + ".bin").encode()
+            lib.fprint_config(a, ptr(x), 6, 6, 4, b"w")
+            lib.fprint_config(a, ptr(x), 6, 6, 4, b"a")
+            lib.fwrite_config(b, ptr(x), x.size, b"w")
+            outs.append((open(a, "rb").read(), open(b, "rb").read()))
+    assert outs[0] == outs[1]
+
+
+def test_state_token_parsed():
+    h = host()
+    h.organic_state_file.restype = C.c_char_p
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "in.txt")
+        open(path, "w").write("CONSTANT_FORCE\nSTATE /tmp/chain.state\nSEED 5\n")
+        f = libc().fopen(path.encode(), b"r")
+        fl = Flags()
+        h.set_flags(f, C.byref(fl))
+        libc().fclose(f)
+    assert fl.constant_force == 1
+    assert h.organic_state_file() == b"/tmp/chain.state"
+    h.organic_set_state_file.argtypes = [C.c_char_p]
+    h.organic_set_state_file(None)
