@@ -100,7 +100,7 @@ __device__ __forceinline__ void gather6(const R *__restrict__ buf, const int (&n
 	for(int d = 0; d < 6; d++){
 		const R *q = buf + (size_t)nbr[d] * C;
 #pragma unroll
-		for(int c = 0; c < C; c++) out[c] += cf[d] * q[c];
+		for(int c = 0; c < C; c++) out[c] = fmac(cf[d], q[c], out[c]);
 	}
 }
 
@@ -183,14 +183,14 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 					R z[C];
 					gather6<R, C>(buf1, nbr[j], cf[j], a.mu, w[j], z);
 #pragma unroll
-					for(int c = 0; c < C; c++) v[j][c] += a.a_lin[p] * w[j][c] + a.a_quad[p] * z[c];
+					for(int c = 0; c < C; c++) v[j][c] = v[j][c] + fmac(a.a_quad[p], z[c], a.a_lin[p] * w[j][c]);
 				}
 			}else{
 				__syncthreads();
 #pragma unroll
 				for(int j = 0; j < S; j++)
 #pragma unroll
-					for(int c = 0; c < C; c++) v[j][c] += a.a_lin[p] * w[j][c];
+					for(int c = 0; c < C; c++) v[j][c] = fmac(a.a_lin[p], w[j][c], v[j][c]);
 			}
 		}
 	}
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 #pragma unroll
 					for(int j = 0; j < S; j++)
 #pragma unroll
-						for(int c = 0; c < C; c++) sa[q][j][c] += a.cm[q + b + 1] * t[j][c];
+						for(int c = 0; c < C; c++) sa[q][j][c] = fmac(a.cm[q + b + 1], t[j][c], sa[q][j][c]);
 		}
 
 		// W powers, half-bond sums H[i][d] = sum_{c,a} W_a[i,c] S_a[nb(i,d),c]
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 					const R *s2 = sb + (size_t)nbr[j][d] * C;
 					R hs = 0;
 #pragma unroll
-					for(int c = 0; c < C; c++) hs += wa[j][c] * s2[c];
+					for(int c = 0; c < C; c++) hs = fmac(wa[j][c], s2[c], hs);
 					h[j][d] += hs;
 				}
 				R wn[C];
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 #pragma unroll
 				for(int c = 0; c < C; c++){
 					wa[j][c] = wn[c];
-					acc[j][c] += a.cm[q + 1] * wn[c];
+					acc[j][c] = fmac(a.cm[q + 1], wn[c], acc[j][c]);
 				}
 			}
 			if(q + 1 < deg){
@@ -409,13 +409,13 @@ __global__ __launch_bounds__(LB) void adjoint_strip_kernel(AdjArgs<R> a)
 #pragma unroll
 				for(int k = 0; k < M; k++)
 #pragma unroll
-					for(int c = 0; c < C; c++) v[k][c] += a.a_lin[p] * w[k][c] + a.a_quad[p] * z[k][c];
+					for(int c = 0; c < C; c++) v[k][c] = v[k][c] + fmac(a.a_quad[p], z[k][c], a.a_lin[p] * w[k][c]);
 			}else{
 				__syncthreads();
 #pragma unroll
 				for(int k = 0; k < M; k++)
 #pragma unroll
-					for(int c = 0; c < C; c++) v[k][c] += a.a_lin[p] * w[k][c];
+					for(int c = 0; c < C; c++) v[k][c] = fmac(a.a_lin[p], w[k][c], v[k][c]);
 			}
 		}
 	}
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(LB) void adjoint_strip_kernel(AdjArgs<R> a)
 #pragma unroll
 					for(int k = 0; k < M; k++)
 #pragma unroll
-						for(int c = 0; c < C; c++) sa[q][k][c] += a.cm[q + b + 1] * t[k][c];
+						for(int c = 0; c < C; c++) sa[q][k][c] = fmac(a.cm[q + b + 1], t[k][c], sa[q][k][c]);
 		}
 
 		// W powers, half-bond sums H[i][d] = sum_{c,a} W_a[i,c] S_a[nb(i,d),c]
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(LB) void adjoint_strip_kernel(AdjArgs<R> a)
 								case 4: sval = sv.up[k][c]; break;
 								default: sval = sv.dn[k + 1][c]; break;
 							}
-							hs += wa[k][c] * sval;
+							hs = fmac(wa[k][c], sval, hs);
 						}
 						h[k][d] += hs;
 					}
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(LB) void adjoint_strip_kernel(AdjArgs<R> a)
 #pragma unroll
 				for(int c = 0; c < C; c++){
 					wa[k][c] = wn[k][c];
-					acc[k][c] += a.cm[q + 1] * wn[k][c];
+					acc[k][c] = fmac(a.cm[q + 1], wn[k][c], acc[k][c]);
 				}
 			if(q + 1 < deg){
 				R *wnext = wbs[(q + 1) & 1];

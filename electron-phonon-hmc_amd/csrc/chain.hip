@@ -73,7 +73,7 @@ __device__ __forceinline__ void stencil_gather(const R *__restrict__ buf, const 
 	for(int d = 0; d < 6; d++){
 		const R *p = buf + (size_t)nbr[d] * C;
 #pragma unroll
-		for(int c = 0; c < C; c++) out[c] += cf[d] * p[c];
+		for(int c = 0; c < C; c++) out[c] = fmac(cf[d], p[c], out[c]);
 	}
 }
 
@@ -147,14 +147,14 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 					R z[C];
 					stencil_gather<R, C>(W, nbr[j], cf[j], a.mu, w[j], z);
 #pragma unroll
-					for(int c = 0; c < C; c++) v[j][c] += a.a_lin[p] * w[j][c] + a.a_quad[p] * z[c];
+					for(int c = 0; c < C; c++) v[j][c] = v[j][c] + fmac(a.a_quad[p], z[c], a.a_lin[p] * w[j][c]);
 				}
 			}else{
 				__syncthreads();
 #pragma unroll
 				for(int j = 0; j < S; j++)
 #pragma unroll
-					for(int c = 0; c < C; c++) v[j][c] += a.a_lin[p] * w[j][c];
+					for(int c = 0; c < C; c++) v[j][c] = fmac(a.a_lin[p], w[j][c], v[j][c]);
 			}
 		}
 	}
@@ -227,13 +227,13 @@ __global__ __launch_bounds__(LB) void chain_strip_kernel(ChainArgs<R> a)
 #pragma unroll
 				for(int k = 0; k < M; k++)
 #pragma unroll
-					for(int c = 0; c < C; c++) v[k][c] += a.a_lin[p] * w[k][c] + a.a_quad[p] * z[k][c];
+					for(int c = 0; c < C; c++) v[k][c] = v[k][c] + fmac(a.a_quad[p], z[k][c], a.a_lin[p] * w[k][c]);
 			}else{
 				__syncthreads();
 #pragma unroll
 				for(int k = 0; k < M; k++)
 #pragma unroll
-					for(int c = 0; c < C; c++) v[k][c] += a.a_lin[p] * w[k][c];
+					for(int c = 0; c < C; c++) v[k][c] = fmac(a.a_lin[p], w[k][c], v[k][c]);
 			}
 		}
 	}

@@ -32,6 +32,11 @@ void set_error(const char *fmt, ...);
 
 #define EPHMC_TRY(expr) do { int r_ = (expr); if(r_ != EPHMC_OK) return r_; } while(0)
 
+// a * b + c rounded once.  The stencil kernels spell every multiply-add with it so that the
+// site and strip forms round identically whatever contraction the compiler would choose.
+__device__ __forceinline__ double fmac(double a, double b, double c){ return fma(a, b, c); }
+__device__ __forceinline__ float fmac(float a, float b, float c){ return fmaf(a, b, c); }
+
 constexpr int MAX_DEG = 8;      // polynomial degree of the slice exponential (2 * pairs)
 constexpr int MAX_PAIRS = 4;
 

@@ -58,12 +58,12 @@ __device__ __forceinline__ void strip_gather(const R *__restrict__ buf, const St
 			const R r = k < M - 1 ? own[k + 1][c] : rt[c];
 			const R l = k > 0 ? own[k - 1][c] : lf[c];
 			R o = mu * own[k][c];
-			o += cf[k][0] * r;
-			o += cf[k][1] * l;
-			o += cf[k][2] * up[k + 1][c];
-			o += cf[k][3] * dn[k][c];
-			o += cf[k][4] * up[k][c];
-			o += cf[k][5] * dn[k + 1][c];
+			o = fmac(cf[k][0], r, o);
+			o = fmac(cf[k][1], l, o);
+			o = fmac(cf[k][2], up[k + 1][c], o);
+			o = fmac(cf[k][3], dn[k][c], o);
+			o = fmac(cf[k][4], up[k][c], o);
+			o = fmac(cf[k][5], dn[k + 1][c], o);
 			out[k][c] = o;
 		}
 }
