@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void gj_panel_np_kernel(const double *__restri
 		}else{
 			const double fi = colb[b][r] * inv;
 #pragma unroll
-			for(int q = 0; q < 4; q++) v[q] = (c0 + q == g) ? -fi : v[q] - fi * rowb[b][c0 + q];
+			for(int q = 0; q < 4; q++) v[q] = (c0 + q == g) ? -fi : fmac(-fi, rowb[b][c0 + q], v[q]);
 			if(r > g && g >= c0 && g < c0 + 4){
 				lm[r][g] = fi;
 				bad |= !(fabs(fi) <= 1.0);
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void gj_panel_np_kernel(const double *__restri
 		for(int j = 0; j < NB; j++){
 			double s = 0;
 #pragma unroll
-			for(int m = 0; m < NB; m++) s += a[m] * dinv[m][j];
+			for(int m = 0; m < NB; m++) s = fmac(a[m], dinv[m][j], s);
 			e[j] = -s;
 		}
 		if(i >= k0 + nbe){
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void gj_update_kernel(const double *__restrict
 	for(int kk = 0; kk < NB; kk++){
 		const double ev = E[kk][tr];
 #pragma unroll
-		for(int j = 0; j < 4; j++) acc[j] += ev * Y[kk][tc + 8 * j];
+		for(int j = 0; j < 4; j++) acc[j] = fmac(ev, Y[kk][tc + 8 * j], acc[j]);
 	}
 	if(row >= n) return;
 #pragma unroll
@@ -180,56 +180,63 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	constexpr int NB = GJ_NB;
 	__shared__ double dinv[NB][NB + 1];
 	__shared__ double lm[NB][NB + 1];
-	__shared__ double rowb[2][NB], colb[2][NB];
+	__shared__ double colb[2][NB];
 	__shared__ double Es[GJ_RB][NB + 1];
 	__shared__ double Ys[NB][GJ_CB + 1];
 	const int tid = threadIdx.x;
 	const unsigned r0 = blockIdx.x * GJ_RB, c0 = blockIdx.y * GJ_CB;
 	const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
-	// ---- D^{-1}: thread owns (r, 4q..4q+3) of the 32 x 32 block
-	{
-		const int r = tid >> 3, cq = (tid & 7) * 4;
-		double v[4];
+	const int wave = tid >> 6, lane = tid & 63;
+	if(wave == 0){
+		// ---- D^{-1} by one wave, no barriers: lane c holds column c of [D | 1] (lanes 0..31 D,
+		// 32..63 the identity); step g broadcasts the pivot column through LDS (colb).  After
+		// the last step lanes 32..63 hold the columns of D^{-1}.
+		double col[NB];
 #pragma unroll
-		for(int q = 0; q < 4; q++){
-			const int cc = cq + q;
-			v[q] = (r < (int)nbe && cc < (int)nbe) ? A[(size_t)(k0 + cc) * n + k0 + r] : (r == cc ? 1.0 : 0.0);
-			lm[r][cc] = 0;
+		for(int r = 0; r < NB; r++){
+			if(lane < NB) col[r] = (r < (int)nbe && lane < (int)nbe) ? A[(size_t)(k0 + lane) * n + k0 + r] : (r == lane ? 1.0 : 0.0);
+			else col[r] = (r == lane - NB) ? 1.0 : 0.0;
+			if(lane < NB) lm[r][lane] = 0;
 		}
 		bool bad = false;
-		for(int g = 0; g < (int)nbe; g++){
-			const int b = g & 1;
-			if(r == g){
 #pragma unroll
-				for(int q = 0; q < 4; q++) rowb[b][cq + q] = v[q];
-			}
-			if(g >= cq && g < cq + 4){
+		for(int g = 0; g < NB; g++){
+			if(g < (int)nbe){
+				double *pc = colb[g & 1];
+				if(lane == g){
 #pragma unroll
-				for(int q = 0; q < 4; q++) if(cq + q == g) colb[b][r] = v[q];
-			}
-			__syncthreads();
-			const double d = rowb[b][g];
-			const double inv = 1.0 / d;
-			if(r == g){
-#pragma unroll
-				for(int q = 0; q < 4; q++) v[q] = (cq + q == g) ? inv : v[q] * inv;
-			}else{
-				const double fi = colb[b][r] * inv;
-#pragma unroll
-				for(int q = 0; q < 4; q++) v[q] = (cq + q == g) ? -fi : v[q] - fi * rowb[b][cq + q];
-				if(r > g && g >= cq && g < cq + 4){
-					lm[r][g] = fi;
-					bad |= !(fabs(fi) <= 1.0);
+					for(int r = 0; r < NB; r++) pc[r] = col[r];
 				}
-			}
-			if(tid == 0 && lead){ diag[k0 + g] = d; piv[k0 + g] = (int)(k0 + g); }
-		}
+				__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				const double d = pc[g];
+				const double inv = 1.0 / d;
+				const double pr = col[g];
 #pragma unroll
-		for(int q = 0; q < 4; q++) dinv[r][cq + q] = v[q];
-		if(bad && lead) scal[SCAL_GJ_FLAG] = 1;
+				for(int r = 0; r < NB; r++){
+					if(r == g) col[r] = pr * inv;
+					else{
+						const double fi = pc[r] * inv;
+						col[r] = fmac(-fi, pr, col[r]);
+						if(r > g){
+							if(lane == 0) lm[r][g] = fi;
+							bad |= !(fabs(fi) <= 1.0);
+						}
+					}
+				}
+				if(lane == 0 && lead){ diag[k0 + g] = d; piv[k0 + g] = (int)(k0 + g); }
+				__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+			}
+		}
+		if(lane >= NB){
+#pragma unroll
+			for(int r = 0; r < NB; r++) dinv[r][lane - NB] = col[r];
+		}
+		if(bad && lead && lane == 0) scal[SCAL_GJ_FLAG] = 1;
 	}
-	// ---- A[K, column block] (old values; the output goes to the other buffer)
-	for(int e = tid; e < NB * GJ_CB; e += 256){
+	// ---- A[K, column block] (old values; the output goes to the other buffer), other waves
+	for(int e = tid - 64; e >= 0 && e < NB * GJ_CB; e += 192){
 		const int m = e % NB, cc = e / NB;
 		const unsigned col = c0 + cc;
 		Ys[m][cc] = (m < (int)nbe && col < n) ? A[(size_t)col * n + k0 + m] : 0.0;
@@ -250,7 +257,7 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 				for(int m = 0; m < (int)nbe; m++){
 					const double a = A[(size_t)(k0 + m) * n + i];
 #pragma unroll
-					for(int j = 0; j < 8; j++) e8[j] += a * dinv[m][j0 + j];
+					for(int j = 0; j < 8; j++) e8[j] = fmac(a, dinv[m][j0 + j], e8[j]);
 				}
 #pragma unroll
 			for(int j = 0; j < 8; j++) e8[j] = -e8[j];
@@ -290,7 +297,7 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		else{
 			double acc = rowK ? 0.0 : A[(size_t)col * n + i];
 #pragma unroll
-			for(int m = 0; m < NB; m++) acc += er[m] * Ys[m][cc];
+			for(int m = 0; m < NB; m++) acc = fmac(er[m], Ys[m][cc], acc);
 			out = acc;
 		}
 		An[(size_t)col * n + i] = out;
