@@ -416,6 +416,8 @@ int ephmc_destroy(ephmc_ctx *c)
 		c->d_x, c->d_p, c->d_pdot, c->d_xnew, c->d_normals, c->d_minv, c->d_mhat, c->d_Rt0, c->d_R0t, c->d_ccg};
 	for(void *b : bufs) if(b) (void)hipFree(b);
 	tl_release(c);
+	if(c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+	if(c->graph) (void)hipGraphDestroy(c->graph);
 	for(int s = 0; s < 2; s++){
 		if(c->h_stage[s]) (void)hipHostFree(c->h_stage[s]);
 		if(c->stage_ev[s]) (void)hipEventDestroy(c->stage_ev[s]);
@@ -607,6 +609,51 @@ int ephmc_chain_set_x(ephmc_ctx *c, const double *x_host)
 	return EPHMC_OK;
 }
 
+// Everything of a trajectory between the momentum draw and the Metropolis decision (fixed
+// device pointers, no host values), so that it can be captured once and replayed.
+static int trajectory_body(ephmc_ctx *c)
+{
+	EPHMC_HIP(hipMemcpyAsync(c->d_xnew, c->d_x, c->dim * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+	EPHMC_TRY(momenta(c, c->d_normals, c->d_p));
+	EPHMC_TRY(hamilton_dev(c, c->d_xnew, c->d_p, SCAL_LOGDET_CHAIN, SCAL_H_OLD));
+	EPHMC_TRY(leap_frog_dev(c, c->d_xnew, c->d_p));
+	EPHMC_TRY(timer_begin(c, "metropolis"));
+	EPHMC_TRY(exact_logdet(c, c->d_xnew));
+	EPHMC_TRY(hamilton_dev(c, c->d_xnew, c->d_p, SCAL_LOGDET, SCAL_H_NEW));
+	return EPHMC_OK;
+}
+
+// Captures trajectory_body into a hipGraph.  A capture that fails leaves the context on the
+// eager path (graph_failed), with the reason in ephmc_last_error.
+static int capture_body(ephmc_ctx *c)
+{
+	if(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess){
+		(void)hipGetLastError();
+		c->graph_failed = true;
+		return EPHMC_OK;
+	}
+	const int rc = trajectory_body(c);
+	hipGraph_t g = nullptr;
+	const hipError_t e = hipStreamEndCapture(c->stream, &g);
+	if(rc != EPHMC_OK || e != hipSuccess || !g){
+		(void)hipGetLastError();
+		if(g) (void)hipGraphDestroy(g);
+		c->graph_failed = true;
+		set_error("graph capture of the trajectory failed; running eagerly");
+		return EPHMC_OK;
+	}
+	if(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0) != hipSuccess){
+		(void)hipGetLastError();
+		(void)hipGraphDestroy(g);
+		c->graph_exec = nullptr;
+		c->graph_failed = true;
+		set_error("graph instantiation failed; running eagerly");
+		return EPHMC_OK;
+	}
+	c->graph = g;
+	return EPHMC_OK;
+}
+
 int ephmc_trajectory(ephmc_ctx *c, const double *normals, double uniform, ephmc_traj_record *rec)
 {
 	if(!c || !normals){ set_error("null argument"); return EPHMC_ERR_ARG; }
@@ -627,13 +674,13 @@ int ephmc_trajectory(ephmc_ctx *c, const double *normals, double uniform, ephmc_
 		return EPHMC_OK;
 	}
 	// organic_hmc.c:213-243
-	EPHMC_HIP(hipMemcpyAsync(c->d_xnew, c->d_x, c->dim * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-	EPHMC_TRY(momenta(c, c->d_normals, c->d_p));
-	EPHMC_TRY(hamilton_dev(c, c->d_xnew, c->d_p, SCAL_LOGDET_CHAIN, SCAL_H_OLD));
-	EPHMC_TRY(leap_frog_dev(c, c->d_xnew, c->d_p));
-	EPHMC_TRY(timer_begin(c, "metropolis"));
-	EPHMC_TRY(exact_logdet(c, c->d_xnew));
-	EPHMC_TRY(hamilton_dev(c, c->d_xnew, c->d_p, SCAL_LOGDET, SCAL_H_NEW));
+	if(c->cfg.graph && !c->timing && !c->graph_failed){
+		if(!c->graph_exec) EPHMC_TRY(capture_body(c));
+		if(c->graph_exec) EPHMC_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+		else EPHMC_TRY(trajectory_body(c));
+	}else{
+		EPHMC_TRY(trajectory_body(c));
+	}
 	EPHMC_TRY(metropolis(c, uniform));
 	EPHMC_TRY(timer_end(c, "metropolis"));
 	if(rec) return ephmc_last_record(c, rec);

@@ -140,6 +140,11 @@ struct ephmc_ctx {
 	int stage_next;
 	bool chain_ready;
 
+	// captured trajectory body (ephmc_config.graph)
+	hipGraph_t graph;
+	hipGraphExec_t graph_exec;
+	bool graph_failed;
+
 	// timing
 	bool timing;
 	std::map<std::string, ephmc::TimerGroup> timers;

@@ -27,7 +27,7 @@ class Config(C.Structure):
                 ("zero_filling", C.c_int), ("single_particle", C.c_int), ("constant_force", C.c_int),
                 ("no_fourier_acc", C.c_int), ("classical_fourier_acc", C.c_int),
                 ("cfa_steps", C.c_uint), ("cfa_mass", C.c_double),
-                ("device", C.c_int), ("stream", C.c_void_p)]
+                ("device", C.c_int), ("stream", C.c_void_p), ("graph", C.c_int)]
 
 
 class Record(C.Structure):

@@ -53,6 +53,8 @@ typedef struct ephmc_config {
 	double cfa_mass;
 	int device;              /* HIP device ordinal                                       */
 	void *stream;            /* hipStream_t to run on; NULL creates a default-priority one */
+	int graph;               /* 1: ephmc_trajectory replays its molecular dynamics as a
+	                            captured hipGraph (captured on first use; not while timing) */
 } ephmc_config;
 
 /* Per-trajectory record (organic_hmc.c:233-245, results columns 3,4,7,8). */
