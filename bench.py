@@ -12,6 +12,7 @@ accept uniforms are synthetic (numpy, seed 3) and resident in host memory; the c
 Prints one JSON line (rank 0, one GPU).
 """
 import argparse
+import collections
 import json
 import math
 import os
@@ -47,15 +48,19 @@ def chain_flops(ns, nt):
     return 60 * ns * ns * nt
 
 
-def run_chain(E, precision, normals_list, uniforms, x0_normals, warmup, steps):
+def run_chain(E, precision, normals_list, uniforms, x0_normals, warmup, steps, graph=False):
+    """Chain of `steps` timed trajectories after `warmup`.  Eager launches record the kernel
+    groups with HIP events on the context stream; graph mode (the trajectory body replayed as one
+    captured hipGraph) records no per-kernel events, only the wall time."""
     t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, NT)
     ctx = E.Context(L, L, NT, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=precision,
-                    checkpoint=K_CKPT)
+                    checkpoint=K_CKPT, graph=int(graph))
     ctx.chain_init(x0_normals)
-    for i in range(warmup):
+    for i in range(max(warmup, 1 if graph else 0)):
         ctx.trajectory(normals_list[i % len(normals_list)], uniforms[i % len(uniforms)], wait=False)
     ctx.sync()
-    ctx.timing(True)
+    if not graph:
+        ctx.timing(True)
     ctx.sync()
     t0 = time.perf_counter()
     for i in range(steps):
@@ -63,9 +68,11 @@ def run_chain(E, precision, normals_list, uniforms, x0_normals, warmup, steps):
         ctx.trajectory(normals_list[j], uniforms[j], wait=False)
     ctx.sync()
     el = time.perf_counter() - t0
-    groups = {g: ctx.timing_get(g) for g in ("chain", "gj", "adjoint", "adjoint_kernel", "force_reduce", "fft",
-                                             "metropolis")}
-    ctx.timing(False)
+    groups = {}
+    if not graph:
+        groups = {g: ctx.timing_get(g) for g in ("chain", "gj", "adjoint", "adjoint_kernel", "force_reduce", "fft",
+                                                 "metropolis")}
+        ctx.timing(False)
     acc = ctx.last_record()
     return ctx, el, groups, acc
 
@@ -156,6 +163,49 @@ def tl_leg(E, nsamples):
             "greens_sum": float(g.sum())}
 
 
+def c4_leg(E, ntraj, nchains=64, graph=True):
+    """Independent chains of uneven size (L from {8,12,16,20,24}, Nt from {32,64,128}, drawn with
+    MT19937-64 seed 0; momentum normals from numpy generators seeded 1000+i), mixed precision,
+    each context on its own HIP stream, trajectories issued round-robin without waiting:
+    aggregate trajectories/s."""
+    import ctypes as C
+    host = C.CDLL(os.path.join(ROOT, "electron-phonon-hmc_amd", "libephmc_host.so"))
+    host.init_genrand64.argtypes = [C.c_ulong]
+    host.genrand64_int64.restype = C.c_ulong
+    host.init_genrand64(0)
+    Ls, Nts = (8, 12, 16, 20, 24), (32, 64, 128)
+    chains = []
+    for i in range(nchains):
+        l = Ls[host.genrand64_int64() % 5]
+        nt = Nts[host.genrand64_int64() % 3]
+        t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, nt)
+        ctx = E.Context(l, l, nt, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=E.MIXED,
+                        checkpoint=K_CKPT, graph=int(graph))
+        rng = np.random.default_rng(1000 + i)
+        dim = nt * 3 * l * l
+        draws = [rng.standard_normal(dim) for _ in range(ntraj + 1)]
+        us = rng.random(ntraj + 1)
+        ctx.chain_init(rng.standard_normal(dim))
+        chains.append((ctx, draws, us, l, nt))
+    for ctx, draws, us, _, _ in chains:              # warm-up (graph capture)
+        ctx.trajectory(draws[0], float(us[0]), wait=False)
+    for ctx, *_ in chains:
+        ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(1, ntraj + 1):
+        for ctx, draws, us, _, _ in chains:
+            ctx.trajectory(draws[k], float(us[k]), wait=False)
+    for ctx, *_ in chains:
+        ctx.sync()
+    el = time.perf_counter() - t0
+    sizes = collections.Counter((l, nt) for *_, l, nt in chains)
+    for ctx, *_ in chains:
+        ctx.close()
+    return {"traj_per_s": nchains * ntraj / el, "chains": nchains, "trajectories_per_chain": ntraj,
+            "seconds": el, "launch": "graph" if graph else "eager",
+            "sizes": {"%dx%d,Nt=%d" % (l, l, nt): n for (l, nt), n in sorted(sizes.items())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,7 +214,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tl", action="store_true", help="skip the transient-localisation leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-chain leg")
+    ap.add_argument("--batch-traj", type=int, default=3)
     ap.add_argument("--tl-samples", type=int, default=256)
+    ap.add_argument("--graph", action="store_true",
+                    help="also time both chains with the trajectory body replayed as a hipGraph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy (known bytes) for HBM counter calibration")
     args = ap.parse_args()
@@ -187,6 +241,11 @@ def main():
         ctx, el, groups, rec = run_chain(E, prec, normals_list, uniforms, x0, args.warmup, args.steps)
         res[name] = {"el": el, "groups": groups, "ctx": ctx}
         log(name, "%.2f ms/traj" % (1e3 * el / args.steps), {g: "%.3f ms x %d" % v for g, v in groups.items()})
+    if args.graph:
+        for name, prec in (("fp64_graph", E.FP64), ("mixed_graph", E.MIXED)):
+            ctx, el, groups, rec = run_chain(E, prec, normals_list, uniforms, x0, args.warmup, args.steps, graph=True)
+            res[name] = {"el": el, "groups": groups, "ctx": ctx}
+            log(name, "%.2f ms/traj" % (1e3 * el / args.steps))
     x_state = res["fp64"]["ctx"].chain_x()
     for r in res.values():
         r["ctx"].close()
@@ -215,9 +274,14 @@ def main():
         "mixed_over_fp64": tpsmx / tps64,
         "force_rel_err": acc["force_rel_err"], "dH_abs_err": acc["dH_abs_err"],
         "logdet_rel_err_metropolis": acc["logdet_rel_err"],
-        "kernel_ms": {p: {g: round(v[0], 4) for g, v in res[p]["groups"].items()} for p in res},
+        "kernel_ms": {p: {g: round(v[0], 4) for g, v in res[p]["groups"].items()} for p in res if res[p]["groups"]},
         "roofline": roof,
     }
+    if args.graph:
+        out["graph_traj_per_s"] = {"fp64": args.steps / res["fp64_graph"]["el"], "mixed": args.steps / res["mixed_graph"]["el"]}
+    if not args.no_batch:
+        out["chains64"] = c4_leg(E, args.batch_traj)
+        log("chains64", out["chains64"])
     if not args.no_tl:
         out["tl"] = tl_leg(E, args.tl_samples)
         log("tl", out["tl"])
