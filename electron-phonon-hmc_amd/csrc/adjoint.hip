@@ -51,40 +51,46 @@ struct AdjArgs {
 };
 
 // Deterministic sum over workgroups of the partial bond sums of one segment:
-// F[tau][link] = f_k sum_w partial[tau - tau_b][w][link].  The links [l0, l0 + nl) are summed by
-// `nthreads` threads: thread t handles link t % P and the workgroup chunk t / P (P = min(nl,
-// nthreads)); chunk sums are combined in chunk order through `scratch` (nthreads doubles).
+// F[tau][link] = f_k sum_w partial[tau - tau_b][w][link].  The workgroups are split into
+// RED_CHUNKS fixed ranges; every link is the sum of its chunk sums taken in chunk order, so the
+// result does not depend on how many threads run the reduction.  The links [l0, l0 + nl) are
+// processed in groups of G = nthreads / RED_CHUNKS (at least 1); `scratch` holds
+// max(nthreads, RED_CHUNKS) doubles.
+constexpr unsigned RED_CHUNKS = 4;
+
 __device__ void reduce_links(const double *__restrict__ part, unsigned nwg, unsigned loc, unsigned l0, unsigned nl,
 		unsigned nlinks_total, int tau_b, const double (&f)[3], const double *__restrict__ fscale, double *__restrict__ pdot,
 		double *__restrict__ p, double kick, double *scratch, unsigned nthreads)
 {
 	const double fs = fscale ? *fscale : 1.0;
 	const unsigned t = threadIdx.x;
-	const unsigned P = nl < nthreads ? nl : nthreads;
-	if(P == 0) return;
-	unsigned CH = nthreads / P;
-	if(CH > nwg) CH = nwg;
-	for(unsigned g0 = 0; g0 < nl; g0 += P){
-		const unsigned ll = t % P, ch = t / P;
-		double s = 0;
-		const unsigned lk = l0 + g0 + ll;
-		if(ch < CH && g0 + ll < nl && lk < nlinks_total){
-			const unsigned tl = lk / loc, link = lk - tl * loc;
-			const unsigned w0 = (unsigned)((size_t)nwg * ch / CH), w1 = (unsigned)((size_t)nwg * (ch + 1) / CH);
-			const double *src = part + (size_t)tl * nwg * loc + link;
-			for(unsigned w = w0; w < w1; w++) s += src[(size_t)w * loc];
+	const unsigned CH = nwg < RED_CHUNKS ? nwg : RED_CHUNKS;
+	const unsigned G = nthreads / CH ? nthreads / CH : 1;
+	for(unsigned g0 = 0; g0 < nl; g0 += G){
+		for(unsigned it = t; it < G * CH; it += nthreads){
+			const unsigned ll = it % G, ch = it / G;
+			double s = 0;
+			const unsigned lk = l0 + g0 + ll;
+			if(g0 + ll < nl && lk < nlinks_total){
+				const unsigned tl = lk / loc, link = lk - tl * loc;
+				const unsigned w0 = (unsigned)((size_t)nwg * ch / CH), w1 = (unsigned)((size_t)nwg * (ch + 1) / CH);
+				const double *src = part + (size_t)tl * nwg * loc + link;
+				for(unsigned w = w0; w < w1; w++) s += src[(size_t)w * loc];
+			}
+			scratch[ch * G + ll] = s;
 		}
-		if(ch < CH) scratch[ch * P + ll] = s;
 		__syncthreads();
-		if(t < P && g0 + t < nl && l0 + g0 + t < nlinks_total){
-			double tot = 0;
-			for(unsigned q = 0; q < CH; q++) tot += scratch[q * P + t];
-			const unsigned lk2 = l0 + g0 + t;
-			const unsigned tl = lk2 / loc, link = lk2 - tl * loc;
-			const double F = f[link % 3] * tot * fs;
-			const size_t pos = (size_t)(tau_b + tl) * loc + link;
-			if(pdot) pdot[pos] = F;
-			if(p) p[pos] += kick * F;
+		for(unsigned ll = t; ll < G; ll += nthreads){
+			const unsigned lk2 = l0 + g0 + ll;
+			if(g0 + ll < nl && lk2 < nlinks_total){
+				double tot = 0;
+				for(unsigned q = 0; q < CH; q++) tot += scratch[q * G + ll];
+				const unsigned tl = lk2 / loc, link = lk2 - tl * loc;
+				const double F = f[link % 3] * tot * fs;
+				const size_t pos = (size_t)(tau_b + tl) * loc + link;
+				if(pdot) pdot[pos] = F;
+				if(p) p[pos] += kick * F;
+			}
 		}
 		__syncthreads();
 	}
@@ -627,7 +633,8 @@ template <typename R, int C, int M>
 static int launch_adjoint_strip(ephmc_ctx *c, AdjArgs<R> a)
 {
 	const unsigned ns = a.ns, bt = ns / M;
-	const size_t lds = (((4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7) + bt * sizeof(double);
+	const size_t lds = (((4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7)
+		+ std::max<unsigned>(bt, RED_CHUNKS) * sizeof(double);
 	if(lds > 160 * 1024){ set_error("adjoint: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
 #define EPHMC_ADJS_LAUNCH2(LB, MAXD)                                                              \
 	do {                                                                                          \
