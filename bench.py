@@ -221,7 +221,11 @@ def main():
                     help="also time both chains with the trajectory body replayed as a hipGraph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy (known bytes) for HBM counter calibration")
+    ap.add_argument("--copy-only", action="store_true", help="run only the calibration copy and exit")
     args = ap.parse_args()
+    if args.copy_only:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; a single chain does not span GPUs here "
               "(independent chains and disorder samples shard trivially, see the design notes)", file=sys.stderr)

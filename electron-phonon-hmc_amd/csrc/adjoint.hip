@@ -216,18 +216,31 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 		}
 
 	const int deg = a.deg;
-	for(int tau = a.tau_e - 1; tau >= a.tau_b; tau--){
+	// coefficients and T of slice tau-1 are loaded while slice tau is processed (each thread
+	// reads back only the scratch entries it wrote itself)
+	R cfn[S][6], tn0[S][C];
+	auto load_slice = [&](int tau){
 		const R *cft = a.coef + (size_t)tau * 6 * ns;
-#pragma unroll
-		for(int j = 0; j < S; j++)
-#pragma unroll
-			for(int d = 0; d < 6; d++) cf[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
 		const R *tsrc = seg + (size_t)(tau - a.tau_b) * C * ns;
+#pragma unroll
+		for(int j = 0; j < S; j++){
+#pragma unroll
+			for(int d = 0; d < 6; d++) cfn[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
+#pragma unroll
+			for(int c = 0; c < C; c++) tn0[j][c] = act[j] ? tsrc[(size_t)c * ns + site[j]] : (R)0;
+		}
+	};
+	load_slice(a.tau_e - 1);
+	for(int tau = a.tau_e - 1; tau >= a.tau_b; tau--){
 		R t[S][C];
 #pragma unroll
-		for(int j = 0; j < S; j++)
+		for(int j = 0; j < S; j++){
 #pragma unroll
-			for(int c = 0; c < C; c++) t[j][c] = act[j] ? tsrc[(size_t)c * ns + site[j]] : (R)0;
+			for(int d = 0; d < 6; d++) cf[j][d] = cfn[j][d];
+#pragma unroll
+			for(int c = 0; c < C; c++) t[j][c] = tn0[j][c];
+		}
+		if(tau > a.tau_b) load_slice(tau - 1);
 
 		// S_a = sum_b c_{a+b+1} T_b, built while the powers T_b = A^b T are formed
 		R sa[MAXD][S][C];
@@ -653,11 +666,13 @@ static int launch_adjoint_strip(ephmc_ctx *c, AdjArgs<R> a)
 	return EPHMC_OK;
 }
 
-// Strip width of the adjoint sweep (0: site-per-thread form): 2 at fp64, 4 at fp32 (register
-// budget), when l1 is a multiple of it.  EPHMC_ADJOINT_STRIP=0/2/4 overrides it.
+// Strip width of the adjoint sweep (0: site-per-thread form).  Off by default: at 32x32 the
+// site form is faster (the strip form's register footprint leaves one wave per SIMD).
+// EPHMC_ADJOINT_STRIP=2/4 selects it when l1 is a multiple of the width.
 static unsigned adjoint_strip(const ephmc_ctx *c, int prec)
 {
-	unsigned m = prec == EPHMC_FP64 ? 2 : 4;
+	(void)prec;
+	unsigned m = 0;
 	if(const char *e = getenv("EPHMC_ADJOINT_STRIP")) m = (unsigned)atoi(e);
 	if(m != 2 && m != 4) return 0;
 	if(c->l1 % m != 0) m = (m == 4 && c->l1 % 2 == 0) ? 2 : 0;

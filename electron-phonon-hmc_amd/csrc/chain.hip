@@ -106,8 +106,18 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 		}
 	}
 
+	// coefficients of the next slice are loaded one slice ahead, so their latency overlaps the
+	// stencils of the current one
+	R cfn[S][6];
+	auto load_cf = [&](int i){
+		const R *cft = a.coef + (size_t)(a.tau0 + a.dir * i) * 6 * ns;
+#pragma unroll
+		for(int j = 0; j < S; j++)
+#pragma unroll
+			for(int d = 0; d < 6; d++) cfn[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
+	};
+	if(a.ntau > 0) load_cf(0);
 	for(int i = 0; i < a.ntau; i++){
-		const int tau = a.tau0 + a.dir * i;
 		if(a.ck && (i % a.ck_every) == 0){
 			R *ck = a.ck + (size_t)(i / a.ck_every) * mat;
 #pragma unroll
@@ -116,11 +126,11 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 				for(int c = 0; c < C; c++)
 					if(act[j] && col0 + c < ns) ck[(size_t)(col0 + c) * ns + site[j]] = v[j][c];
 		}
-		const R *cft = a.coef + (size_t)tau * 6 * ns;
 #pragma unroll
 		for(int j = 0; j < S; j++)
 #pragma unroll
-			for(int d = 0; d < 6; d++) cf[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
+			for(int d = 0; d < 6; d++) cf[j][d] = cfn[j][d];
+		if(i + 1 < a.ntau) load_cf(i + 1);
 
 		for(int p = 0; p < a.np; p++){
 #pragma unroll
@@ -246,11 +256,12 @@ __global__ __launch_bounds__(LB) void chain_strip_kernel(ChainArgs<R> a)
 	}
 }
 
-// Strip width for this lattice (0: the strip form does not apply).  EPHMC_CHAIN_STRIP=0/2/4
-// overrides it for tuning sweeps.
+// Strip width for this lattice (0: site-per-thread form).  The strip form halves the LDS reads
+// but its register footprint leaves one wave per SIMD; measured at 32x32 it is slower, so it is
+// off unless EPHMC_CHAIN_STRIP=2/4 asks for it.
 unsigned strip_width(const ephmc_ctx *c)
 {
-	unsigned m = 4;
+	unsigned m = 0;
 	if(const char *e = getenv("EPHMC_CHAIN_STRIP")) m = (unsigned)atoi(e);
 	if(m != 2 && m != 4) return 0;
 	if(c->l1 % m != 0) m = (m == 4 && c->l1 % 2 == 0) ? 2 : 0;

@@ -183,6 +183,7 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	__shared__ double colb[2][NB];
 	__shared__ double Es[GJ_RB][NB + 1];
 	__shared__ double Ys[NB][GJ_CB + 1];
+	__shared__ double Ar[GJ_RB][NB + 1];     // A[row block, K]
 	const int tid = threadIdx.x;
 	const unsigned r0 = blockIdx.x * GJ_RB, c0 = blockIdx.y * GJ_CB;
 	const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
@@ -235,11 +236,27 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		}
 		if(bad && lead && lane == 0) scal[SCAL_GJ_FLAG] = 1;
 	}
-	// ---- A[K, column block] (old values; the output goes to the other buffer), other waves
+	// ---- loads that do not depend on D^{-1}, issued by waves 1..3 while wave 0 inverts:
+	// A[K, column block] (old values; the output goes to the other buffer) and A[row block, K]
 	for(int e = tid - 64; e >= 0 && e < NB * GJ_CB; e += 192){
 		const int m = e % NB, cc = e / NB;
 		const unsigned col = c0 + cc;
 		Ys[m][cc] = (m < (int)nbe && col < n) ? A[(size_t)col * n + k0 + m] : 0.0;
+	}
+	for(int e = tid - 64; e >= 0 && e < GJ_RB * NB; e += 192){
+		const int rr = e % GJ_RB, m = e / GJ_RB;
+		const unsigned i = r0 + rr;
+		Ar[rr][m] = (m < (int)nbe && i < n) ? A[(size_t)(k0 + m) * n + i] : 0.0;
+	}
+	// this thread's tile values: row tid % 64, columns (tid / 64) + 4 j
+	const int trr = tid & 63, cbase = tid >> 6;
+	const unsigned ti = r0 + trr;
+	const bool rowK = ti >= k0 && ti < k0 + nbe;
+	double av[GJ_CB / 4];
+#pragma unroll
+	for(int j = 0; j < GJ_CB / 4; j++){
+		const unsigned col = c0 + cbase + 4 * j;
+		av[j] = (ti < n && col < n && !rowK) ? A[(size_t)col * n + ti] : 0.0;
 	}
 	__syncthreads();
 	// ---- transform rows of this row block: thread (row tid/4, 8 columns)
@@ -253,12 +270,12 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		}else{
 #pragma unroll
 			for(int j = 0; j < 8; j++) e8[j] = 0;
-			if(i < n)
-				for(int m = 0; m < (int)nbe; m++){
-					const double a = A[(size_t)(k0 + m) * n + i];
+#pragma unroll 8
+			for(int m = 0; m < NB; m++){
+				const double a = Ar[rr][m];
 #pragma unroll
-					for(int j = 0; j < 8; j++) e8[j] = fmac(a, dinv[m][j0 + j], e8[j]);
-				}
+				for(int j = 0; j < 8; j++) e8[j] = fmac(a, dinv[m][j0 + j], e8[j]);
+			}
 #pragma unroll
 			for(int j = 0; j < 8; j++) e8[j] = -e8[j];
 		}
@@ -279,28 +296,25 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 			if(bad) scal[SCAL_GJ_FLAG] = 1;
 		}
 	}
-	// ---- tile update: thread owns row tid % 64 and columns (tid / 64) + 4 j, j < 16
-	const int rr = tid & 63, cbase = tid >> 6;
-	const unsigned i = r0 + rr;
-	if(i >= n) return;
-	const bool rowK = i >= k0 && i < k0 + nbe;
+	// ---- tile update
+	if(ti >= n) return;
 	double er[NB];
 #pragma unroll
-	for(int m = 0; m < NB; m++) er[m] = Es[rr][m];
-#pragma unroll 4
+	for(int m = 0; m < NB; m++) er[m] = Es[trr][m];
+#pragma unroll
 	for(int j = 0; j < GJ_CB / 4; j++){
 		const int cc = cbase + 4 * j;
 		const unsigned col = c0 + cc;
 		if(col >= n) continue;
 		double out;
-		if(col >= k0 && col < k0 + nbe) out = Es[rr][col - k0];
+		if(col >= k0 && col < k0 + nbe) out = Es[trr][col - k0];
 		else{
-			double acc = rowK ? 0.0 : A[(size_t)col * n + i];
+			double acc = av[j];
 #pragma unroll
 			for(int m = 0; m < NB; m++) acc = fmac(er[m], Ys[m][cc], acc);
 			out = acc;
 		}
-		An[(size_t)col * n + i] = out;
+		An[(size_t)col * n + ti] = out;
 	}
 }
 
