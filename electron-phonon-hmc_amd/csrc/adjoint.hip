@@ -17,8 +17,6 @@
 // T_tau comes from checkpoints T_{s k} stored by the forward chain every k slices and is
 // recomputed inside the segment (stored to a workgroup-private scratch, then read backwards).
 #include "internal.h"
-#include "stencil.h"
-#include <cstdlib>
 
 namespace ephmc {
 
@@ -277,63 +275,55 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 						for(int c = 0; c < C; c++) sa[q][j][c] = fmac(a.cm[q + b + 1], t[j][c], sa[q][j][c]);
 		}
 
-		// W powers, half-bond sums H[i][d] = sum_{c,a} W_a[i,c] S_a[nb(i,d),c]
+		// W powers with the half-bond sums taken from the same neighbour reads:
+		// H[i][d] = sum_{q,c} S_q[i,c] W_q[nb(i,d),c]
 		R h[S][6];
-		R acc[S][C];
+		R acc[S][C], wa[S][C];
 #pragma unroll
 		for(int j = 0; j < S; j++){
 #pragma unroll
 			for(int d = 0; d < 6; d++) h[j][d] = 0;
 #pragma unroll
-			for(int c = 0; c < C; c++) acc[j][c] = a.cm[0] * wv[j][c];
+			for(int c = 0; c < C; c++){
+				acc[j][c] = a.cm[0] * wv[j][c];
+				wa[j][c] = wv[j][c];
+			}
 		}
-		R wa[S][C];
-#pragma unroll
-		for(int j = 0; j < S; j++)
-#pragma unroll
-			for(int c = 0; c < C; c++) wa[j][c] = wv[j][c];
-#pragma unroll
-		for(int j = 0; j < S; j++)
-			if(act[j])
-#pragma unroll
-				for(int c = 0; c < C; c++) wbs[0][(size_t)site[j] * C + c] = wa[j][c];
 #pragma unroll
 		for(int q = 0; q < MAXD; q++){
 			if(q >= deg) break;
-			R *sb = bufs[(q + deg - 1) & 1];
 			R *wb = wbs[q & 1];
 #pragma unroll
 			for(int j = 0; j < S; j++)
 				if(act[j])
 #pragma unroll
-					for(int c = 0; c < C; c++) sb[(size_t)site[j] * C + c] = sa[q][j][c];
+					for(int c = 0; c < C; c++) wb[(size_t)site[j] * C + c] = wa[j][c];
 			__syncthreads();
 #pragma unroll
 			for(int j = 0; j < S; j++){
 				if(!act[j]) continue;
+				R wn[C];
+#pragma unroll
+				for(int c = 0; c < C; c++) wn[c] = a.mu * wa[j][c];
 #pragma unroll
 				for(int d = 0; d < 6; d++){
-					const R *s2 = sb + (size_t)nbr[j][d] * C;
+					const R *qn = wb + (size_t)nbr[j][d] * C;
+					R nv[C];
+#pragma unroll
+					for(int c = 0; c < C; c++) nv[c] = qn[c];
 					R hs = 0;
 #pragma unroll
-					for(int c = 0; c < C; c++) hs = fmac(wa[j][c], s2[c], hs);
+					for(int c = 0; c < C; c++){
+						hs = fmac(sa[q][j][c], nv[c], hs);
+						wn[c] = fmac(cf[j][d], nv[c], wn[c]);
+					}
 					h[j][d] += hs;
 				}
-				R wn[C];
-				gather6<R, C>(wb, nbr[j], cf[j], a.mu, wa[j], wn);
 #pragma unroll
 				for(int c = 0; c < C; c++){
 					wa[j][c] = wn[c];
 					acc[j][c] = fmac(a.cm[q + 1], wn[c], acc[j][c]);
 				}
-			}
-			if(q + 1 < deg){
-				R *wnext = wbs[(q + 1) & 1];
-#pragma unroll
-				for(int j = 0; j < S; j++)
-					if(act[j])
-#pragma unroll
-						for(int c = 0; c < C; c++) wnext[(size_t)site[j] * C + c] = wa[j][c];
 			}
 		}
 		// bond(i, 2k) = H[i][2k] + H[nb(i,2k)][2k+1]
@@ -362,236 +352,6 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			if(act[j] && colok(c)) a.wbuf[(size_t)(col0 + c) * ns + site[j]] = wv[j][c];
 }
 
-// Strip form of adjoint_kernel (stencil.h): a thread owns M consecutive sites of a lattice row;
-// stencils and half-bond sums take the shared neighbours of the strip from registers.  Every
-// sum is taken in the same order as in adjoint_kernel, so both forms give identical bits.
-template <typename R, int C, int M, int MAXD, int LB>
-__global__ __launch_bounds__(LB) void adjoint_strip_kernel(AdjArgs<R> a)
-{
-	extern __shared__ __align__(16) unsigned char smem_raw[];
-	const unsigned ns = a.ns;
-	R *buf0 = reinterpret_cast<R *>(smem_raw);
-	R *buf1 = buf0 + (size_t)C * ns;
-	R *wb0 = buf1 + (size_t)C * ns;
-	R *wb1 = wb0 + (size_t)C * ns;
-	R *hx = wb1 + (size_t)C * ns;          // [site][3]
-	double *rscratch = reinterpret_cast<double *>(smem_raw + (((4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7));
-	R *bufs[2] = {buf0, buf1};
-	R *wbs[2] = {wb0, wb1};
-	if(a.red_src){
-		const unsigned total = (unsigned)a.red_ntau * 3 * ns;
-		const unsigned per = (total + a.nwg - 1) / a.nwg;
-		reduce_links(a.red_src, a.nwg, 3 * ns, blockIdx.x * per, per, total, a.red_tau_b, a.f, a.fscale, a.pdot, a.p, a.kick,
-			rscratch, blockDim.x);
-	}
-	const unsigned col0 = blockIdx.x * C;
-	R *seg = a.seg + (size_t)blockIdx.x * a.seg_stride;
-	const unsigned s0 = threadIdx.x * M;   // blockDim.x * M == ns
-	StripNb<M> sn;
-	strip_neighbours<M>(a.nb, ns, s0, sn);
-	auto colok = [&](int c){ return col0 + c < ns; };
-	R v[M][C], cf[M][6];
-
-	// ---- recompute T_tau, tau in [tau_b, tau_e), from the checkpoint --------------------
-#pragma unroll
-	for(int k = 0; k < M; k++)
-#pragma unroll
-		for(int c = 0; c < C; c++) v[k][c] = colok(c) ? a.ck[(size_t)(col0 + c) * ns + s0 + k] : (R)0;
-	for(int tau = a.tau_b; tau < a.tau_e; tau++){
-		R *dst = seg + (size_t)(tau - a.tau_b) * C * ns;
-#pragma unroll
-		for(int k = 0; k < M; k++)
-#pragma unroll
-			for(int c = 0; c < C; c++) dst[(size_t)c * ns + s0 + k] = v[k][c];
-		if(tau == a.tau_e - 1) break;
-		const R *cft = a.coef + (size_t)tau * 6 * ns;
-#pragma unroll
-		for(int k = 0; k < M; k++)
-#pragma unroll
-			for(int d = 0; d < 6; d++) cf[k][d] = cft[d * ns + s0 + k];
-		for(int p = 0; p < a.np; p++){
-			R w[M][C];
-#pragma unroll
-			for(int k = 0; k < M; k++)
-#pragma unroll
-				for(int c = 0; c < C; c++) buf0[(size_t)(s0 + k) * C + c] = v[k][c];
-			__syncthreads();
-			strip_gather<R, C, M>(buf0, sn, cf, a.mu, v, w);
-			if(a.cplx[p]){
-#pragma unroll
-				for(int k = 0; k < M; k++)
-#pragma unroll
-					for(int c = 0; c < C; c++) buf1[(size_t)(s0 + k) * C + c] = w[k][c];
-				__syncthreads();
-				R z[M][C];
-				strip_gather<R, C, M>(buf1, sn, cf, a.mu, w, z);
-#pragma unroll
-				for(int k = 0; k < M; k++)
-#pragma unroll
-					for(int c = 0; c < C; c++) v[k][c] = v[k][c] + fmac(a.a_quad[p], z[k][c], a.a_lin[p] * w[k][c]);
-			}else{
-				__syncthreads();
-#pragma unroll
-				for(int k = 0; k < M; k++)
-#pragma unroll
-					for(int c = 0; c < C; c++) v[k][c] = fmac(a.a_lin[p], w[k][c], v[k][c]);
-			}
-		}
-	}
-
-	// ---- W state --------------------------------------------------------------------------
-	R wv[M][C];
-#pragma unroll
-	for(int k = 0; k < M; k++)
-#pragma unroll
-		for(int c = 0; c < C; c++){
-			R val = 0;
-			if(colok(c)){
-				const size_t e = (size_t)(col0 + c) * ns + s0 + k;
-				val = a.w_init ? (R)a.w_init[e] : a.wbuf[e];
-			}
-			wv[k][c] = val;
-		}
-
-	const int deg = a.deg;
-	for(int tau = a.tau_e - 1; tau >= a.tau_b; tau--){
-		const R *cft = a.coef + (size_t)tau * 6 * ns;
-#pragma unroll
-		for(int k = 0; k < M; k++)
-#pragma unroll
-			for(int d = 0; d < 6; d++) cf[k][d] = cft[d * ns + s0 + k];
-		const R *tsrc = seg + (size_t)(tau - a.tau_b) * C * ns;
-		R t[M][C];
-#pragma unroll
-		for(int k = 0; k < M; k++)
-#pragma unroll
-			for(int c = 0; c < C; c++) t[k][c] = tsrc[(size_t)c * ns + s0 + k];
-
-		// S_a = sum_b c_{a+b+1} T_b, built while the powers T_b = A^b T are formed
-		R sa[MAXD][M][C];
-#pragma unroll
-		for(int q = 0; q < MAXD; q++)
-#pragma unroll
-			for(int k = 0; k < M; k++)
-#pragma unroll
-				for(int c = 0; c < C; c++) sa[q][k][c] = (q < deg) ? a.cm[q + 1] * t[k][c] : (R)0;
-		for(int b = 1; b < deg; b++){
-			R *bb = bufs[(b - 1) & 1];
-#pragma unroll
-			for(int k = 0; k < M; k++)
-#pragma unroll
-				for(int c = 0; c < C; c++) bb[(size_t)(s0 + k) * C + c] = t[k][c];
-			__syncthreads();
-			R tn[M][C];
-			strip_gather<R, C, M>(bb, sn, cf, a.mu, t, tn);
-#pragma unroll
-			for(int k = 0; k < M; k++)
-#pragma unroll
-				for(int c = 0; c < C; c++) t[k][c] = tn[k][c];
-#pragma unroll
-			for(int q = 0; q < MAXD; q++)
-				if(q + b < deg)
-#pragma unroll
-					for(int k = 0; k < M; k++)
-#pragma unroll
-						for(int c = 0; c < C; c++) sa[q][k][c] = fmac(a.cm[q + b + 1], t[k][c], sa[q][k][c]);
-		}
-
-		// W powers, half-bond sums H[i][d] = sum_{c,a} W_a[i,c] S_a[nb(i,d),c]
-		R h[M][6], acc[M][C], wa[M][C];
-#pragma unroll
-		for(int k = 0; k < M; k++){
-#pragma unroll
-			for(int d = 0; d < 6; d++) h[k][d] = 0;
-#pragma unroll
-			for(int c = 0; c < C; c++){
-				acc[k][c] = a.cm[0] * wv[k][c];
-				wa[k][c] = wv[k][c];
-			}
-		}
-#pragma unroll
-		for(int k = 0; k < M; k++)
-#pragma unroll
-			for(int c = 0; c < C; c++) wbs[0][(size_t)(s0 + k) * C + c] = wa[k][c];
-#pragma unroll
-		for(int q = 0; q < MAXD; q++){
-			if(q >= deg) break;
-			R *sb = bufs[(q + deg - 1) & 1];
-			R *wb = wbs[q & 1];
-#pragma unroll
-			for(int k = 0; k < M; k++)
-#pragma unroll
-				for(int c = 0; c < C; c++) sb[(size_t)(s0 + k) * C + c] = sa[q][k][c];
-			__syncthreads();
-			{
-				StripVals<R, C, M> sv;
-				strip_load<R, C, M>(sb, sn, sv);
-#pragma unroll
-				for(int k = 0; k < M; k++){
-#pragma unroll
-					for(int d = 0; d < 6; d++){
-						R hs = 0;
-#pragma unroll
-						for(int c = 0; c < C; c++){
-							R sval;
-							switch(d){
-								case 0: sval = k < M - 1 ? sa[q][k + 1][c] : sv.rt[c]; break;
-								case 1: sval = k > 0 ? sa[q][k - 1][c] : sv.lf[c]; break;
-								case 2: sval = sv.up[k + 1][c]; break;
-								case 3: sval = sv.dn[k][c]; break;
-								case 4: sval = sv.up[k][c]; break;
-								default: sval = sv.dn[k + 1][c]; break;
-							}
-							hs = fmac(wa[k][c], sval, hs);
-						}
-						h[k][d] += hs;
-					}
-				}
-			}
-			R wn[M][C];
-			strip_gather<R, C, M>(wb, sn, cf, a.mu, wa, wn);
-#pragma unroll
-			for(int k = 0; k < M; k++)
-#pragma unroll
-				for(int c = 0; c < C; c++){
-					wa[k][c] = wn[k][c];
-					acc[k][c] = fmac(a.cm[q + 1], wn[k][c], acc[k][c]);
-				}
-			if(q + 1 < deg){
-				R *wnext = wbs[(q + 1) & 1];
-#pragma unroll
-				for(int k = 0; k < M; k++)
-#pragma unroll
-					for(int c = 0; c < C; c++) wnext[(size_t)(s0 + k) * C + c] = wa[k][c];
-			}
-		}
-		// bond(i, 2k) = H[i][2k] + H[nb(i,2k)][2k+1]
-#pragma unroll
-		for(int k = 0; k < M; k++)
-#pragma unroll
-			for(int kk = 0; kk < 3; kk++) hx[(size_t)(s0 + k) * 3 + kk] = h[k][2 * kk + 1];
-		__syncthreads();
-		double *out = a.partial + ((size_t)(tau - a.tau_b) * a.nwg + blockIdx.x) * 3 * ns;
-#pragma unroll
-		for(int k = 0; k < M; k++){
-			const int nbr[3] = {k < M - 1 ? (int)(s0 + k + 1) : sn.rt, sn.up[k + 1], sn.up[k]};
-#pragma unroll
-			for(int kk = 0; kk < 3; kk++)
-				out[(size_t)(s0 + k) * 3 + kk] = (double)h[k][2 * kk] + (double)hx[(size_t)nbr[kk] * 3 + kk];
-		}
-#pragma unroll
-		for(int k = 0; k < M; k++)
-#pragma unroll
-			for(int c = 0; c < C; c++) wv[k][c] = acc[k][c];
-	}
-#pragma unroll
-	for(int k = 0; k < M; k++)
-#pragma unroll
-		for(int c = 0; c < C; c++)
-			if(colok(c)) a.wbuf[(size_t)(col0 + c) * ns + s0 + k] = wv[k][c];
-}
-
-// Stand-alone form of reduce_links for the last segment: 64 links x 4 chunks per workgroup.
 __global__ __launch_bounds__(256) void force_reduce_kernel(const double *__restrict__ partial, unsigned nwg, unsigned loc,
 		int tau_b, int ntau, double f0, double f1, double f2, const double *__restrict__ fscale, double *__restrict__ pdot,
 		double *__restrict__ p, double kick)
@@ -642,44 +402,6 @@ static int launch_adjoint_c(ephmc_ctx *c, AdjArgs<R> a)
 	return EPHMC_OK;
 }
 
-template <typename R, int C, int M>
-static int launch_adjoint_strip(ephmc_ctx *c, AdjArgs<R> a)
-{
-	const unsigned ns = a.ns, bt = ns / M;
-	const size_t lds = (((4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7)
-		+ std::max<unsigned>(bt, RED_CHUNKS) * sizeof(double);
-	if(lds > 160 * 1024){ set_error("adjoint: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
-#define EPHMC_ADJS_LAUNCH2(LB, MAXD)                                                              \
-	do {                                                                                          \
-		(void)hipFuncSetAttribute((const void *)adjoint_strip_kernel<R, C, M, MAXD, LB>,              \
-			hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                \
-		adjoint_strip_kernel<R, C, M, MAXD, LB><<<a.nwg, bt, lds, c->stream>>>(a);                \
-	} while(0)
-#define EPHMC_ADJS_LAUNCH(LB)                                                                     \
-	do { if(a.deg <= 4) EPHMC_ADJS_LAUNCH2(LB, 4); else EPHMC_ADJS_LAUNCH2(LB, 8); } while(0)
-	if(bt <= 256) EPHMC_ADJS_LAUNCH(256);
-	else if(bt <= 512) EPHMC_ADJS_LAUNCH(512);
-	else EPHMC_ADJS_LAUNCH(1024);
-#undef EPHMC_ADJS_LAUNCH
-#undef EPHMC_ADJS_LAUNCH2
-	EPHMC_LAUNCHED();
-	return EPHMC_OK;
-}
-
-// Strip width of the adjoint sweep (0: site-per-thread form).  Off by default: at 32x32 the
-// site form is faster (the strip form's register footprint leaves one wave per SIMD).
-// EPHMC_ADJOINT_STRIP=2/4 selects it when l1 is a multiple of the width.
-static unsigned adjoint_strip(const ephmc_ctx *c, int prec)
-{
-	(void)prec;
-	unsigned m = 0;
-	if(const char *e = getenv("EPHMC_ADJOINT_STRIP")) m = (unsigned)atoi(e);
-	if(m != 2 && m != 4) return 0;
-	if(c->l1 % m != 0) m = (m == 4 && c->l1 % 2 == 0) ? 2 : 0;
-	if(m == 0 || c->ns / m > 1024) return 0;
-	return m;
-}
-
 // Segments s = nseg-1 .. 0 of the reverse sweep; W starts from minv (fp64) or, when minv is null,
 // from the W buffer (already in the precision R).  nt_eff < nt restricts the sweep to slices
 // [0, nt_eff) (CONSTANT_FORCE: nt_eff = 1 with W_0 prepared by the caller).
@@ -716,7 +438,6 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 	const unsigned nseg = (nt + k - 1) / k;
 	int prev_b = 0, prev_n = 0;
 	const double *prev = nullptr;
-	const unsigned m = adjoint_strip(c, prec);
 	for(int s = (int)nseg - 1; s >= 0; s--){
 		a.tau_b = s * (int)k;
 		a.tau_e = std::min<int>((s + 1) * (int)k, (int)nt);
@@ -729,24 +450,10 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 		a.red_ntau = prev_n;
 		int rc;
 		EPHMC_TRY(timer_begin(c, "adjoint_kernel"));
-		if(m == 2){
-			switch(C){
-				case 1: rc = launch_adjoint_strip<R, 1, 2>(c, a); break;
-				case 2: rc = launch_adjoint_strip<R, 2, 2>(c, a); break;
-				default: rc = launch_adjoint_strip<R, 4, 2>(c, a); break;
-			}
-		}else if(m == 4){
-			switch(C){
-				case 1: rc = launch_adjoint_strip<R, 1, 4>(c, a); break;
-				case 2: rc = launch_adjoint_strip<R, 2, 4>(c, a); break;
-				default: rc = launch_adjoint_strip<R, 4, 4>(c, a); break;
-			}
-		}else{
-			switch(C){
-				case 1: rc = launch_adjoint_c<R, 1>(c, a); break;
-				case 2: rc = launch_adjoint_c<R, 2>(c, a); break;
-				default: rc = launch_adjoint_c<R, 4>(c, a); break;
-			}
+		switch(C){
+			case 1: rc = launch_adjoint_c<R, 1>(c, a); break;
+			case 2: rc = launch_adjoint_c<R, 2>(c, a); break;
+			default: rc = launch_adjoint_c<R, 4>(c, a); break;
 		}
 		if(rc != EPHMC_OK) return rc;
 		EPHMC_TRY(timer_end(c, "adjoint_kernel"));

@@ -9,11 +9,12 @@
 // the lattice parameters of the reference (mu at the band bottom) the multipliers are of order
 // 1e-2, so:
 //
-//   fast path: blocked Gauss-Jordan without row exchanges, 32 columns per block K = [k0, k0+nb):
-//     panel  D^{-1} of the diagonal block D = A[K,K] (one wave), then every row i:
-//            E[i,K] = -A[i,K] D^{-1} (i not in K) or D^{-1} rows (i in K); the LU multipliers
-//            of D and of the rows below K are checked |l| <= 1
-//     update every column c outside K: A'[:,c] = [i not in K] A[:,c] + E[:,K] A[K,c]
+//   fast path: blocked Gauss-Jordan without row exchanges, 32 columns per block K = [k0, k0+nb),
+//     one launch per block over 64 x 64 output tiles.  Every workgroup inverts the diagonal
+//     block D = A[K,K] (four waves, two pivots per barrier), forms the transform rows
+//     E[i,:] = -A[i,K] D^{-1} (i not in K) or D^{-1} rows (i in K) of its rows, and writes its
+//     tile of A' = [i not in K] A + E A[K,:] (columns inside K: E itself).  The LU multipliers
+//     of D and of the rows below K are checked |l| <= 1
 //   check:   any multiplier > 1 sets a device flag; the single-workgroup fallback kernel, always
 //            launched, then inverts the untouched input with partial pivoting (in-place
 //            Gauss-Jordan with row exchanges and the columns un-permuted at the end); without
@@ -30,291 +31,233 @@ __global__ void gj_clear_kernel(double *scal)
 	scal[SCAL_GJ_FLAG] = 0;
 }
 
-// Block step, part 1 (grid over rows, 256 threads per workgroup).
-// Every workgroup first inverts the diagonal block D = A[K,K] by Gauss-Jordan without row
-// exchanges: thread t owns the four entries (r, 4q..4q+3), r = t/8, q = t%8, in registers; at
-// step g the owners of row g and of column g publish them to LDS (double-buffered by step
-// parity, one barrier per step).  D^{-1} and the LU multipliers (rows below the pivot) end in
-// LDS; workgroup 0 records the pivots and checks |L| <= 1.  Then every thread forms the
-// transform row of one matrix row i:
-//   i in K:      E[i,:] = D^{-1}[i-k0,:]
-//   i not in K:  E[i,:] = -A[i,K] D^{-1};  rows below K check their LU multipliers
-//                l = A[i,K] U^{-1} = -E[i,:] L  (|l| <= 1).
-template <int NB>
-__global__ __launch_bounds__(256) void gj_panel_np_kernel(const double *__restrict__ A, double *__restrict__ An,
-		unsigned n, unsigned k0, unsigned nbe, double *__restrict__ diag, int *__restrict__ piv, double *__restrict__ scal)
+// 1/x: hardware reciprocal refined by two Newton steps
+__device__ inline double rcp_nr(double x)
 {
-	static_assert(NB == 32, "one 32 x 32 block per 256 threads");
-	__shared__ double dinv[NB][NB + 1];
-	__shared__ double lm[NB][NB + 1];        // lm[r][g]: multiplier of row r at step g (r > g)
-	__shared__ double rowb[2][NB], colb[2][NB];
-	const int tid = threadIdx.x;
-	const int r = tid >> 3, c0 = (tid & 7) * 4;
-	double v[4];
-#pragma unroll
-	for(int q = 0; q < 4; q++){
-		const int cc = c0 + q;
-		v[q] = (r < (int)nbe && cc < (int)nbe) ? A[(size_t)(k0 + cc) * n + k0 + r] : (r == cc ? 1.0 : 0.0);
-		lm[r][cc] = 0;
-	}
-	bool bad = false;
-	for(int g = 0; g < (int)nbe; g++){
-		const int b = g & 1;
-		if(r == g){
-#pragma unroll
-			for(int q = 0; q < 4; q++) rowb[b][c0 + q] = v[q];
-		}
-		if(g >= c0 && g < c0 + 4){
-#pragma unroll
-			for(int q = 0; q < 4; q++) if(c0 + q == g) colb[b][r] = v[q];
-		}
-		__syncthreads();
-		const double d = rowb[b][g];
-		const double inv = 1.0 / d;
-		if(r == g){
-#pragma unroll
-			for(int q = 0; q < 4; q++) v[q] = (c0 + q == g) ? inv : v[q] * inv;
-		}else{
-			const double fi = colb[b][r] * inv;
-#pragma unroll
-			for(int q = 0; q < 4; q++) v[q] = (c0 + q == g) ? -fi : fmac(-fi, rowb[b][c0 + q], v[q]);
-			if(r > g && g >= c0 && g < c0 + 4){
-				lm[r][g] = fi;
-				bad |= !(fabs(fi) <= 1.0);
-			}
-		}
-		if(tid == 0 && blockIdx.x == 0){ diag[k0 + g] = d; piv[k0 + g] = (int)(k0 + g); }
-	}
-#pragma unroll
-	for(int q = 0; q < 4; q++) dinv[r][c0 + q] = v[q];
-	if(bad && blockIdx.x == 0) scal[SCAL_GJ_FLAG] = 1;
-	__syncthreads();
-	const unsigned i = blockIdx.x * 256 + tid;
-	if(i >= n) return;
-	const bool inK = i >= k0 && i < k0 + nbe;
-	double e[NB];
-	if(inK){
-#pragma unroll
-		for(int j = 0; j < NB; j++) e[j] = dinv[i - k0][j];
-	}else{
-		double a[NB];
-#pragma unroll
-		for(int m = 0; m < NB; m++) a[m] = m < (int)nbe ? A[(size_t)(k0 + m) * n + i] : 0.0;
-#pragma unroll
-		for(int j = 0; j < NB; j++){
-			double s = 0;
-#pragma unroll
-			for(int m = 0; m < NB; m++) s = fmac(a[m], dinv[m][j], s);
-			e[j] = -s;
-		}
-		if(i >= k0 + nbe){
-			bool bad = false;
-#pragma unroll
-			for(int j = 0; j < NB; j++){
-				double y = e[j];
-#pragma unroll
-				for(int m = j + 1; m < NB; m++) y += e[m] * lm[m][j];
-				bad |= !(fabs(y) <= 1.0);
-			}
-			if(bad) scal[SCAL_GJ_FLAG] = 1;
-		}
-	}
-#pragma unroll
-	for(int j = 0; j < NB; j++)
-		if(j < (int)nbe) An[(size_t)(k0 + j) * n + i] = e[j];
-}
-
-// Block step, part 2: rank-nb update of every column outside K (32 x 32 tiles, 2 x 2 per thread).
-template <int NB>
-__global__ __launch_bounds__(256) void gj_update_kernel(const double *__restrict__ A, double *__restrict__ An,
-		unsigned n, unsigned k0, unsigned nbe)
-{
-	__shared__ double Y[NB][32 + 1];      // rows K of this column tile
-	__shared__ double E[NB][32 + 1];      // transform columns of this row tile (transposed)
-	const unsigned row0 = blockIdx.x * 32, col0 = blockIdx.y * 32;
-	const int tid = threadIdx.x;
-	if(col0 >= k0 && col0 + 32 <= k0 + nbe) return;   // tile entirely inside K
-	for(int q = tid; q < NB * 32; q += 256){
-		const int kk = q / 32, cc = q % 32;
-		const unsigned col = col0 + cc;
-		Y[kk][cc] = (kk < (int)nbe && col < n) ? A[(size_t)col * n + k0 + kk] : 0.0;
-		const unsigned row = row0 + cc;
-		E[kk][cc] = (kk < (int)nbe && row < n) ? An[(size_t)(k0 + kk) * n + row] : 0.0;
-	}
-	__syncthreads();
-	const int tr = tid & 31, tc = tid >> 5;           // row tr, columns tc + 8 j
-	const unsigned row = row0 + tr;
-	const bool rowK = row >= k0 && row < k0 + nbe;
-	double acc[4];
-#pragma unroll
-	for(int j = 0; j < 4; j++){
-		const unsigned col = col0 + tc + 8 * j;
-		acc[j] = (row < n && col < n && !rowK) ? A[(size_t)col * n + row] : 0.0;
-	}
-#pragma unroll 8
-	for(int kk = 0; kk < NB; kk++){
-		const double ev = E[kk][tr];
-#pragma unroll
-		for(int j = 0; j < 4; j++) acc[j] = fmac(ev, Y[kk][tc + 8 * j], acc[j]);
-	}
-	if(row >= n) return;
-#pragma unroll
-	for(int j = 0; j < 4; j++){
-		const unsigned col = col0 + tc + 8 * j;
-		if(col >= n || (col >= k0 && col < k0 + nbe)) continue;
-		An[(size_t)col * n + row] = acc[j];
-	}
+	double r = __builtin_amdgcn_rcp(x);
+	double t = fma(-x, r, 1.0);
+	r = fma(r, t, r);
+	t = fma(-x, r, 1.0);
+	return fma(r, t, r);
 }
 
 // Fused block step: one launch per 32-column block K, grid over 64 x 64 output tiles.
-// Every workgroup inverts D = A[K,K] in LDS (the same arithmetic as gj_panel_np_kernel), forms
-// the transform rows E[i,:] of its 64 rows (D^{-1} rows inside K, -A[i,K] D^{-1} outside), and
-// writes its tile of A' = [i not in K] A + E A[K,:] (columns inside K: E itself).  Workgroups
-// of the first column block check the LU multipliers of their rows below K; workgroup (0,0)
-// records the pivots and the multipliers inside D.
-constexpr int GJ_RB = 64, GJ_CB = 64;
+//
+// D^{-1}: Gauss-Jordan on [D | 1] (32 x 64) without exchanges, two pivots (g, g+1) per step
+// eliminated together through their 2 x 2 block P = [[a_gg, a_g,g+1], [a_g+1,g, a_g+1,g+1]]:
+// pivot rows <- P^{-1} pivot rows, every other row r -= [a_rg a_r,g+1] (new pivot rows).  Wave w
+// holds rows 8w..8w+7, lane c column c (lanes 32..63 end with D^{-1}); one barrier per step
+// publishes the two pivot rows and the two pivot columns.  The pivot columns of every step are
+// kept, and the LU multipliers of D (and the pivots U_gg) are formed from them afterwards:
+// l[r][g] = a_rg / a_gg, U_hh = a_hh - l[h][g] a_gh, l[r][h] = (a_rh - l[r][g] a_gh) / U_hh.
+//
+// The tile operands (A[K, column block], A[row block, K], the tile itself) are loaded into
+// registers before the inversion and stored to LDS after it.  The transform rows are kept
+// transposed (Et[m][row]); a thread forms a 4 x 2 block of them and updates a 4 x 4 block of the
+// tile, reading two rows of four from LDS per m.  Workgroups of the first column block check
+// the LU multipliers of their rows below K (l = -E L, L unit lower triangular, |l| <= 1);
+// workgroup (0,0) records the pivots.
+constexpr int GJ_RB = 64, GJ_CB = 64, GJ_LD = 68;
 
 __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__ A, double *__restrict__ An,
 		unsigned n, unsigned k0, unsigned nbe, double *__restrict__ diag, int *__restrict__ piv, double *__restrict__ scal)
 {
 	constexpr int NB = GJ_NB;
-	__shared__ double dinv[NB][NB + 1];
-	__shared__ double lm[NB][NB + 1];
-	__shared__ double colb[2][NB];
-	__shared__ double Es[GJ_RB][NB + 1];
-	__shared__ double Ys[NB][GJ_CB + 1];
-	__shared__ double Ar[GJ_RB][NB + 1];     // A[row block, K]
-	const int tid = threadIdx.x;
+	__shared__ __align__(16) double prow[2][2][2 * NB];     // pivot rows of the step, by step parity
+	__shared__ __align__(16) double pcs[NB / 2][2][NB];     // pivot columns of every step
+	__shared__ __align__(16) double dinv[NB][NB + 2];
+	__shared__ double lm[NB][NB + 1];                       // unit lower triangular L of D (zero above)
+	__shared__ __align__(16) double Art[NB][GJ_LD];         // A[row block, K], transposed
+	__shared__ __align__(16) double Et[NB][GJ_LD];          // transform rows, transposed
+	__shared__ __align__(16) double Ys[NB][GJ_LD];          // A[K, column block]
+	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 	const unsigned r0 = blockIdx.x * GJ_RB, c0 = blockIdx.y * GJ_CB;
 	const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
-	const int wave = tid >> 6, lane = tid & 63;
-	if(wave == 0){
-		// ---- D^{-1} by one wave, no barriers: lane c holds column c of [D | 1] (lanes 0..31 D,
-		// 32..63 the identity); step g broadcasts the pivot column through LDS (colb).  After
-		// the last step lanes 32..63 hold the columns of D^{-1}.
-		double col[NB];
+	// ---- loads
+	double a[8];
 #pragma unroll
-		for(int r = 0; r < NB; r++){
-			if(lane < NB) col[r] = (r < (int)nbe && lane < (int)nbe) ? A[(size_t)(k0 + lane) * n + k0 + r] : (r == lane ? 1.0 : 0.0);
-			else col[r] = (r == lane - NB) ? 1.0 : 0.0;
-			if(lane < NB) lm[r][lane] = 0;
+	for(int q = 0; q < 8; q++){
+		const int r = 8 * wave + q;
+		if(lane < NB) a[q] = (r < (int)nbe && lane < (int)nbe) ? A[(size_t)(k0 + lane) * n + k0 + r] : (r == lane ? 1.0 : 0.0);
+		else a[q] = r == lane - NB ? 1.0 : 0.0;
+	}
+	double ysv[NB * GJ_CB / 256], arv[GJ_RB * NB / 256];
+#pragma unroll
+	for(int j = 0; j < NB * GJ_CB / 256; j++){
+		const int e = tid + 256 * j, m = e % NB, cc = e / NB;
+		const unsigned col = c0 + cc;
+		ysv[j] = (m < (int)nbe && col < n) ? A[(size_t)col * n + k0 + m] : 0.0;
+	}
+#pragma unroll
+	for(int j = 0; j < GJ_RB * NB / 256; j++){
+		const int e = tid + 256 * j, rr = e % GJ_RB, m = e / GJ_RB;
+		const unsigned i = r0 + rr;
+		arv[j] = (m < (int)nbe && i < n) ? A[(size_t)(k0 + m) * n + i] : 0.0;
+	}
+	// this thread's 4 x 4 tile block: rows 4 tr + i, columns 4 tc + j
+	const int tr = tid & 15, tc = tid >> 4;
+	double acc[4][4];
+#pragma unroll
+	for(int i = 0; i < 4; i++){
+		const unsigned row = r0 + 4 * tr + i;
+		const bool rowK = row >= k0 && row < k0 + nbe;
+#pragma unroll
+		for(int j = 0; j < 4; j++){
+			const unsigned col = c0 + 4 * tc + j;
+			acc[i][j] = (row < n && col < n && !rowK) ? A[(size_t)col * n + row] : 0.0;
 		}
+	}
+#pragma unroll
+	for(int e = tid; e < NB * (NB + 1); e += 256) lm[e / (NB + 1)][e % (NB + 1)] = (e / (NB + 1) == e % (NB + 1)) ? 1.0 : 0.0;
+	// ---- D^{-1}, two pivots per step (steps past nbe are identity steps and skipped)
+#pragma unroll
+	for(int g = 0; g < NB; g += 2){
+		if(g >= (int)nbe) break;
+		const int h = g + 1, b = (g >> 1) & 1, gq = g & 7;
+		if(wave == (g >> 3)){
+			prow[b][0][lane] = a[gq];
+			prow[b][1][lane] = a[gq + 1];
+		}
+		if(lane == g || lane == h){
+			double *dst = &pcs[g >> 1][lane - g][8 * wave];
+#pragma unroll
+			for(int q = 0; q < 8; q += 2) *(double2 *)(dst + q) = make_double2(a[q], a[q + 1]);
+		}
+		__syncthreads();
+		const double *pg = pcs[g >> 1][0], *ph = pcs[g >> 1][1];
+		const double pgg = pg[g], phg = pg[h], pgh = ph[g], phh = ph[h];
+		const double idet = rcp_nr(fmac(pgg, phh, -pgh * phg));
+		const double u = prow[b][0][lane], v = prow[b][1][lane];
+		const double u2 = fmac(phh, u, -pgh * v) * idet, v2 = fmac(pgg, v, -phg * u) * idet;
+		double cg[8], ch[8];
+#pragma unroll
+		for(int q = 0; q < 8; q += 2){
+			const double2 x = *(const double2 *)(pg + 8 * wave + q), y = *(const double2 *)(ph + 8 * wave + q);
+			cg[q] = x.x;
+			cg[q + 1] = x.y;
+			ch[q] = y.x;
+			ch[q + 1] = y.y;
+		}
+#pragma unroll
+		for(int q = 0; q < 8; q++){
+			const int r = 8 * wave + q;
+			a[q] = r == g ? u2 : (r == h ? v2 : fmac(-ch[q], v2, fmac(-cg[q], u2, a[q])));
+		}
+	}
+	if(lane >= NB){
+#pragma unroll
+		for(int q = 0; q < 8; q++) dinv[8 * wave + q][lane - NB] = a[q];
+	}
+#pragma unroll
+	for(int j = 0; j < NB * GJ_CB / 256; j++){
+		const int e = tid + 256 * j;
+		Ys[e % NB][e / NB] = ysv[j];
+	}
+#pragma unroll
+	for(int j = 0; j < GJ_RB * NB / 256; j++){
+		const int e = tid + 256 * j;
+		Art[e / GJ_RB][e % GJ_RB] = arv[j];
+	}
+	// LU multipliers and pivots of D from the kept pivot columns: thread (step s, row r)
+	{
 		bool bad = false;
 #pragma unroll
-		for(int g = 0; g < NB; g++){
+		for(int j = 0; j < 2; j++){
+			const int e = tid + 256 * j, st = e / NB, r = e % NB, g = 2 * st, h = g + 1;
 			if(g < (int)nbe){
-				double *pc = colb[g & 1];
-				if(lane == g){
-#pragma unroll
-					for(int r = 0; r < NB; r++) pc[r] = col[r];
-				}
-				__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-				__builtin_amdgcn_wave_barrier();
-				const double d = pc[g];
-				const double inv = 1.0 / d;
-				const double pr = col[g];
-#pragma unroll
-				for(int r = 0; r < NB; r++){
-					if(r == g) col[r] = pr * inv;
-					else{
-						const double fi = pc[r] * inv;
-						col[r] = fmac(-fi, pr, col[r]);
-						if(r > g){
-							if(lane == 0) lm[r][g] = fi;
-							bad |= !(fabs(fi) <= 1.0);
-						}
+				const double *pg = pcs[st][0], *ph = pcs[st][1];
+				const double pgg = pg[g], lhg = pg[h] / pgg, uhh = ph[h] - lhg * ph[g];
+				if(r > g && r < (int)nbe){
+					const double lg = pg[r] / pgg;
+					lm[r][g] = lg;
+					bad |= !(fabs(lg) <= 1.0);
+					if(r > h){
+						const double lh = (ph[r] - lg * ph[g]) / uhh;
+						lm[r][h] = lh;
+						bad |= !(fabs(lh) <= 1.0);
 					}
 				}
-				if(lane == 0 && lead){ diag[k0 + g] = d; piv[k0 + g] = (int)(k0 + g); }
-				__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-				__builtin_amdgcn_wave_barrier();
+				if(lead && r == 0){
+					diag[k0 + g] = pgg;
+					piv[k0 + g] = (int)(k0 + g);
+					if(h < (int)nbe){
+						diag[k0 + h] = uhh;
+						piv[k0 + h] = (int)(k0 + h);
+					}
+				}
 			}
 		}
-		if(lane >= NB){
-#pragma unroll
-			for(int r = 0; r < NB; r++) dinv[r][lane - NB] = col[r];
-		}
-		if(bad && lead && lane == 0) scal[SCAL_GJ_FLAG] = 1;
-	}
-	// ---- loads that do not depend on D^{-1}, issued by waves 1..3 while wave 0 inverts:
-	// A[K, column block] (old values; the output goes to the other buffer) and A[row block, K]
-	for(int e = tid - 64; e >= 0 && e < NB * GJ_CB; e += 192){
-		const int m = e % NB, cc = e / NB;
-		const unsigned col = c0 + cc;
-		Ys[m][cc] = (m < (int)nbe && col < n) ? A[(size_t)col * n + k0 + m] : 0.0;
-	}
-	for(int e = tid - 64; e >= 0 && e < GJ_RB * NB; e += 192){
-		const int rr = e % GJ_RB, m = e / GJ_RB;
-		const unsigned i = r0 + rr;
-		Ar[rr][m] = (m < (int)nbe && i < n) ? A[(size_t)(k0 + m) * n + i] : 0.0;
-	}
-	// this thread's tile values: row tid % 64, columns (tid / 64) + 4 j
-	const int trr = tid & 63, cbase = tid >> 6;
-	const unsigned ti = r0 + trr;
-	const bool rowK = ti >= k0 && ti < k0 + nbe;
-	double av[GJ_CB / 4];
-#pragma unroll
-	for(int j = 0; j < GJ_CB / 4; j++){
-		const unsigned col = c0 + cbase + 4 * j;
-		av[j] = (ti < n && col < n && !rowK) ? A[(size_t)col * n + ti] : 0.0;
+		if(bad && lead) scal[SCAL_GJ_FLAG] = 1;
 	}
 	__syncthreads();
-	// ---- transform rows of this row block: thread (row tid/4, 8 columns)
+	// ---- transform rows: thread (rows 4 rq + i, columns 2 cq + jj) of E = -A[rows, K] D^{-1}
+	// (rows inside K: the rows of D^{-1})
 	{
-		const int rr = tid >> 2, j0 = (tid & 3) * 8;
-		const unsigned i = r0 + rr;
-		double e8[8];
-		if(i < n && i >= k0 && i < k0 + nbe){
-#pragma unroll
-			for(int j = 0; j < 8; j++) e8[j] = dinv[i - k0][j0 + j];
-		}else{
-#pragma unroll
-			for(int j = 0; j < 8; j++) e8[j] = 0;
+		const int rq = tid & 15, cq = tid >> 4;
+		double e[4][2] = {};
 #pragma unroll 8
-			for(int m = 0; m < NB; m++){
-				const double a = Ar[rr][m];
+		for(int m = 0; m < NB; m++){
+			const double2 x0 = *(const double2 *)&Art[m][4 * rq], x1 = *(const double2 *)&Art[m][4 * rq + 2];
+			const double2 dv = *(const double2 *)&dinv[m][2 * cq];
+			const double xr[4] = {x0.x, x0.y, x1.x, x1.y};
 #pragma unroll
-				for(int j = 0; j < 8; j++) e8[j] = fmac(a, dinv[m][j0 + j], e8[j]);
+			for(int i = 0; i < 4; i++){
+				e[i][0] = fmac(xr[i], dv.x, e[i][0]);
+				e[i][1] = fmac(xr[i], dv.y, e[i][1]);
 			}
-#pragma unroll
-			for(int j = 0; j < 8; j++) e8[j] = -e8[j];
 		}
 #pragma unroll
-		for(int j = 0; j < 8; j++) Es[rr][j0 + j] = e8[j];
+		for(int i = 0; i < 4; i++){
+			const unsigned row = r0 + 4 * rq + i;
+			const bool inK = row >= k0 && row < k0 + nbe;
+#pragma unroll
+			for(int jj = 0; jj < 2; jj++)
+				Et[2 * cq + jj][4 * rq + i] = inK ? dinv[row - k0][2 * cq + jj] : -e[i][jj];
+		}
 	}
 	__syncthreads();
-	// LU multipliers of the rows below K: l = -E L (first column block only)
-	if(blockIdx.y == 0 && tid < GJ_RB){
-		const unsigned i = r0 + tid;
+	// LU multipliers of the rows below K (first column block only): thread (row, columns
+	// jg + 4 t), y_j = sum_m E[row][m] L[m][j]
+	if(blockIdx.y == 0){
+		const int row = tid & 63, jg = tid >> 6;
+		const unsigned i = r0 + row;
 		if(i < n && i >= k0 + nbe){
+			double er[NB];
+#pragma unroll
+			for(int m = 0; m < NB; m++) er[m] = Et[m][row];
 			bool bad = false;
-			for(int j = 0; j < (int)nbe; j++){
-				double y = Es[tid][j];
-				for(int m = j + 1; m < (int)nbe; m++) y += Es[tid][m] * lm[m][j];
-				bad |= !(fabs(y) <= 1.0);
+#pragma unroll
+			for(int t = 0; t < NB / 4; t++){
+				const int j = jg + 4 * t;
+				double y = 0;
+#pragma unroll
+				for(int m = 0; m < NB; m++) y = fmac(er[m], lm[m][j], y);
+				bad |= j < (int)nbe && !(fabs(y) <= 1.0);
 			}
 			if(bad) scal[SCAL_GJ_FLAG] = 1;
 		}
 	}
-	// ---- tile update
-	if(ti >= n) return;
-	double er[NB];
+	// ---- tile update: A'[rows, cols] = acc + E[rows, :] A[K, cols]; columns inside K take E
+#pragma unroll 8
+	for(int m = 0; m < NB; m++){
+		const double2 e0 = *(const double2 *)&Et[m][4 * tr], e1 = *(const double2 *)&Et[m][4 * tr + 2];
+		const double2 y0 = *(const double2 *)&Ys[m][4 * tc], y1 = *(const double2 *)&Ys[m][4 * tc + 2];
+		const double ev[4] = {e0.x, e0.y, e1.x, e1.y}, yv[4] = {y0.x, y0.y, y1.x, y1.y};
 #pragma unroll
-	for(int m = 0; m < NB; m++) er[m] = Es[trr][m];
+		for(int i = 0; i < 4; i++)
 #pragma unroll
-	for(int j = 0; j < GJ_CB / 4; j++){
-		const int cc = cbase + 4 * j;
-		const unsigned col = c0 + cc;
+			for(int j = 0; j < 4; j++) acc[i][j] = fmac(ev[i], yv[j], acc[i][j]);
+	}
+#pragma unroll
+	for(int j = 0; j < 4; j++){
+		const unsigned col = c0 + 4 * tc + j;
 		if(col >= n) continue;
-		double out;
-		if(col >= k0 && col < k0 + nbe) out = Es[trr][col - k0];
-		else{
-			double acc = av[j];
+		const bool colK = col >= k0 && col < k0 + nbe;
 #pragma unroll
-			for(int m = 0; m < NB; m++) acc = fmac(er[m], Ys[m][cc], acc);
-			out = acc;
+		for(int i = 0; i < 4; i++){
+			const unsigned row = r0 + 4 * tr + i;
+			if(row < n) An[(size_t)col * n + row] = colK ? Et[col - k0][4 * tr + i] : acc[i][j];
 		}
-		An[(size_t)col * n + ti] = out;
 	}
 }
 
@@ -436,22 +379,12 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M)
 	const double *cur = M;
 	double *buf[2] = {c->d_gj[0], c->d_gj[1]};
 	int which = 0;
-	const dim3 ugrid((n + 31) / 32, (n + 31) / 32);
-	const bool fused = !(getenv("EPHMC_GJ_SPLIT") && atoi(getenv("EPHMC_GJ_SPLIT")) != 0);
 	const dim3 sgrid((n + GJ_RB - 1) / GJ_RB, (n + GJ_CB - 1) / GJ_CB);
 	for(unsigned k0 = 0; k0 < n; k0 += GJ_NB){
 		const unsigned nbe = std::min<unsigned>(GJ_NB, n - k0);
 		double *next = buf[which];
-		if(fused){
-			gj_step_kernel<<<sgrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe, c->d_diag, c->d_piv, c->d_scal);
-			EPHMC_LAUNCHED();
-		}else{
-			gj_panel_np_kernel<GJ_NB><<<(n + 255) / 256, 256, 0, c->stream>>>(cur, next, n, k0, nbe, c->d_diag, c->d_piv,
-				c->d_scal);
-			EPHMC_LAUNCHED();
-			gj_update_kernel<GJ_NB><<<ugrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe);
-			EPHMC_LAUNCHED();
-		}
+		gj_step_kernel<<<sgrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe, c->d_diag, c->d_piv, c->d_scal);
+		EPHMC_LAUNCHED();
 		cur = next;
 		which ^= 1;
 	}

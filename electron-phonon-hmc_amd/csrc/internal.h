@@ -73,13 +73,14 @@ struct TlWork {
 	unsigned cap_b = 0, cap_nbins = 0;
 	size_t rot_cap = 0;               // rotations recorded per sample
 	unsigned sw_cap = 0;              // sweeps recorded per sample
-	double *A = nullptr, *Z = nullptr, *W0 = nullptr, *W1 = nullptr, *V = nullptr;   // [B][N][N]
-	double2 *rot = nullptr;           // [B][rot_cap] (c, s)
+	double *A = nullptr, *Z = nullptr;   // [B][N][N]; the sorted eigenvectors V reuse A
+	double2 *rot = nullptr;           // [B][rot_cap] (c, s); W_0, W_1 ([B][N][N] each) reuse it
+	double *V = nullptr, *W0 = nullptr, *W1 = nullptr;
 	int4 *sw = nullptr;               // [B][sw_cap] (first rotation, lowest, highest position)
 	unsigned *nsw = nullptr;
 	int *status = nullptr, *perm = nullptr;
 	double *x = nullptr, *d = nullptr, *e = nullptr, *tau = nullptr, *ev = nullptr, *evs = nullptr, *ems = nullptr;
-	double *res = nullptr, *part = nullptr, *acc = nullptr;
+	double *res = nullptr, *part = nullptr, *acc = nullptr, *red = nullptr;
 };
 
 } // namespace ephmc

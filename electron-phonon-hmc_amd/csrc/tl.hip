@@ -17,11 +17,12 @@
 //                   matrix-vector product of step j+1, so the trailing block is read and written
 //                   once per step.  Reflectors v_j overwrite column j below the diagonal.
 //   tl_ql_scalar    implicit QL with Wilkinson shifts on (d, e) (the scalar recurrence of the
-//                   EISPACK tql2 method, one lane per sample), recording every rotation (c, s)
-//                   and the position range of every sweep
+//                   EISPACK tql2 method, one wave per sample computing it in every lane),
+//                   recording every rotation (c, s) and the position range of every sweep
 //   tl_ql_apply     the recorded rotations applied to Z = I, thread = row of Z: G sweeps per
 //                   pass with a register window of G+1 columns (a wavefront: sweep g runs one
-//                   position behind sweep g-1), so Z is read and written once per G sweeps
+//                   position behind sweep g-1), so Z is read and written once per G sweeps;
+//                   columns are loaded several steps ahead
 //   tl_back         V = H_0 H_1 ... H_{N-3} Z: a wave holds CW columns of Z in registers and
 //                   applies every reflector (last first)
 //   tl_sort         eigenvalue ranks, e^{-E}, the partition sum
@@ -41,6 +42,7 @@ namespace ephmc {
 
 constexpr int TL_BT = 1024;     // tridiagonalisation / sort workgroup
 constexpr int TL_G = 16;        // sweeps per wavefront pass
+constexpr int TL_PF = 8;        // columns loaded ahead by the rotation pass
 constexpr int TL_TILE = 64;
 constexpr int TL_KC = 16;       // rows of V / W per LDS stage of the pair tiles
 constexpr unsigned TL_NONE = 0xffffffffu;
@@ -253,33 +255,58 @@ __global__ __launch_bounds__(BT) void tl_tridiag_kernel(double *__restrict__ Aal
 // e[i] couples i and i+1.  Rotation (c, s) at position i acts on columns (i, i+1) of Z:
 // Z(:,i+1) <- s Z(:,i) + c Z(:,i+1),  Z(:,i) <- c Z(:,i) - s Z(:,i+1).  Sweep records: (first
 // rotation index, lowest position, highest position); positions run from high to low.
+// One wave per sample runs the recurrence with every lane computing the same values (d, e in
+// LDS, written by all lanes alike); the deflation test scans 64 positions per step.
+
+// 1/x: hardware reciprocal refined by two Newton steps
+__device__ inline double rcp_nr(double x)
+{
+	double r = __builtin_amdgcn_rcp(x);
+	double t = fma(-x, r, 1.0);
+	r = fma(r, t, r);
+	t = fma(-x, r, 1.0);
+	return fma(r, t, r);
+}
+
+// sqrt(f^2 + g^2): the plain form while the square sum is a normal number, hypot otherwise
+__device__ inline double norm2(double f, double g)
+{
+	const double h2 = fma(f, f, g * g);
+	if(h2 > 0x1p-1000 && h2 < 0x1p1000) return sqrt(h2);
+	return hypot(f, g);
+}
+
 __global__ __launch_bounds__(64) void tl_ql_scalar_kernel(const double *__restrict__ dall, const double *__restrict__ eall,
 		unsigned n, double *__restrict__ evall, double2 *__restrict__ rotall, size_t rot_cap, int4 *__restrict__ swall,
 		unsigned sw_cap, unsigned *__restrict__ nsw_all, int *__restrict__ status)
 {
 	extern __shared__ double sh[];
 	double *d = sh, *e = sh + n;
-	const unsigned s = blockIdx.x;
-	for(unsigned i = threadIdx.x; i < n; i += 64){
+	const unsigned s = blockIdx.x, lane = threadIdx.x;
+	for(unsigned i = lane; i < n; i += 64){
 		d[i] = dall[(size_t)s * n + i];
-		e[i] = eall[(size_t)s * n + i];
+		e[i] = i + 1 < n ? eall[(size_t)s * n + i] : 0.0;
 	}
 	__syncthreads();
-	if(threadIdx.x != 0) return;
 	double2 *rot = rotall + (size_t)s * rot_cap;
 	int4 *sw = swall + (size_t)s * sw_cap;
 	const double eps = 2.220446049250313e-16;
 	size_t nrot = 0;
 	unsigned nsw = 0;
 	int st = 0;
-	e[n - 1] = 0;
 	for(unsigned l = 0; l < n && !st; l++){
 		int it = 0;
 		for(;;){
-			unsigned m = l;
-			for(; m + 1 < n; m++){
-				const double dd = fabs(d[m]) + fabs(d[m + 1]);
-				if(fabs(e[m]) <= eps * dd) break;
+			// first m >= l with |e[m]| <= eps (|d[m]| + |d[m+1]|), else n - 1
+			unsigned m = n - 1;
+			for(unsigned m0 = l; m0 + 1 < n; m0 += 64){
+				const unsigned mm = m0 + lane;
+				const bool hit = mm + 1 < n && fabs(e[mm]) <= eps * (fabs(d[mm]) + fabs(d[mm + 1]));
+				const unsigned long long bal = __ballot(hit);
+				if(bal){
+					m = m0 + (unsigned)__ffsll(bal) - 1;
+					break;
+				}
 			}
 			if(m == l) break;
 			if(++it > 60){ st = 1; break; }
@@ -291,40 +318,59 @@ __global__ __launch_bounds__(64) void tl_ql_scalar_kernel(const double *__restri
 			const size_t base = nrot;
 			int i = (int)m - 1;
 			bool broke = false;
+			// d[i+1], d[i], e[i] are not yet modified by this sweep when position i is reached;
+			// position i-1's values are loaded one step ahead
+			double di1 = d[m], di = d[i], ei = e[i];
 			for(; i >= (int)l; i--){
-				const double f = sn * e[i], b = cs * e[i];
-				r = hypot(f, g);
+				const int ip = i > 0 ? i - 1 : 0;
+				const double dn = d[ip], en = e[ip];
+				const double f = sn * ei, b = cs * ei;
+				r = norm2(f, g);
 				e[i + 1] = r;
 				if(r == 0){
-					d[i + 1] -= p;
+					d[i + 1] = di1 - p;
 					e[m] = 0;
 					broke = true;
 					break;
 				}
-				sn = f / r;
-				cs = g / r;
-				g = d[i + 1] - p;
-				r = (d[i] - g) * sn + 2 * cs * b;
+				const double ir = rcp_nr(r);
+				sn = f * ir;
+				cs = g * ir;
+				g = di1 - p;
+				r = (di - g) * sn + 2 * cs * b;
 				p = sn * r;
 				d[i + 1] = g + p;
 				g = cs * r - b;
-				rot[nrot++] = make_double2(cs, sn);
+				if(lane == 0) rot[nrot] = make_double2(cs, sn);
+				nrot++;
+				di1 = di;
+				di = dn;
+				ei = en;
 			}
 			const int hi = (int)m - 1, lo = broke ? i + 1 : (int)l;
-			if(lo <= hi) sw[nsw++] = make_int4((int)base, lo, hi, 0);
+			if(lo <= hi){
+				if(lane == 0) sw[nsw] = make_int4((int)base, lo, hi, 0);
+				nsw++;
+			}
 			if(broke) continue;
 			d[l] -= p;
 			e[l] = g;
 			e[m] = 0;
 		}
 	}
-	for(unsigned i = 0; i < n; i++) evall[(size_t)s * n + i] = d[i];
-	nsw_all[s] = nsw;
-	status[s] = st;
+	for(unsigned i = lane; i < n; i += 64) evall[(size_t)s * n + i] = d[i];
+	if(lane == 0){
+		nsw_all[s] = nsw;
+		status[s] = st;
+	}
 }
 
 // ---- rotations applied to Z = I (thread = row, column-major Z) ------------------------------
-template <int G>
+// G sweeps per pass: sweep k acts at position q + k while the window steps q downwards, so the
+// pass holds G+1 columns in registers.  Column q-1-j is loaded PF steps ahead and the G
+// rotations of step q-1 are loaded during step q; a sweep outside its range at a step gets the
+// identity rotation (c, s) = (1, 0), which leaves both columns unchanged.
+template <int G, int PF>
 __global__ __launch_bounds__(256) void tl_ql_apply_kernel(double *__restrict__ Zall, unsigned n, const double2 *__restrict__ rotall,
 		size_t rot_cap, const int4 *__restrict__ swall, unsigned sw_cap, const unsigned *__restrict__ nsw_all)
 {
@@ -355,23 +401,36 @@ __global__ __launch_bounds__(256) void tl_ql_apply_kernel(double *__restrict__ Z
 				hi[k] = 0;
 			}
 		}
-		double W[G + 1];
+		auto fetch = [&](int q, double2 *cs) {
+#pragma unroll
+			for(int k = 0; k < G; k++){
+				const int pos = q + k;
+				const bool in = pos >= lo[k] && pos <= hi[k];
+				const double2 v = rot[in ? base[k] + (hi[k] - pos) : 0];
+				cs[k] = in ? v : make_double2(1.0, 0.0);
+			}
+		};
+		double W[G + 1], pf[PF];
 #pragma unroll
 		for(int k = 0; k <= G; k++){
 			const int col = P + k;
 			W[k] = (act && col < (int)n) ? Z[(size_t)col * n + r] : 0.0;
 		}
+#pragma unroll
+		for(int j = 0; j < PF; j++){
+			const int col = P - 1 - j;
+			pf[j] = (act && col >= 0) ? Z[(size_t)col * n + r] : 0.0;
+		}
+		double2 cs[G], csn[G];
+		fetch(P, cs);
 		int q = P;
 		for(;;){
+			if(q > qend) fetch(q - 1, csn);
 #pragma unroll
 			for(int k = 0; k < G; k++){
-				const int pos = q + k;
-				if(pos >= lo[k] && pos <= hi[k]){
-					const double2 cs = rot[base[k] + (hi[k] - pos)];
-					const double a = W[k], f = W[k + 1];
-					W[k + 1] = cs.y * a + cs.x * f;
-					W[k] = cs.x * a - cs.y * f;
-				}
+				const double a = W[k], f = W[k + 1];
+				W[k + 1] = cs[k].y * a + cs[k].x * f;
+				W[k] = cs[k].x * a - cs[k].y * f;
 			}
 			if(q <= qend) break;
 			const int col = q + G;
@@ -379,7 +438,13 @@ __global__ __launch_bounds__(256) void tl_ql_apply_kernel(double *__restrict__ Z
 #pragma unroll
 			for(int k = G; k > 0; k--) W[k] = W[k - 1];
 			q--;
-			W[0] = (act && q >= 0) ? Z[(size_t)q * n + r] : 0.0;
+			W[0] = pf[0];
+#pragma unroll
+			for(int j = 0; j + 1 < PF; j++) pf[j] = pf[j + 1];
+			const int pcol = q - PF;
+			pf[PF - 1] = (act && pcol >= 0) ? Z[(size_t)pcol * n + r] : 0.0;
+#pragma unroll
+			for(int k = 0; k < G; k++) cs[k] = csn[k];
 		}
 #pragma unroll
 		for(int k = 0; k <= G; k++){
@@ -607,12 +672,32 @@ __global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict_
 	}
 }
 
-__global__ void tl_reduce_kernel(const double *__restrict__ part, unsigned nparts, unsigned nbins, double *__restrict__ acc)
+// Fixed-order sum of the tile histograms in two stages: chunk c of TL_RC tiles is summed by one
+// workgroup (wave w takes tiles w, w+4, ... of the chunk for 64 bins, the four wave sums are
+// added in wave order), then every bin adds its chunk sums in chunk order.
+constexpr unsigned TL_RC = 512;
+
+__global__ __launch_bounds__(256) void tl_reduce_chunks_kernel(const double *__restrict__ part, unsigned nparts, unsigned nbins,
+		double *__restrict__ red)
+{
+	__shared__ double ws[4][64];
+	const unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const unsigned b = blockIdx.x * 64 + lane, c = blockIdx.y;
+	const unsigned t0 = c * TL_RC, t1 = min(t0 + TL_RC, nparts);
+	double s = 0;
+	if(b < nbins)
+		for(unsigned t = t0 + w; t < t1; t += 4) s += part[(size_t)t * nbins + b];
+	ws[w][lane] = s;
+	__syncthreads();
+	if(w == 0 && b < nbins) red[(size_t)c * nbins + b] = ((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane];
+}
+
+__global__ void tl_reduce_kernel(const double *__restrict__ red, unsigned nchunks, unsigned nbins, double *__restrict__ acc)
 {
 	const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
 	if(b >= nbins) return;
 	double s = 0;
-	for(unsigned t = 0; t < nparts; t++) s += part[(size_t)t * nbins + b];
+	for(unsigned c = 0; c < nchunks; c++) s += red[(size_t)c * nbins + b];
 	acc[b] += s;
 }
 
@@ -641,8 +726,7 @@ static int tl_reserve(ephmc_ctx *c, unsigned B, unsigned nbins)
 		return EPHMC_OK;
 	};
 	int rc;
-	if((rc = al((void **)&w.A, B * mat * 8)) || (rc = al((void **)&w.Z, B * mat * 8)) || (rc = al((void **)&w.W0, B * mat * 8))
-		|| (rc = al((void **)&w.W1, B * mat * 8)) || (rc = al((void **)&w.V, B * mat * 8))
+	if((rc = al((void **)&w.A, B * mat * 8)) || (rc = al((void **)&w.Z, B * mat * 8))
 		|| (rc = al((void **)&w.rot, B * w.rot_cap * sizeof(double2))) || (rc = al((void **)&w.sw, B * (size_t)w.sw_cap * sizeof(int4)))
 		|| (rc = al((void **)&w.nsw, B * sizeof(unsigned))) || (rc = al((void **)&w.status, B * sizeof(int)))
 		|| (rc = al((void **)&w.x, B * 3 * (size_t)n * 8)) || (rc = al((void **)&w.d, B * (size_t)n * 8))
@@ -650,10 +734,16 @@ static int tl_reserve(ephmc_ctx *c, unsigned B, unsigned nbins)
 		|| (rc = al((void **)&w.ev, B * (size_t)n * 8)) || (rc = al((void **)&w.evs, B * (size_t)n * 8))
 		|| (rc = al((void **)&w.ems, B * (size_t)n * 8)) || (rc = al((void **)&w.perm, B * (size_t)n * sizeof(int)))
 		|| (rc = al((void **)&w.res, B * 8)) || (rc = al((void **)&w.part, B * (size_t)nblk * nblk * nbins * 8))
-		|| (rc = al((void **)&w.acc, (size_t)nbins * 8))){
+		|| (rc = al((void **)&w.acc, (size_t)nbins * 8))
+		|| (rc = al((void **)&w.red, ((B * (size_t)nblk * nblk + TL_RC - 1) / TL_RC) * nbins * 8))){
 		tl_release(c);
 		return rc;
 	}
+	// A is free once the back-transformation has run, the rotations once they are applied
+	// (2 N^2 doubles fit in 1.5 N^2 double pairs)
+	w.V = w.A;
+	w.W0 = (double *)w.rot;
+	w.W1 = (double *)w.rot + B * mat;
 	w.cap_b = B;
 	w.cap_nbins = nbins;
 	return EPHMC_OK;
@@ -662,7 +752,7 @@ static int tl_reserve(ephmc_ctx *c, unsigned B, unsigned nbins)
 void tl_release(ephmc_ctx *c)
 {
 	TlWork &w = c->tl;
-	void *bufs[] = {w.A, w.Z, w.W0, w.W1, w.V, w.rot, w.sw, w.nsw, w.status, w.x, w.d, w.e, w.tau, w.ev, w.evs, w.ems, w.perm,
+	void *bufs[] = {w.A, w.Z, w.rot, w.red, w.sw, w.nsw, w.status, w.x, w.d, w.e, w.tau, w.ev, w.evs, w.ems, w.perm,
 		w.res, w.part, w.acc};
 	for(void *b : bufs) if(b) (void)hipFree(b);
 	w = TlWork();
@@ -683,10 +773,11 @@ extern "C" int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamp
 	if(n > 2 * TL_BT){ set_error("transient localisation supports N <= %d sites", 2 * TL_BT); return EPHMC_ERR_ARG; }
 	const size_t mat = (size_t)n * n;
 	const unsigned nblk = (n + TL_TILE - 1) / TL_TILE;
-	// samples per batch: about 8 GB of workspace (8.5 N^2 doubles per sample), at most 256
-	const size_t per = (size_t)(8.5 * mat * 8) + (size_t)nblk * nblk * nbins * 8;
-	unsigned B = (unsigned)std::max<size_t>(1, ((size_t)8 << 30) / per);
-	B = std::min(B, 256u);
+	// samples per batch: about 24 GB of workspace (5.5 N^2 doubles per sample), at most 512, so
+	// that the one-wave-per-sample QL recurrence has many samples in flight
+	const size_t per = (size_t)(5.5 * mat * 8) + (size_t)nblk * nblk * nbins * 8;
+	unsigned B = (unsigned)std::max<size_t>(1, ((size_t)24 << 30) / per);
+	B = std::min(B, 512u);
 	B = std::min(B, nsamples);
 	EPHMC_TRY(tl_reserve(c, B, nbins));
 	TlWork &w = c->tl;
@@ -715,7 +806,7 @@ extern "C" int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamp
 		tl_ql_scalar_kernel<<<nb_, 64, 2 * (size_t)n * 8, c->stream>>>(w.d, w.e, n, w.ev, w.rot, w.rot_cap, w.sw, w.sw_cap, w.nsw,
 			w.status);
 		EPHMC_LAUNCHED();
-		tl_ql_apply_kernel<TL_G><<<dim3((n + 255) / 256, nb_), 256, 0, c->stream>>>(w.Z, n, w.rot, w.rot_cap, w.sw, w.sw_cap, w.nsw);
+		tl_ql_apply_kernel<TL_G, TL_PF><<<dim3((n + 255) / 256, nb_), 256, 0, c->stream>>>(w.Z, n, w.rot, w.rot_cap, w.sw, w.sw_cap, w.nsw);
 		EPHMC_LAUNCHED();
 		EPHMC_TRY(timer_end(c, "tl_ql"));
 		EPHMC_TRY(timer_begin(c, "tl_back"));
@@ -732,8 +823,13 @@ extern "C" int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamp
 		EPHMC_TRY(timer_begin(c, "tl_pairs"));
 		tl_pairs_kernel<<<dim3(nblk, nblk, nb_), 256, 0, c->stream>>>(w.V, w.W0, w.W1, w.evs, w.ems, n, width, nbins, w.part);
 		EPHMC_LAUNCHED();
-		tl_reduce_kernel<<<(nbins + 255) / 256, 256, 0, c->stream>>>(w.part, nb_ * nblk * nblk, nbins, w.acc);
-		EPHMC_LAUNCHED();
+		{
+			const unsigned nparts = nb_ * nblk * nblk, nch = (nparts + TL_RC - 1) / TL_RC;
+			tl_reduce_chunks_kernel<<<dim3((nbins + 63) / 64, nch), 256, 0, c->stream>>>(w.part, nparts, nbins, w.red);
+			EPHMC_LAUNCHED();
+			tl_reduce_kernel<<<(nbins + 255) / 256, 256, 0, c->stream>>>(w.red, nch, nbins, w.acc);
+			EPHMC_LAUNCHED();
+		}
 		EPHMC_TRY(timer_end(c, "tl_pairs"));
 		EPHMC_HIP(hipMemcpyAsync(results + s0, w.res, (size_t)nb_ * 8, hipMemcpyDeviceToHost, c->stream));
 		EPHMC_HIP(hipMemcpyAsync(status.data(), w.status, (size_t)nb_ * sizeof(int), hipMemcpyDeviceToHost, c->stream));
