@@ -5,7 +5,8 @@ FP64 path, with the force, dH and log-det errors of the mixed path against FP64.
 Workload (one "step" = one HMC trajectory): triangular lattice 32x32, Nt=128, beta=40/eV,
 J=(0.10,-0.03,0.02) eV, lambda=0.3 J (input line 0.3 0.3 0.3), mu0=0, omega0=0.006 eV, Nmd=10,
 trajectory length pi/2 (Fourier acceleration default), 2 flavours, partial products
-checkpointed every 16 slices, initial field = free-boson draw with seed 3.  Momentum normals and
+of every slice stored for the force sweep (checkpoint k=1; one force launch covers 32 slices),
+initial field = free-boson draw with seed 3.  Momentum normals and
 accept uniforms are synthetic (numpy, seed 3) and resident in host memory; the chain state
 (field, M_hat^{-1}) lives in HBM.
 
@@ -25,7 +26,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "electron-phonon-hmc_amd"))
 
-L, NT, NMD, K_CKPT, SEED = 32, 128, 10, 16, 3
+L, NT, NMD, K_CKPT, SPAN, SEED = 32, 128, 10, 1, 32, 3
 BETA, J, LAM, MU0, OMEGA0, FLAVORS = 40.0, (0.10, -0.03, 0.02), (0.3, 0.3, 0.3), 0.0, 0.006, 2
 PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix), FLOP/s
 PEAK_FP32 = 157.3e12     # MI355X FP32 published dense peak
@@ -259,20 +260,20 @@ def main():
     el64, elmx = res["fp64"]["el"], res["mixed"]["el"]
     tps64, tpsmx = args.steps / el64, args.steps / elmx
     # dominant kernel of the measured (mixed) path: the adjoint force sweep, one launch per
-    # checkpoint segment; HIP-event average over the timed region
+    # span of 32 slices; HIP-event average over the timed region
     per_launch_ms, adj_calls = res["mixed"]["groups"]["adjoint_kernel"]
-    flops = stencil_flops_adjoint_per_launch(L * L, K_CKPT)
+    flops = stencil_flops_adjoint_per_launch(L * L, SPAN)
     achieved = flops / (per_launch_ms * 1e-3)
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32, "traffic": None,
-            "kernel": "adjoint_kernel<float,4,1024,1,4> (FP32 VALU stencils; one launch = one 16-slice segment)"}
+            "kernel": "adjoint_kernel<float,4,1024,1,4> (FP32 VALU stencils; one launch = 32 slices)"}
     out = {
         "metric": "HMC traj/s at 32x32, Nt=128: mixed-prec vs FP64 GPU; force rel err vs FP64",
         "value": tpsmx, "unit": "traj/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * elmx / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (numpy normals, seed 3)",
         "config": {"workload": "HMC 32x32 triangular, Nt=128, Nmd=10, traj=pi/2, 2 flavours, "
-                               "checkpoint 16, mixed precision", "global_batch": 1, "seq_len": NT,
+                               "every slice stored (k=1), mixed precision", "global_batch": 1, "seq_len": NT,
                    "parallelism": "none (1 GPU)"},
         "fp64_traj_per_s": tps64, "fp64_ms_per_step": 1e3 * el64 / args.steps,
         "mixed_over_fp64": tpsmx / tps64,

@@ -31,7 +31,9 @@ struct AdjArgs {
 	int np;                 // pair form, for the recompute of T (same arithmetic as the forward chain)
 	R a_lin[MAX_PAIRS], a_quad[MAX_PAIRS];
 	int cplx[MAX_PAIRS];
-	const R *ck;            // checkpoint T_{tau_b} of this segment (column-major N x N)
+	const R *ck;            // checkpoint T_{tau_b} of this segment (column-major N x N); with
+	                        // direct, T_tau for every tau of the segment at ck + (tau - tau_b) N^2
+	int direct;             // every slice stored: no recompute
 	R *seg;                 // scratch: blockIdx.x * (k * C * ns) + (tau - tau_b) * C * ns + c * ns + site
 	unsigned seg_stride;    // k * C * ns
 	const double *w_init;   // M_hat^{-1} (first segment only), else null
@@ -145,58 +147,61 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 	auto colok = [&](int c){ return col0 + c < ns; };
 
 	// ---- recompute T_tau, tau in [tau_b, tau_e), from the checkpoint --------------------
-#pragma unroll
-	for(int j = 0; j < S; j++)
-#pragma unroll
-		for(int c = 0; c < C; c++)
-			v[j][c] = (act[j] && colok(c)) ? a.ck[(size_t)(col0 + c) * ns + site[j]] : (R)0;
-	for(int tau = a.tau_b; tau < a.tau_e; tau++){
-		R *dst = seg + (size_t)(tau - a.tau_b) * C * ns;
-#pragma unroll
-		for(int j = 0; j < S; j++)
-			if(act[j])
-#pragma unroll
-				for(int c = 0; c < C; c++) dst[(size_t)c * ns + site[j]] = v[j][c];
-		if(tau == a.tau_e - 1) break;
-		const R *cft = a.coef + (size_t)tau * 6 * ns;
+	if(!a.direct){
 #pragma unroll
 		for(int j = 0; j < S; j++)
 #pragma unroll
-			for(int d = 0; d < 6; d++) cf[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
-		for(int p = 0; p < a.np; p++){
-			R w[S][C];
+			for(int c = 0; c < C; c++)
+				v[j][c] = (act[j] && colok(c)) ? a.ck[(size_t)(col0 + c) * ns + site[j]] : (R)0;
+		for(int tau = a.tau_b; tau < a.tau_e; tau++){
+			R *dst = seg + (size_t)(tau - a.tau_b) * C * ns;
 #pragma unroll
 			for(int j = 0; j < S; j++)
 				if(act[j])
 #pragma unroll
-					for(int c = 0; c < C; c++) buf0[(size_t)site[j] * C + c] = v[j][c];
-			__syncthreads();
+					for(int c = 0; c < C; c++) dst[(size_t)c * ns + site[j]] = v[j][c];
+			if(tau == a.tau_e - 1) break;
+			const R *cft = a.coef + (size_t)tau * 6 * ns;
 #pragma unroll
 			for(int j = 0; j < S; j++)
-				if(act[j]) gather6<R, C>(buf0, nbr[j], cf[j], a.mu, v[j], w[j]);
-			if(a.cplx[p]){
+#pragma unroll
+				for(int d = 0; d < 6; d++) cf[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
+			for(int p = 0; p < a.np; p++){
+				R w[S][C];
 #pragma unroll
 				for(int j = 0; j < S; j++)
 					if(act[j])
 #pragma unroll
-						for(int c = 0; c < C; c++) buf1[(size_t)site[j] * C + c] = w[j][c];
-				__syncthreads();
-#pragma unroll
-				for(int j = 0; j < S; j++){
-					if(!act[j]) continue;
-					R z[C];
-					gather6<R, C>(buf1, nbr[j], cf[j], a.mu, w[j], z);
-#pragma unroll
-					for(int c = 0; c < C; c++) v[j][c] = v[j][c] + fmac(a.a_quad[p], z[c], a.a_lin[p] * w[j][c]);
-				}
-			}else{
+						for(int c = 0; c < C; c++) buf0[(size_t)site[j] * C + c] = v[j][c];
 				__syncthreads();
 #pragma unroll
 				for(int j = 0; j < S; j++)
+					if(act[j]) gather6<R, C>(buf0, nbr[j], cf[j], a.mu, v[j], w[j]);
+				if(a.cplx[p]){
 #pragma unroll
-					for(int c = 0; c < C; c++) v[j][c] = fmac(a.a_lin[p], w[j][c], v[j][c]);
+					for(int j = 0; j < S; j++)
+						if(act[j])
+#pragma unroll
+							for(int c = 0; c < C; c++) buf1[(size_t)site[j] * C + c] = w[j][c];
+					__syncthreads();
+#pragma unroll
+					for(int j = 0; j < S; j++){
+						if(!act[j]) continue;
+						R z[C];
+						gather6<R, C>(buf1, nbr[j], cf[j], a.mu, w[j], z);
+#pragma unroll
+						for(int c = 0; c < C; c++) v[j][c] = v[j][c] + fmac(a.a_quad[p], z[c], a.a_lin[p] * w[j][c]);
+					}
+				}else{
+					__syncthreads();
+#pragma unroll
+					for(int j = 0; j < S; j++)
+#pragma unroll
+						for(int c = 0; c < C; c++) v[j][c] = fmac(a.a_lin[p], w[j][c], v[j][c]);
+				}
 			}
 		}
+
 	}
 
 	// ---- W state --------------------------------------------------------------------------
@@ -219,13 +224,14 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 	R cfn[S][6], tn0[S][C];
 	auto load_slice = [&](int tau){
 		const R *cft = a.coef + (size_t)tau * 6 * ns;
-		const R *tsrc = seg + (size_t)(tau - a.tau_b) * C * ns;
+		const R *tsrc = a.direct ? a.ck + (size_t)(tau - a.tau_b) * ns * ns + (size_t)col0 * ns
+		                         : seg + (size_t)(tau - a.tau_b) * C * ns;
 #pragma unroll
 		for(int j = 0; j < S; j++){
 #pragma unroll
 			for(int d = 0; d < 6; d++) cfn[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
 #pragma unroll
-			for(int c = 0; c < C; c++) tn0[j][c] = act[j] ? tsrc[(size_t)c * ns + site[j]] : (R)0;
+			for(int c = 0; c < C; c++) tn0[j][c] = (act[j] && colok(c)) ? tsrc[(size_t)c * ns + site[j]] : (R)0;
 		}
 	};
 	load_slice(a.tau_e - 1);
@@ -409,7 +415,7 @@ template <typename R>
 static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p,
 		const double *fscale, unsigned nt_eff)
 {
-	const unsigned ns = c->ns, nt = nt_eff, k = c->ckpt, C = adjoint_cols(c, prec);
+	const unsigned ns = c->ns, nt = nt_eff, k = c->ckpt, span = c->span, C = adjoint_cols(c, prec);
 	const size_t mat = c->mat;
 	AdjArgs<R> a{};
 	a.coef = prec == EPHMC_FP64 ? (const R *)c->d_coef64 : (const R *)c->d_coef32;
@@ -435,13 +441,14 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 	a.p = p;
 	a.kick = kick;
 	a.fscale = fscale;
-	const unsigned nseg = (nt + k - 1) / k;
+	const unsigned nseg = (nt + span - 1) / span;
 	int prev_b = 0, prev_n = 0;
 	const double *prev = nullptr;
 	for(int s = (int)nseg - 1; s >= 0; s--){
-		a.tau_b = s * (int)k;
-		a.tau_e = std::min<int>((s + 1) * (int)k, (int)nt);
-		a.ck = ck + (size_t)s * mat;
+		a.tau_b = s * (int)span;
+		a.tau_e = std::min<int>((s + 1) * (int)span, (int)nt);
+		a.ck = ck + (size_t)(a.tau_b / (int)k) * mat;
+		a.direct = k == 1;
 		a.seg = (R *)c->d_seg;
 		a.w_init = (s == (int)nseg - 1) ? minv : nullptr;
 		a.partial = c->d_partial[s & 1];

@@ -338,9 +338,13 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 		c->own_stream = true;
 	}
 	make_lattice(c->l1, c->l2, c->nnt);
-	c->ckpt = cfg->checkpoint ? std::min(cfg->checkpoint, c->nt) : std::min(16u, c->nt);
-	c->nseg = (c->nt + c->ckpt - 1) / c->ckpt;
 	const size_t rsz = cfg->precision == EPHMC_FP64 ? 8 : 4;
+	// checkpoint interval: every slice while the Nt partial products fit in 4 GiB (the force
+	// sweep then reads T_tau back instead of recomputing it), else 16
+	if(cfg->checkpoint) c->ckpt = std::min(cfg->checkpoint, c->nt);
+	else c->ckpt = (size_t)c->nt * c->mat * rsz <= ((size_t)4 << 30) ? 1u : std::min(16u, c->nt);
+	c->nseg = (c->nt + c->ckpt - 1) / c->ckpt;
+	c->span = c->ckpt == 1 ? std::min(32u, c->nt) : c->ckpt;
 	c->cols = adjoint_cols_auto(ns, rsz);
 	c->nwg = (ns + c->cols - 1) / c->cols;
 	c->red_blocks = (unsigned)std::min<size_t>((c->dim + 255) / 256, 1024);
@@ -360,8 +364,8 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 		|| (rc = dalloc(&c->d_coef32, mixed ? (size_t)c->nt * 6 * ns : 0))
 		|| (rc = dalloc((unsigned char **)&c->d_ck, (size_t)c->nseg * c->mat * rsz))
 		|| (rc = dalloc((unsigned char **)&c->d_seg, (size_t)c->nwg * c->cols * ns * c->ckpt * rsz))
-		|| (rc = dalloc(&c->d_partial[0], (size_t)c->ckpt * c->nwg * c->loc))
-		|| (rc = dalloc(&c->d_partial[1], (size_t)c->ckpt * c->nwg * c->loc))
+		|| (rc = dalloc(&c->d_partial[0], (size_t)c->span * c->nwg * c->loc))
+		|| (rc = dalloc(&c->d_partial[1], (size_t)c->span * c->nwg * c->loc))
 		|| (rc = dalloc(&c->d_Tfin, c->mat))
 		|| (rc = dalloc(&c->d_Tfin32, mixed ? c->mat : 0))
 		|| (rc = dalloc(&c->d_gj[0], c->mat)) || (rc = dalloc(&c->d_gj[1], c->mat))

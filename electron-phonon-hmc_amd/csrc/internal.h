@@ -104,13 +104,14 @@ struct ephmc_ctx {
 	float *d_coef32;
 
 	// chain / force workspace
-	unsigned ckpt;                    // k
-	unsigned nseg;                    // ceil(nt / k)
+	unsigned ckpt;                    // k: T stored every k slices (k = 1: every slice, read back directly)
+	unsigned nseg;                    // ceil(nt / k) stored matrices
+	unsigned span;                    // slices per launch of the force sweep (k, or 32 when k = 1)
 	unsigned cols;                    // columns per workgroup of the force sweep
 	unsigned nwg;                     // ceil(ns / cols)
-	void *d_ck;                       // checkpoints T_{s k}: nseg+1 matrices (R = fp64 or fp32)
-	void *d_seg;                      // per-workgroup segment scratch: k * ns * ns
-	double *d_partial[2];             // [k][nwg][3 ns] force partial sums, double-buffered
+	void *d_ck;                       // checkpoints T_{s k}: nseg matrices (R = fp64 or fp32)
+	void *d_seg;                      // per-workgroup segment scratch of the recompute: k * ns * ns (>= ns * ns)
+	double *d_partial[2];             // [span][nwg][3 ns] force partial sums, double-buffered
 	double *d_Tfin;                   // T_nt (fp64) or scratch for conversions
 	float *d_Tfin32;
 

@@ -147,7 +147,7 @@ class Context:
     """One chain on one GPU (ephmc_ctx)."""
 
     def __init__(self, l1, l2, nt, t, kappa, mu, omega, flavors=2, steps=10, traj_length=0.0,
-                 precision=FP64, checkpoint=16, taylor_order=4, device=0, **flags):
+                 precision=FP64, checkpoint=0, taylor_order=4, device=0, **flags):
         cfg = Config()
         cfg.l1, cfg.l2, cfg.nt = l1, l2, nt
         cfg.t[:] = list(t)

@@ -46,7 +46,8 @@ struct organic_engine {
 static struct organic_engine *engines[MAX_ENGINES];
 
 /* Precision of the engine: FP64 drop-in unless EPHMC_PRECISION=mixed; partial products are
- * checkpointed every EPHMC_CHECKPOINT slices (default 16). */
+ * checkpointed every EPHMC_CHECKPOINT slices (default: the engine's choice, every slice while
+ * they fit in 4 GiB). */
 static int env_precision(void)
 {
 	const char *e = getenv("EPHMC_PRECISION");
@@ -56,7 +57,7 @@ static int env_precision(void)
 static unsigned env_checkpoint(void)
 {
 	const char *e = getenv("EPHMC_CHECKPOINT");
-	return e ? (unsigned)atoi(e) : 16;
+	return e ? (unsigned)atoi(e) : 0;
 }
 
 static const organic_flags no_flags;

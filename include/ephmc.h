@@ -47,7 +47,10 @@ typedef struct ephmc_config {
 	double traj_length;      /* 0 selects pi/2 (organic_main.c:114-116)                  */
 	int taylor_order;        /* 2, 4 (reference default), 6 or 8                          */
 	int precision;           /* EPHMC_FP64 or EPHMC_MIXED                                */
-	unsigned checkpoint;     /* slices between stored partial products in the force (k) */
+	unsigned checkpoint;     /* slices between stored partial products in the force (k);
+	                            k = 1 stores every slice and the force sweep reads them back
+	                            instead of recomputing; 0: k = 1 while nt N^2 values fit in
+	                            4 GiB, else 16                                               */
 	int zero_filling, single_particle, constant_force, no_fourier_acc, classical_fourier_acc;
 	unsigned cfa_steps;
 	double cfa_mass;

@@ -125,7 +125,7 @@ def test_logdet_inverse_M_hat(l1, l2, nt):
 
 
 @pytest.mark.parametrize("l1,l2,nt,k", [(3, 3, 4, 16), (6, 6, 32, 16), (6, 6, 32, 5), (6, 6, 32, 1), (5, 7, 12, 4),
-                                        (2, 2, 8, 3), (16, 16, 16, 8)])
+                                        (2, 2, 8, 3), (16, 16, 16, 8), (5, 7, 40, 1), (3, 3, 4, 1)])
 def test_force_fp64(l1, l2, nt, k):
     say("force fp64", l1, l2, nt, k)
     om = OracleModel(l1, l2, nt, beta=nt * 40 / 32)
@@ -136,6 +136,20 @@ def test_force_fp64(l1, l2, nt, k):
     Fo = om.force(x, Minv, ld)
     Fg = ctx.force(x, Minv, ld)
     assert rel(Fg, Fo) < 1e-11
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("l1,l2,nt", [(6, 6, 40), (16, 16, 72), (32, 32, 8)])
+def test_force_every_slice_matches_recompute(precision, l1, l2, nt):
+    """k = 1 stores every partial product and the sweep reads them back (32 slices per launch);
+    k = 8 recomputes them from checkpoints with the forward chain's arithmetic: same bits."""
+    say("force k=1 vs k=8", precision, l1, l2, nt)
+    om = OracleModel(l1, l2, nt, beta=nt * 40 / 32)
+    x = field(om, 7)
+    ref = gpu_context(om)
+    _, _, Minv = ref.logdet_inv(ref.M_hat(x))
+    F = {k: gpu_context(om, precision=precision, checkpoint=k).force(x, Minv, 0.0) for k in (1, 8)}
+    np.testing.assert_array_equal(F[1], F[8])
 
 
 def test_force_mixed():
