@@ -150,7 +150,7 @@ def tl_leg(E, nsamples):
     a, b = (1.0, 0.0), (0.5, 0.866)
     cm = [a[0], 0.5 * (a[0] + b[0]), 0.5 * (b[0] - a[0]), a[1], 0.5 * (a[1] + b[1]), 0.5 * (b[1] - a[1])]
     normals = np.random.default_rng(7).standard_normal((nsamples, 3 * L * L))
-    ctx.tl_hist(normals[:min(4, nsamples)], cm, width, nbins)       # warm-up (workspace, code load)
+    ctx.tl_hist(normals, cm, width, nbins)       # warm-up: workspace of the full batch, code load
     ctx.sync()
     ctx.timing(True)
     t0 = time.perf_counter()
@@ -217,7 +217,7 @@ def main():
     ap.add_argument("--no-tl", action="store_true", help="skip the transient-localisation leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-chain leg")
     ap.add_argument("--batch-traj", type=int, default=3)
-    ap.add_argument("--tl-samples", type=int, default=256)
+    ap.add_argument("--tl-samples", type=int, default=512)
     ap.add_argument("--graph", action="store_true",
                     help="also time both chains with the trajectory body replayed as a hipGraph")
     ap.add_argument("--copy-calibration", action="store_true",

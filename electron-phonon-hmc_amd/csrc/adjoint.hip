@@ -38,11 +38,11 @@ struct AdjArgs {
 	unsigned seg_stride;    // k * C * ns
 	const double *w_init;   // M_hat^{-1} (first segment only), else null
 	R *wbuf;                // W state between segments (column-major N x N)
-	double *partial;        // [tau - tau_b][nwg][3 ns]
+	R *partial;             // [tau - tau_b][nwg][3 ns], in the sweep's precision
 	unsigned nwg;
 	int tau_b, tau_e;
 	// reduction of the previous segment's partials, folded into this launch
-	const double *red_src;  // partials of the segment reduced here, or null
+	const R *red_src;       // partials of the segment reduced here, or null
 	int red_tau_b, red_ntau;
 	double f[3];            // -kappa_k flavors
 	const double *fscale;   // device scalar multiplying the force (SINGLE_PARTICLE: exp(-logdet)), or null
@@ -58,7 +58,8 @@ struct AdjArgs {
 // max(nthreads, RED_CHUNKS) doubles.
 constexpr unsigned RED_CHUNKS = 4;
 
-__device__ void reduce_links(const double *__restrict__ part, unsigned nwg, unsigned loc, unsigned l0, unsigned nl,
+template <typename P>
+__device__ void reduce_links(const P *__restrict__ part, unsigned nwg, unsigned loc, unsigned l0, unsigned nl,
 		unsigned nlinks_total, int tau_b, const double (&f)[3], const double *__restrict__ fscale, double *__restrict__ pdot,
 		double *__restrict__ p, double kick, double *scratch, unsigned nthreads)
 {
@@ -74,8 +75,8 @@ __device__ void reduce_links(const double *__restrict__ part, unsigned nwg, unsi
 			if(g0 + ll < nl && lk < nlinks_total){
 				const unsigned tl = lk / loc, link = lk - tl * loc;
 				const unsigned w0 = (unsigned)((size_t)nwg * ch / CH), w1 = (unsigned)((size_t)nwg * (ch + 1) / CH);
-				const double *src = part + (size_t)tl * nwg * loc + link;
-				for(unsigned w = w0; w < w1; w++) s += src[(size_t)w * loc];
+				const P *src = part + (size_t)tl * nwg * loc + link;
+				for(unsigned w = w0; w < w1; w++) s += (double)src[(size_t)w * loc];
 			}
 			scratch[ch * G + ll] = s;
 		}
@@ -339,13 +340,13 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 #pragma unroll
 				for(int k = 0; k < 3; k++) hx[(size_t)site[j] * 3 + k] = h[j][2 * k + 1];
 		__syncthreads();
-		double *out = a.partial + ((size_t)(tau - a.tau_b) * a.nwg + blockIdx.x) * 3 * ns;
+		R *out = a.partial + ((size_t)(tau - a.tau_b) * a.nwg + blockIdx.x) * 3 * ns;
 #pragma unroll
 		for(int j = 0; j < S; j++)
 			if(act[j])
 #pragma unroll
 				for(int k = 0; k < 3; k++)
-					out[(size_t)site[j] * 3 + k] = (double)h[j][2 * k] + (double)hx[(size_t)nbr[j][2 * k] * 3 + k];
+					out[(size_t)site[j] * 3 + k] = h[j][2 * k] + hx[(size_t)nbr[j][2 * k] * 3 + k];
 #pragma unroll
 		for(int j = 0; j < S; j++)
 #pragma unroll
@@ -358,7 +359,8 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			if(act[j] && colok(c)) a.wbuf[(size_t)(col0 + c) * ns + site[j]] = wv[j][c];
 }
 
-__global__ __launch_bounds__(256) void force_reduce_kernel(const double *__restrict__ partial, unsigned nwg, unsigned loc,
+template <typename P>
+__global__ __launch_bounds__(256) void force_reduce_kernel(const P *__restrict__ partial, unsigned nwg, unsigned loc,
 		int tau_b, int ntau, double f0, double f1, double f2, const double *__restrict__ fscale, double *__restrict__ pdot,
 		double *__restrict__ p, double kick)
 {
@@ -399,6 +401,7 @@ static int launch_adjoint_c(ephmc_ctx *c, AdjArgs<R> a)
 	do { if(a.deg <= 4) EPHMC_ADJ_LAUNCH2(BT, S, 4); else EPHMC_ADJ_LAUNCH2(BT, S, 8); } while(0)
 	if(ns <= 256) EPHMC_ADJ_LAUNCH(256, 1);
 	else if(ns <= 512) EPHMC_ADJ_LAUNCH(512, 1);
+	else if(ns <= 1024 && sites_per_thread("EPHMC_ADJOINT_SITES") == 2) EPHMC_ADJ_LAUNCH(512, 2);
 	else if(ns <= 1024) EPHMC_ADJ_LAUNCH(1024, 1);
 	else if(ns <= 2048) EPHMC_ADJ_LAUNCH(1024, 2);
 	else { set_error("adjoint: lattice of %u sites exceeds 2048", ns); return EPHMC_ERR_UNSUPPORTED; }
@@ -443,7 +446,7 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 	a.fscale = fscale;
 	const unsigned nseg = (nt + span - 1) / span;
 	int prev_b = 0, prev_n = 0;
-	const double *prev = nullptr;
+	const R *prev = nullptr;
 	for(int s = (int)nseg - 1; s >= 0; s--){
 		a.tau_b = s * (int)span;
 		a.tau_e = std::min<int>((s + 1) * (int)span, (int)nt);
@@ -451,7 +454,7 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 		a.direct = k == 1;
 		a.seg = (R *)c->d_seg;
 		a.w_init = (s == (int)nseg - 1) ? minv : nullptr;
-		a.partial = c->d_partial[s & 1];
+		a.partial = (R *)(void *)c->d_partial[s & 1];
 		a.red_src = prev;
 		a.red_tau_b = prev_b;
 		a.red_ntau = prev_n;
@@ -470,7 +473,7 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 	}
 	const size_t n = (size_t)prev_n * c->loc;
 	EPHMC_TRY(timer_begin(c, "force_reduce"));
-	force_reduce_kernel<<<(unsigned)((n + 63) / 64), 256, 0, c->stream>>>(prev, a.nwg, c->loc, prev_b, prev_n,
+	force_reduce_kernel<R><<<(unsigned)((n + 63) / 64), 256, 0, c->stream>>>(prev, a.nwg, c->loc, prev_b, prev_n,
 		f[0], f[1], f[2], fscale, pdot, p, kick);
 	EPHMC_LAUNCHED();
 	EPHMC_TRY(timer_end(c, "force_reduce"));

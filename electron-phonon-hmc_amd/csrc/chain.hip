@@ -290,6 +290,14 @@ static int launch_strip(ephmc_ctx *c, const ChainArgs<R> &a)
 	return EPHMC_OK;
 }
 
+// Sites per thread requested by a tuning variable (1 when unset): 2 runs lattices of 513..1024
+// sites on 512 threads.
+unsigned sites_per_thread(const char *var)
+{
+	const char *e = getenv(var);
+	return (e && atoi(e) == 2) ? 2u : 1u;
+}
+
 // Launch shape: threads per workgroup and sites per thread from the lattice size; columns
 // per workgroup from the tuning table (ctx->cols) capped by LDS.
 template <typename R, int C>
@@ -307,6 +315,7 @@ static int launch_chain_c(ephmc_ctx *c, const ChainArgs<R> &a)
 	} while(0)
 	if(ns <= 256) EPHMC_CHAIN_LAUNCH(256, 1);
 	else if(ns <= 512) EPHMC_CHAIN_LAUNCH(512, 1);
+	else if(ns <= 1024 && sites_per_thread("EPHMC_CHAIN_SITES") == 2) EPHMC_CHAIN_LAUNCH(512, 2);
 	else if(ns <= 1024) EPHMC_CHAIN_LAUNCH(1024, 1);
 	else if(ns <= 2048) EPHMC_CHAIN_LAUNCH(1024, 2);
 	else if(ns <= 4096) EPHMC_CHAIN_LAUNCH(1024, 4);

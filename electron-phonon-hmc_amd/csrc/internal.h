@@ -178,6 +178,7 @@ int timer_begin(ephmc_ctx *c, const char *group);
 int timer_end(ephmc_ctx *c, const char *group);
 
 // chain.hip
+unsigned sites_per_thread(const char *var);
 int coefficients(ephmc_ctx *c, const double *x, bool want32);
 int chain_forward(ephmc_ctx *c, int prec, const void *src, void *dst, void *ck, unsigned ck_every,
 		int descending, bool from_identity);
