@@ -370,7 +370,7 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 		|| (rc = dalloc(&c->d_Tfin32, mixed ? c->mat : 0))
 		|| (rc = dalloc(&c->d_gj[0], c->mat)) || (rc = dalloc(&c->d_gj[1], c->mat))
 		|| (rc = dalloc(&c->d_piv, ns)) || (rc = dalloc(&c->d_src, ns)) || (rc = dalloc(&c->d_diag, ns))
-		|| (rc = dalloc(&c->d_dlu, 32 * 32))
+		|| (rc = dalloc(&c->d_dlu, 4 * 32 * 32))
 		|| (rc = dalloc(&c->d_scal, SCAL_COUNT))
 		|| (rc = dalloc(&c->d_twiddle, 2 * (size_t)c->nt)) || (rc = dalloc(&c->d_f2, c->nt)) || (rc = dalloc(&c->d_f2mom, c->nt))
 		|| (rc = dalloc(&c->d_red, std::max<size_t>(2 * c->red_blocks, fft_blocks)))

@@ -41,93 +41,45 @@ __device__ inline double rcp_nr(double x)
 	return fma(r, t, r);
 }
 
-// Fused block step: one launch per 32-column block K, grid over 64 x 64 output tiles.
-//
-// D^{-1}: Gauss-Jordan on [D | 1] (32 x 64) without exchanges, two pivots (g, g+1) per step
-// eliminated together through their 2 x 2 block P = [[a_gg, a_g,g+1], [a_g+1,g, a_g+1,g+1]]:
-// pivot rows <- P^{-1} pivot rows, every other row r -= [a_rg a_r,g+1] (new pivot rows).  Wave w
-// holds rows 8w..8w+7, lane c column c (lanes 32..63 end with D^{-1}); one barrier per step
-// publishes the two pivot rows and the two pivot columns.  The pivot columns of every step are
-// kept, and the LU multipliers of D (and the pivots U_gg) are formed from them afterwards:
-// l[r][g] = a_rg / a_gg, U_hh = a_hh - l[h][g] a_gh, l[r][h] = (a_rh - l[r][g] a_gh) / U_hh.
-//
-// The tile operands (A[K, column block], A[row block, K], the tile itself) are loaded into
-// registers before the inversion and stored to LDS after it.  The transform rows are kept
-// transposed (Et[m][row]); a thread forms a 4 x 2 block of them and updates a 4 x 4 block of the
-// tile, reading two rows of four from LDS per m.  Workgroups of the first column block check
-// the LU multipliers of their rows below K (l = -E L, L unit lower triangular, |l| <= 1);
-// workgroup (0,0) records the pivots.
-constexpr int GJ_RB = 64, GJ_CB = 64, GJ_LD = 68;
+// Diagonal-block inversion.  D^{-1} by Gauss-Jordan on [D | 1] (32 x 64) without exchanges,
+// two pivots (g, g+1) per step eliminated together through their 2 x 2 block
+// P = [[a_gg, a_g,g+1], [a_g+1,g, a_g+1,g+1]]: pivot rows <- P^{-1} pivot rows, every other row
+// r -= [a_rg a_r,g+1] (new pivot rows).  Wave w holds rows 8w..8w+7 in a[], lane c column c
+// (lanes 32..63 end with D^{-1}); one barrier per step publishes the two pivot rows and the two
+// pivot columns.  The pivot columns of every step are kept, and the LU multipliers of D (and the
+// pivots U_gg) are formed from them afterwards: l[r][g] = a_rg / a_gg,
+// U_hh = a_hh - l[h][g] a_gh, l[r][h] = (a_rh - l[r][g] a_gh) / U_hh.
+// Writes dinv_out[r][c] = D^{-1}[r][c] and lm_out[m][j] = L[m][j] (unit lower triangular, zero
+// elsewhere, NB x NB row-major), the pivots of rows kb.. to diag/piv, and raises the flag when a
+// multiplier exceeds 1.  Called by all 256 threads of a workgroup.
+struct GjInvLds {
+	__align__(16) double prow[2][2][2 * GJ_NB];     // pivot rows of the step, by step parity
+	__align__(16) double pcs[GJ_NB / 2][2][GJ_NB];  // pivot columns of every step
+};
 
-__global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__ A, double *__restrict__ An,
-		unsigned n, unsigned k0, unsigned nbe, double *__restrict__ diag, int *__restrict__ piv, double *__restrict__ scal)
+__device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, GjInvLds &sh, double *__restrict__ dinv_out,
+		double *__restrict__ lm_out, double *__restrict__ diag, int *__restrict__ piv, unsigned kb, double *__restrict__ scal)
 {
 	constexpr int NB = GJ_NB;
-	__shared__ __align__(16) double prow[2][2][2 * NB];     // pivot rows of the step, by step parity
-	__shared__ __align__(16) double pcs[NB / 2][2][NB];     // pivot columns of every step
-	__shared__ __align__(16) double dinv[NB][NB + 2];
-	__shared__ double lm[NB][NB + 1];                       // unit lower triangular L of D (zero above)
-	__shared__ __align__(16) double Art[NB][GJ_LD];         // A[row block, K], transposed
-	__shared__ __align__(16) double Et[NB][GJ_LD];          // transform rows, transposed
-	__shared__ __align__(16) double Ys[NB][GJ_LD];          // A[K, column block]
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-	const unsigned r0 = blockIdx.x * GJ_RB, c0 = blockIdx.y * GJ_CB;
-	const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
-	// ---- loads
-	double a[8];
-#pragma unroll
-	for(int q = 0; q < 8; q++){
-		const int r = 8 * wave + q;
-		if(lane < NB) a[q] = (r < (int)nbe && lane < (int)nbe) ? A[(size_t)(k0 + lane) * n + k0 + r] : (r == lane ? 1.0 : 0.0);
-		else a[q] = r == lane - NB ? 1.0 : 0.0;
-	}
-	double ysv[NB * GJ_CB / 256], arv[GJ_RB * NB / 256];
-#pragma unroll
-	for(int j = 0; j < NB * GJ_CB / 256; j++){
-		const int e = tid + 256 * j, m = e % NB, cc = e / NB;
-		const unsigned col = c0 + cc;
-		ysv[j] = (m < (int)nbe && col < n) ? A[(size_t)col * n + k0 + m] : 0.0;
-	}
-#pragma unroll
-	for(int j = 0; j < GJ_RB * NB / 256; j++){
-		const int e = tid + 256 * j, rr = e % GJ_RB, m = e / GJ_RB;
-		const unsigned i = r0 + rr;
-		arv[j] = (m < (int)nbe && i < n) ? A[(size_t)(k0 + m) * n + i] : 0.0;
-	}
-	// this thread's 4 x 4 tile block: rows 4 tr + i, columns 4 tc + j
-	const int tr = tid & 15, tc = tid >> 4;
-	double acc[4][4];
-#pragma unroll
-	for(int i = 0; i < 4; i++){
-		const unsigned row = r0 + 4 * tr + i;
-		const bool rowK = row >= k0 && row < k0 + nbe;
-#pragma unroll
-		for(int j = 0; j < 4; j++){
-			const unsigned col = c0 + 4 * tc + j;
-			acc[i][j] = (row < n && col < n && !rowK) ? A[(size_t)col * n + row] : 0.0;
-		}
-	}
-#pragma unroll
-	for(int e = tid; e < NB * (NB + 1); e += 256) lm[e / (NB + 1)][e % (NB + 1)] = (e / (NB + 1) == e % (NB + 1)) ? 1.0 : 0.0;
-	// ---- D^{-1}, two pivots per step (steps past nbe are identity steps and skipped)
 #pragma unroll
 	for(int g = 0; g < NB; g += 2){
 		if(g >= (int)nbe) break;
 		const int h = g + 1, b = (g >> 1) & 1, gq = g & 7;
 		if(wave == (g >> 3)){
-			prow[b][0][lane] = a[gq];
-			prow[b][1][lane] = a[gq + 1];
+			sh.prow[b][0][lane] = a[gq];
+			sh.prow[b][1][lane] = a[gq + 1];
 		}
 		if(lane == g || lane == h){
-			double *dst = &pcs[g >> 1][lane - g][8 * wave];
+			double *dst = &sh.pcs[g >> 1][lane - g][8 * wave];
 #pragma unroll
 			for(int q = 0; q < 8; q += 2) *(double2 *)(dst + q) = make_double2(a[q], a[q + 1]);
 		}
 		__syncthreads();
-		const double *pg = pcs[g >> 1][0], *ph = pcs[g >> 1][1];
+		const double *pg = sh.pcs[g >> 1][0], *ph = sh.pcs[g >> 1][1];
 		const double pgg = pg[g], phg = pg[h], pgh = ph[g], phh = ph[h];
 		const double idet = rcp_nr(fmac(pgg, phh, -pgh * phg));
-		const double u = prow[b][0][lane], v = prow[b][1][lane];
+		const double u = sh.prow[b][0][lane], v = sh.prow[b][1][lane];
 		const double u2 = fmac(phh, u, -pgh * v) * idet, v2 = fmac(pgg, v, -phg * u) * idet;
 		double cg[8], ch[8];
 #pragma unroll
@@ -146,52 +98,167 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	}
 	if(lane >= NB){
 #pragma unroll
-		for(int q = 0; q < 8; q++) dinv[8 * wave + q][lane - NB] = a[q];
+		for(int q = 0; q < 8; q++) dinv_out[(8 * wave + q) * NB + lane - NB] = a[q];
+	}
+	// L, entry (r, j) by thread tid + 256 i
+	bool bad = false;
+#pragma unroll
+	for(int i = 0; i < NB * NB / 256; i++){
+		const int e = tid + 256 * i, r = e / NB, j = e % NB;
+		double l = r == j ? 1.0 : 0.0;
+		if(r > j && r < (int)nbe){
+			const int st = j >> 1, g = 2 * st;
+			const double *pg = sh.pcs[st][0], *ph = sh.pcs[st][1];
+			const double pgg = pg[g], lg = pg[r] / pgg;
+			if(j == g) l = lg;
+			else{
+				const double uhh = ph[j] - (pg[j] / pgg) * ph[g];
+				l = (ph[r] - lg * ph[g]) / uhh;
+			}
+			bad |= !(fabs(l) <= 1.0);
+		}
+		lm_out[e] = l;
+	}
+	if(tid < NB / 2 && 2 * tid < (int)nbe){
+		const int g = 2 * tid, h = g + 1;
+		const double *pg = sh.pcs[tid][0], *ph = sh.pcs[tid][1];
+		diag[kb + g] = pg[g];
+		piv[kb + g] = (int)(kb + g);
+		if(h < (int)nbe){
+			diag[kb + h] = ph[h] - (pg[h] / pg[g]) * ph[g];
+			piv[kb + h] = (int)(kb + h);
+		}
+	}
+	if(bad) scal[SCAL_GJ_FLAG] = 1;
+}
+
+// Inversion of the first diagonal block A[0:nbe, 0:nbe] (one workgroup).
+__global__ __launch_bounds__(256) void gj_first_kernel(const double *__restrict__ A, unsigned n, unsigned nbe,
+		double *__restrict__ dinv_out, double *__restrict__ lm_out, double *__restrict__ diag, int *__restrict__ piv,
+		double *__restrict__ scal)
+{
+	constexpr int NB = GJ_NB;
+	__shared__ GjInvLds sh;
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	double a[8];
+#pragma unroll
+	for(int q = 0; q < 8; q++){
+		const int r = 8 * wave + q;
+		if(lane < NB) a[q] = (r < (int)nbe && lane < (int)nbe) ? A[(size_t)lane * n + r] : (r == lane ? 1.0 : 0.0);
+		else a[q] = r == lane - NB ? 1.0 : 0.0;
+	}
+	gj_invert_block(a, nbe, sh, dinv_out, lm_out, diag, piv, 0, scal);
+}
+
+// Block step for K = [k0, k0+nbe), grid (row tiles + 1, column tiles) of 64 x 64 tiles, with
+// D^{-1} and L of D = A[K,K] read from dinv_cur / lm_cur.
+//
+// Tile workgroups: load their operands (A[K, column block], A[row block, K], the tile), form the
+// transform rows E[i,:] = -A[i,K] D^{-1} (rows inside K: the rows of D^{-1}) as 4 x 2 register
+// blocks into a transposed LDS panel Et[m][row], and update 4 rows (16 apart) x 4 columns each:
+// A'[rows, cols] = [rows not in K] A + E A[K, cols] (columns inside K: E itself).  Workgroups of
+// the first column block check the LU multipliers of their rows below K (l = -E L, |l| <= 1).
+//
+// Lookahead workgroup (blockIdx = (row tiles, 0)): loads only the 32 x 32 operands of the next
+// diagonal block D' = A'[K', K'], K' = [k0+nbe, ...), forms it with the same operations in the
+// same order as the tile that holds it (so both hold the same bits), inverts it and writes
+// D'^{-1}, L' and its pivots for the next step, so that no step waits for an inversion of its own.
+constexpr int GJ_RB = 64, GJ_CB = 64, GJ_LD = 68;
+
+__global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__ A, double *__restrict__ An,
+		unsigned n, unsigned k0, unsigned nbe, const double *__restrict__ dinv_cur, const double *__restrict__ lm_cur,
+		double *__restrict__ dinv_next, double *__restrict__ lm_next, double *__restrict__ diag, int *__restrict__ piv,
+		double *__restrict__ scal)
+{
+	constexpr int NB = GJ_NB;
+	__shared__ __align__(16) double dinv[NB][NB + 2];
+	__shared__ double lm[NB][NB + 1];
+	__shared__ __align__(16) double Art[NB][GJ_LD];         // A[row block, K], transposed
+	__shared__ __align__(16) double Et[NB][GJ_LD];          // transform rows, transposed
+	__shared__ __align__(16) double Ys[NB][GJ_LD];          // A[K, column block]
+	__shared__ GjInvLds sh;
+	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+	const unsigned nrb = (n + GJ_RB - 1) / GJ_RB;
+	const unsigned k1 = k0 + nbe;
+	if(blockIdx.x == nrb){
+		// ---- lookahead: D' = A[K',K'] + E[K',:] A[K,K'] with E[K',:] = -A[K',K] D^{-1}, entry by
+		// entry the same fma sequence as the tile that holds it (m ascending over all NB columns)
+		if(blockIdx.y != 0 || k1 >= n) return;
+		const unsigned nb1 = min((unsigned)NB, n - k1);
+		double av[NB * NB / 256];
+#pragma unroll
+		for(int q = 0; q < NB * NB / 256; q++){
+			const int e = tid + 256 * q, i = e % NB, j = e / NB;     // i: row (fastest), j: column / m
+			dinv[e / NB][e % NB] = dinv_cur[e];
+			Art[j][i] = (j < (int)nbe && i < (int)nb1) ? A[(size_t)(k0 + j) * n + k1 + i] : 0.0;
+			Ys[i][j] = (i < (int)nbe && j < (int)nb1) ? A[(size_t)(k1 + j) * n + k0 + i] : 0.0;
+			av[q] = (i < (int)nb1 && j < (int)nb1) ? A[(size_t)(k1 + j) * n + k1 + i] : 0.0;
+		}
+		__syncthreads();
+#pragma unroll
+		for(int q = 0; q < NB * NB / 256; q++){
+			const int e = tid + 256 * q, i = e % NB, j = e / NB;
+			double x = 0;
+#pragma unroll 8
+			for(int m = 0; m < NB; m++) x = fmac(Art[m][i], dinv[m][j], x);
+			Et[j][i] = -x;
+		}
+		__syncthreads();
+		double (*Dn)[GJ_LD] = Art;           // free once E' is formed
+#pragma unroll
+		for(int q = 0; q < NB * NB / 256; q++){
+			const int e = tid + 256 * q, i = e % NB, j = e / NB;
+			double x = av[q];
+#pragma unroll 8
+			for(int m = 0; m < NB; m++) x = fmac(Et[m][i], Ys[m][j], x);
+			Dn[i][j] = x;
+		}
+		__syncthreads();
+		double a[8];
+#pragma unroll
+		for(int q = 0; q < 8; q++){
+			const int r = 8 * wave + q;
+			if(lane < NB) a[q] = (r < (int)nb1 && lane < (int)nb1) ? Dn[r][lane] : (r == lane ? 1.0 : 0.0);
+			else a[q] = r == lane - NB ? 1.0 : 0.0;
+		}
+		gj_invert_block(a, nb1, sh, dinv_next, lm_next, diag, piv, k1, scal);
+		return;
+	}
+	const unsigned r0 = blockIdx.x * GJ_RB, c0 = blockIdx.y * GJ_CB;
+	// ---- loads: D^{-1}, L (first column block), panels and tile values
+#pragma unroll
+	for(int i = 0; i < NB * NB / 256; i++){
+		const int e = tid + 256 * i;
+		dinv[e / NB][e % NB] = dinv_cur[e];
+		if(blockIdx.y == 0) lm[e / NB][e % NB] = lm_cur[e];
 	}
 #pragma unroll
 	for(int j = 0; j < NB * GJ_CB / 256; j++){
-		const int e = tid + 256 * j;
-		Ys[e % NB][e / NB] = ysv[j];
+		const int e = tid + 256 * j, m = e % NB, cc = e / NB;
+		const unsigned col = c0 + cc;
+		Ys[m][cc] = (m < (int)nbe && col < n) ? A[(size_t)col * n + k0 + m] : 0.0;
 	}
 #pragma unroll
 	for(int j = 0; j < GJ_RB * NB / 256; j++){
-		const int e = tid + 256 * j;
-		Art[e / GJ_RB][e % GJ_RB] = arv[j];
+		const int e = tid + 256 * j, rr = e % GJ_RB, m = e / GJ_RB;
+		const unsigned i = r0 + rr;
+		Art[m][rr] = (m < (int)nbe && i < n) ? A[(size_t)(k0 + m) * n + i] : 0.0;
 	}
-	// LU multipliers and pivots of D from the kept pivot columns: thread (step s, row r)
-	{
-		bool bad = false;
+	// this thread's tile block: rows tr + 16 i, columns 4 tc + j (16 lanes read 16 consecutive rows)
+	const int tr = tid & 15, tc = tid >> 4;
+	double acc[4][4];
 #pragma unroll
-		for(int j = 0; j < 2; j++){
-			const int e = tid + 256 * j, st = e / NB, r = e % NB, g = 2 * st, h = g + 1;
-			if(g < (int)nbe){
-				const double *pg = pcs[st][0], *ph = pcs[st][1];
-				const double pgg = pg[g], lhg = pg[h] / pgg, uhh = ph[h] - lhg * ph[g];
-				if(r > g && r < (int)nbe){
-					const double lg = pg[r] / pgg;
-					lm[r][g] = lg;
-					bad |= !(fabs(lg) <= 1.0);
-					if(r > h){
-						const double lh = (ph[r] - lg * ph[g]) / uhh;
-						lm[r][h] = lh;
-						bad |= !(fabs(lh) <= 1.0);
-					}
-				}
-				if(lead && r == 0){
-					diag[k0 + g] = pgg;
-					piv[k0 + g] = (int)(k0 + g);
-					if(h < (int)nbe){
-						diag[k0 + h] = uhh;
-						piv[k0 + h] = (int)(k0 + h);
-					}
-				}
-			}
+	for(int i = 0; i < 4; i++){
+		const unsigned row = r0 + tr + 16 * i;
+		const bool rowK = row >= k0 && row < k1;
+#pragma unroll
+		for(int j = 0; j < 4; j++){
+			const unsigned col = c0 + 4 * tc + j;
+			acc[i][j] = (row < n && col < n && !rowK) ? A[(size_t)col * n + row] : 0.0;
 		}
-		if(bad && lead) scal[SCAL_GJ_FLAG] = 1;
 	}
 	__syncthreads();
-	// ---- transform rows: thread (rows 4 rq + i, columns 2 cq + jj) of E = -A[rows, K] D^{-1}
-	// (rows inside K: the rows of D^{-1})
+	// ---- transform rows: thread (rows 4 rq + i, columns 2 cq + jj)
 	{
 		const int rq = tid & 15, cq = tid >> 4;
 		double e[4][2] = {};
@@ -209,7 +276,7 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 #pragma unroll
 		for(int i = 0; i < 4; i++){
 			const unsigned row = r0 + 4 * rq + i;
-			const bool inK = row >= k0 && row < k0 + nbe;
+			const bool inK = row >= k0 && row < k1;
 #pragma unroll
 			for(int jj = 0; jj < 2; jj++)
 				Et[2 * cq + jj][4 * rq + i] = inK ? dinv[row - k0][2 * cq + jj] : -e[i][jj];
@@ -221,7 +288,7 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	if(blockIdx.y == 0){
 		const int row = tid & 63, jg = tid >> 6;
 		const unsigned i = r0 + row;
-		if(i < n && i >= k0 + nbe){
+		if(i < n && i >= k1){
 			double er[NB];
 #pragma unroll
 			for(int m = 0; m < NB; m++) er[m] = Et[m][row];
@@ -237,12 +304,12 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 			if(bad) scal[SCAL_GJ_FLAG] = 1;
 		}
 	}
-	// ---- tile update: A'[rows, cols] = acc + E[rows, :] A[K, cols]; columns inside K take E
+	// ---- tile update: acc += E[rows, :] A[K, cols]
 #pragma unroll 8
 	for(int m = 0; m < NB; m++){
-		const double2 e0 = *(const double2 *)&Et[m][4 * tr], e1 = *(const double2 *)&Et[m][4 * tr + 2];
 		const double2 y0 = *(const double2 *)&Ys[m][4 * tc], y1 = *(const double2 *)&Ys[m][4 * tc + 2];
-		const double ev[4] = {e0.x, e0.y, e1.x, e1.y}, yv[4] = {y0.x, y0.y, y1.x, y1.y};
+		const double ev[4] = {Et[m][tr], Et[m][tr + 16], Et[m][tr + 32], Et[m][tr + 48]};
+		const double yv[4] = {y0.x, y0.y, y1.x, y1.y};
 #pragma unroll
 		for(int i = 0; i < 4; i++)
 #pragma unroll
@@ -252,11 +319,11 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	for(int j = 0; j < 4; j++){
 		const unsigned col = c0 + 4 * tc + j;
 		if(col >= n) continue;
-		const bool colK = col >= k0 && col < k0 + nbe;
+		const bool colK = col >= k0 && col < k1;
 #pragma unroll
 		for(int i = 0; i < 4; i++){
-			const unsigned row = r0 + 4 * tr + i;
-			if(row < n) An[(size_t)col * n + row] = colK ? Et[col - k0][4 * tr + i] : acc[i][j];
+			const unsigned row = r0 + tr + 16 * i;
+			if(row < n) An[(size_t)col * n + row] = colK ? Et[col - k0][tr + 16 * i] : acc[i][j];
 		}
 	}
 }
@@ -379,14 +446,21 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M)
 	const double *cur = M;
 	double *buf[2] = {c->d_gj[0], c->d_gj[1]};
 	int which = 0;
-	const dim3 sgrid((n + GJ_RB - 1) / GJ_RB, (n + GJ_CB - 1) / GJ_CB);
+	// D^{-1} and L of the current and the next diagonal block (double-buffered by step parity)
+	double *dinv_g[2] = {c->d_dlu, c->d_dlu + GJ_NB * GJ_NB}, *lm_g[2] = {c->d_dlu + 2 * GJ_NB * GJ_NB, c->d_dlu + 3 * GJ_NB * GJ_NB};
+	gj_first_kernel<<<1, 256, 0, c->stream>>>(M, n, std::min<unsigned>(GJ_NB, n), dinv_g[0], lm_g[0], c->d_diag, c->d_piv, c->d_scal);
+	EPHMC_LAUNCHED();
+	const dim3 sgrid((n + GJ_RB - 1) / GJ_RB + 1, (n + GJ_CB - 1) / GJ_CB);
+	int par = 0;
 	for(unsigned k0 = 0; k0 < n; k0 += GJ_NB){
 		const unsigned nbe = std::min<unsigned>(GJ_NB, n - k0);
 		double *next = buf[which];
-		gj_step_kernel<<<sgrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe, c->d_diag, c->d_piv, c->d_scal);
+		gj_step_kernel<<<sgrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe, dinv_g[par], lm_g[par], dinv_g[par ^ 1], lm_g[par ^ 1],
+			c->d_diag, c->d_piv, c->d_scal);
 		EPHMC_LAUNCHED();
 		cur = next;
 		which ^= 1;
+		par ^= 1;
 	}
 	gj_logdet_kernel<<<1, 1024, 0, c->stream>>>(c->d_diag, n, c->d_scal);
 	EPHMC_LAUNCHED();

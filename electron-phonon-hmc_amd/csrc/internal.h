@@ -119,7 +119,7 @@ struct ephmc_ctx {
 	double *d_gj[2];
 	int *d_piv, *d_src;
  	double *d_diag;
-	double *d_dlu;                    // LU of the current diagonal block (32 x 32)
+	double *d_dlu;                    // D^{-1}, L of the current / next diagonal block: 4 x (32 x 32)
 	double *d_scal;                   // device scalars (see SCAL_* below)
 
 	// bosons
