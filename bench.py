@@ -4,13 +4,14 @@ FP64 path, with the force, dH and log-det errors of the mixed path against FP64.
 
 Workload (one "step" = one HMC trajectory): triangular lattice 32x32, Nt=128, beta=40/eV,
 J=(0.10,-0.03,0.02) eV, lambda=0.3 J (input line 0.3 0.3 0.3), mu0=0, omega0=0.006 eV, Nmd=10,
-trajectory length pi/2 (Fourier acceleration default), 2 flavours, partial products
+trajectory length pi/2 (Fourier acceleration default), 2 flavours, the partial product
 of every slice stored for the force sweep (checkpoint k=1; one force launch covers 32 slices),
-initial field = free-boson draw with seed 3.  Momentum normals and
-accept uniforms are synthetic (numpy, seed 3) and resident in host memory; the chain state
-(field, M_hat^{-1}) lives in HBM.
+initial field = free-boson draw with seed 3.  Momentum normals and accept uniforms are synthetic
+(numpy, seed 3) and resident in host memory; the chain state (field, M_hat^{-1}) lives in HBM.
 
-Prints one JSON line (rank 0, one GPU).
+Each precision runs eagerly (per-kernel HIP-event timings) and with the trajectory body replayed
+as a captured hipGraph; the headline value is the graph-replayed mixed chain (--no-graph: the
+eager one).  Prints one JSON line (rank 0, one GPU).
 """
 import argparse
 import collections
@@ -30,6 +31,7 @@ L, NT, NMD, K_CKPT, SPAN, SEED = 32, 128, 10, 1, 32, 3
 BETA, J, LAM, MU0, OMEGA0, FLAVORS = 40.0, (0.10, -0.03, 0.02), (0.3, 0.3, 0.3), 0.0, 0.006, 2
 PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix), FLOP/s
 PEAK_FP32 = 157.3e12     # MI355X FP32 published dense peak
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "adjoint_fp32_32x32_nt128_traffic.json")
 
 
 def log(*a):
@@ -78,8 +80,16 @@ def run_chain(E, precision, normals_list, uniforms, x0_normals, warmup, steps, g
     return ctx, el, groups, acc
 
 
-def accuracy(E, x_host, normals, uniform):
-    """Mixed vs FP64 on the same field and the same trajectory inputs."""
+def accuracy(E, x_host, normals, uniform, nobs=40):
+    """Mixed vs FP64 on the same field and the same trajectory inputs:
+    force_rel_err       |F32 - F64| / |F64| at the chain's field, same M_hat^{-1};
+    dH_abs_err          Metropolis dH of the mixed path against the FP64 path evaluated at the
+                        same leapfrog endpoint (the mixed chain evaluates H in FP64);
+    dH_md_drift         dH of a trajectory integrated with FP32 forces minus the same trajectory
+                        with FP64 forces (the force error carried along the molecular dynamics);
+    obs_*               means and standard errors of log det / N and the boson action per link over
+                        nobs trajectories of each chain, started from the same field with the same
+                        random inputs."""
     t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, NT)
     out = {}
     ctxs = {}
@@ -87,21 +97,63 @@ def accuracy(E, x_host, normals, uniform):
         c = E.Context(L, L, NT, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=prec, checkpoint=K_CKPT)
         c.chain_set_x(x_host)
         ctxs[name] = c
-    minv = ctxs["fp64"].chain_minv()
-    ld = ctxs["fp64"].chain_logdet()
-    F64 = ctxs["fp64"].force(x_host, minv, ld)
-    F32 = ctxs["mixed"].force(x_host, minv, ld)
+    c64, c32 = ctxs["fp64"], ctxs["mixed"]
+    minv = c64.chain_minv()
+    ld = c64.chain_logdet()
+    F64 = c64.force(x_host, minv, ld)
+    F32 = c32.force(x_host, minv, ld)
     out["force_rel_err"] = float(np.linalg.norm(F32 - F64) / np.linalg.norm(F64))
-    out["logdet_rel_err"] = abs(ctxs["mixed"].chain_logdet() - ld) / abs(ld)
-    r64 = ctxs["fp64"].trajectory(normals, uniform)
-    r32 = ctxs["mixed"].trajectory(normals, uniform)
-    out["dH_fp64"] = r64.dH
-    out["dH_mixed"] = r32.dH
-    out["dH_abs_err"] = abs(r32.dH - r64.dH)
+    out["logdet_rel_err"] = abs(c32.chain_logdet() - ld) / abs(ld)
+
+    def H(c, x, p):
+        lx = c.logdet_inv(c.M_hat(x))[0]
+        return c.hamilton(x, lx, c.energy_momenta(p))
+
+    p0 = c64.sample_fourier_momenta(normals)
+    x1, p1 = c32.leap_frog(x_host, p0)             # mixed molecular dynamics
+    dH_mixed = H(c32, x1, p1) - H(c32, x_host, p0)
+    dH_fp64_same = H(c64, x1, p1) - H(c64, x_host, p0)
+    x1d, p1d = c64.leap_frog(x_host, p0)           # FP64 molecular dynamics
+    dH_fp64 = H(c64, x1d, p1d) - H(c64, x_host, p0)
+    out["dH_mixed"] = dH_mixed
+    out["dH_fp64"] = dH_fp64
+    out["dH_abs_err"] = abs(dH_mixed - dH_fp64_same)
+    out["dH_md_drift"] = abs(dH_mixed - dH_fp64)
+    r64 = c64.trajectory(normals, uniform)
+    r32 = c32.trajectory(normals, uniform)
     out["same_decision"] = bool(r64.accepted == r32.accepted)
+    # short chains from the same state with the same random inputs
+    rng = np.random.default_rng(SEED + 1)
+    draws = [(rng.standard_normal(len(x_host)), float(rng.random())) for _ in range(nobs)]
+    for name, c in ctxs.items():
+        c.chain_set_x(x_host)
+        lds, sbs, acc = [], [], 0
+        for nrm, u in draws:
+            rec = c.trajectory(nrm, u)
+            acc += rec.accepted
+            lds.append(rec.logdet / (L * L))
+            sbs.append(c.hamilton(c.chain_x(), 0.0, 0.0) / len(x_host))
+        for key, v in (("logdet_per_site", lds), ("boson_action_per_link", sbs)):
+            v = np.asarray(v)
+            out["obs_%s_%s" % (key, name)] = [float(v.mean()), float(v.std(ddof=1) / np.sqrt(len(v)))]
+        out["obs_acceptance_%s" % name] = acc / nobs
     for c in ctxs.values():
         c.close()
     return out
+
+
+def force_only(n):
+    """n mixed-precision forces at the bench size (no torch, no graphs): hardware-counter passes."""
+    import ephmc as E
+    t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, NT)
+    x = np.random.default_rng(SEED).standard_normal(NT * 3 * L * L) * 0.3
+    ctx = E.Context(L, L, NT, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=E.MIXED, checkpoint=K_CKPT)
+    ld, _, minv = ctx.logdet_inv(ctx.M_hat(x))
+    for _ in range(n):
+        F = ctx.force(x, minv, ld)
+    ctx.sync()
+    log("force-only: %d forces, |F|_1 = %.6g" % (n, float(np.abs(F).sum())))
+    ctx.close()
 
 
 def cpu_baseline(sample_threads):
@@ -218,14 +270,21 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-chain leg")
     ap.add_argument("--batch-traj", type=int, default=3)
     ap.add_argument("--tl-samples", type=int, default=512)
-    ap.add_argument("--graph", action="store_true",
-                    help="also time both chains with the trajectory body replayed as a hipGraph")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the chains with the trajectory body replayed as a hipGraph (the headline value "
+                         "then comes from the eager chain)")
+    ap.add_argument("--force-only", type=int, default=0, metavar="N",
+                    help="run N mixed-precision forces at 32x32, Nt=128 (eager, no graphs) and exit: the "
+                         "workload of the hardware-counter passes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy (known bytes) for HBM counter calibration")
     ap.add_argument("--copy-only", action="store_true", help="run only the calibration copy and exit")
     args = ap.parse_args()
     if args.copy_only:
         copy_calibration()
+        return
+    if args.force_only:
+        force_only(args.force_only)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; a single chain does not span GPUs here "
@@ -246,7 +305,7 @@ def main():
         ctx, el, groups, rec = run_chain(E, prec, normals_list, uniforms, x0, args.warmup, args.steps)
         res[name] = {"el": el, "groups": groups, "ctx": ctx}
         log(name, "%.2f ms/traj" % (1e3 * el / args.steps), {g: "%.3f ms x %d" % v for g, v in groups.items()})
-    if args.graph:
+    if not args.no_graph:
         for name, prec in (("fp64_graph", E.FP64), ("mixed_graph", E.MIXED)):
             ctx, el, groups, rec = run_chain(E, prec, normals_list, uniforms, x0, args.warmup, args.steps, graph=True)
             res[name] = {"el": el, "groups": groups, "ctx": ctx}
@@ -267,6 +326,13 @@ def main():
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32, "traffic": None,
             "kernel": "adjoint_kernel<float,4,1024,1,4> (FP32 VALU stencils; one launch = 32 slices)"}
+    # HBM bytes per launch of the same kernel from the two counter passes (FETCH_SIZE, WRITE_SIZE of
+    # `bench.py --force-only 3`, each scaled by the counter's reading for a 512 MiB copy)
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            tr = json.load(fh)
+        roof["traffic"] = tr["bytes_per_launch"]
+        roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
     out = {
         "metric": "HMC traj/s at 32x32, Nt=128: mixed-prec vs FP64 GPU; force rel err vs FP64",
         "value": tpsmx, "unit": "traj/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -278,12 +344,23 @@ def main():
         "fp64_traj_per_s": tps64, "fp64_ms_per_step": 1e3 * el64 / args.steps,
         "mixed_over_fp64": tpsmx / tps64,
         "force_rel_err": acc["force_rel_err"], "dH_abs_err": acc["dH_abs_err"],
+        "dH_md_drift": acc["dH_md_drift"],
         "logdet_rel_err_metropolis": acc["logdet_rel_err"],
+        "observables": {k[4:]: v for k, v in acc.items() if k.startswith("obs_")},
         "kernel_ms": {p: {g: round(v[0], 4) for g, v in res[p]["groups"].items()} for p in res if res[p]["groups"]},
         "roofline": roof,
     }
-    if args.graph:
+    if not args.no_graph:
+        # headline: the trajectory body replayed as a captured hipGraph (the same kernels; the eager
+        # chains above give the per-kernel HIP-event timings)
+        out["eager_traj_per_s"] = {"fp64": tps64, "mixed": tpsmx}
         out["graph_traj_per_s"] = {"fp64": args.steps / res["fp64_graph"]["el"], "mixed": args.steps / res["mixed_graph"]["el"]}
+        out["value"] = out["graph_traj_per_s"]["mixed"]
+        out["ms_per_step"] = 1e3 * res["mixed_graph"]["el"] / args.steps
+        out["fp64_traj_per_s"] = out["graph_traj_per_s"]["fp64"]
+        out["fp64_ms_per_step"] = 1e3 * res["fp64_graph"]["el"] / args.steps
+        out["mixed_over_fp64"] = out["value"] / out["fp64_traj_per_s"]
+        out["launch"] = "hipGraph replay of the trajectory body"
     if not args.no_batch:
         out["chains64"] = c4_leg(E, args.batch_traj)
         log("chains64", out["chains64"])
