@@ -273,6 +273,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the chains with the trajectory body replayed as a hipGraph (the headline value "
                          "then comes from the eager chain)")
+    ap.add_argument("--tl-only", action="store_true",
+                    help="run only the transient-localisation leg (--tl-samples samples) and print its JSON")
     ap.add_argument("--force-only", type=int, default=0, metavar="N",
                     help="run N mixed-precision forces at 32x32, Nt=128 (eager, no graphs) and exit: the "
                          "workload of the hardware-counter passes")
@@ -285,6 +287,10 @@ def main():
         return
     if args.force_only:
         force_only(args.force_only)
+        return
+    if args.tl_only:
+        import ephmc as E
+        print(json.dumps({"tl": tl_leg(E, args.tl_samples)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; a single chain does not span GPUs here "
