@@ -235,6 +235,21 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			for(int c = 0; c < C; c++) tn0[j][c] = (act[j] && colok(c)) ? tsrc[(size_t)c * ns + site[j]] : (R)0;
 		}
 	};
+	// bond(i, 2k) = H[i][2k] + H[nb(i,2k)][2k+1]: the H[.][2k+1] of a slice go to LDS (hx) at
+	// its end and, in FP32, the bond sums are written after the next slice's first barrier
+	// (after the last slice, after one more barrier); FP64 keeps a barrier of its own per slice,
+	// as holding H[.][2k] across the slice would spill registers
+	constexpr bool FOLD = sizeof(R) == 4;
+	R hb[S][3];
+	int tau_out = -1;
+	auto flush_bonds = [&](){
+		R *out = a.partial + ((size_t)(tau_out - a.tau_b) * a.nwg + blockIdx.x) * 3 * ns;
+#pragma unroll
+		for(int j = 0; j < S; j++)
+			if(act[j])
+#pragma unroll
+				for(int k = 0; k < 3; k++) out[(size_t)site[j] * 3 + k] = hb[j][k] + hx[(size_t)nbr[j][2 * k] * 3 + k];
+	};
 	load_slice(a.tau_e - 1);
 	for(int tau = a.tau_e - 1; tau >= a.tau_b; tau--){
 		R t[S][C];
@@ -306,6 +321,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 #pragma unroll
 					for(int c = 0; c < C; c++) wb[(size_t)site[j] * C + c] = wa[j][c];
 			__syncthreads();
+			if(FOLD && q == 0 && tau_out >= 0) flush_bonds();
 #pragma unroll
 			for(int j = 0; j < S; j++){
 				if(!act[j]) continue;
@@ -333,25 +349,27 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 				}
 			}
 		}
-		// bond(i, 2k) = H[i][2k] + H[nb(i,2k)][2k+1]
 #pragma unroll
-		for(int j = 0; j < S; j++)
+		for(int j = 0; j < S; j++){
 			if(act[j])
 #pragma unroll
 				for(int k = 0; k < 3; k++) hx[(size_t)site[j] * 3 + k] = h[j][2 * k + 1];
-		__syncthreads();
-		R *out = a.partial + ((size_t)(tau - a.tau_b) * a.nwg + blockIdx.x) * 3 * ns;
 #pragma unroll
-		for(int j = 0; j < S; j++)
-			if(act[j])
-#pragma unroll
-				for(int k = 0; k < 3; k++)
-					out[(size_t)site[j] * 3 + k] = h[j][2 * k] + hx[(size_t)nbr[j][2 * k] * 3 + k];
+			for(int k = 0; k < 3; k++) hb[j][k] = h[j][2 * k];
+		}
+		tau_out = tau;
+		if(!FOLD){
+			__syncthreads();
+			flush_bonds();
+			tau_out = -1;
+		}
 #pragma unroll
 		for(int j = 0; j < S; j++)
 #pragma unroll
 			for(int c = 0; c < C; c++) wv[j][c] = acc[j][c];
 	}
+	__syncthreads();
+	if(tau_out >= 0) flush_bonds();
 #pragma unroll
 	for(int j = 0; j < S; j++)
 #pragma unroll
