@@ -32,6 +32,7 @@ BETA, J, LAM, MU0, OMEGA0, FLAVORS = 40.0, (0.10, -0.03, 0.02), (0.3, 0.3, 0.3),
 PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix), FLOP/s
 PEAK_FP32 = 157.3e12     # MI355X FP32 published dense peak
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "adjoint_fp32_32x32_nt128_traffic.json")
+AUTOTUNE = True          # launch shapes measured per context before its chain starts (--no-autotune: rules)
 
 
 def log(*a):
@@ -58,6 +59,8 @@ def run_chain(E, precision, normals_list, uniforms, x0_normals, warmup, steps, g
     t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, NT)
     ctx = E.Context(L, L, NT, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=precision,
                     checkpoint=K_CKPT, graph=int(graph))
+    if AUTOTUNE:
+        log("autotune", precision, ctx.autotune())
     ctx.chain_init(x0_normals)
     for i in range(max(warmup, 1 if graph else 0)):
         ctx.trajectory(normals_list[i % len(normals_list)], uniforms[i % len(uniforms)], wait=False)
@@ -238,6 +241,8 @@ def c4_leg(E, ntraj, nchains=64, graph=True):
         dim = nt * 3 * l * l
         draws = [rng.standard_normal(dim) for _ in range(ntraj + 1)]
         us = rng.random(ntraj + 1)
+        if AUTOTUNE:
+            ctx.autotune()
         ctx.chain_init(rng.standard_normal(dim))
         chains.append((ctx, draws, us, l, nt))
     for ctx, draws, us, _, _ in chains:              # warm-up (graph capture)
@@ -281,7 +286,11 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy (known bytes) for HBM counter calibration")
     ap.add_argument("--copy-only", action="store_true", help="run only the calibration copy and exit")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the rule-based launch shapes instead of tuning each context")
     args = ap.parse_args()
+    global AUTOTUNE
+    AUTOTUNE = not args.no_autotune
     if args.copy_only:
         copy_calibration()
         return

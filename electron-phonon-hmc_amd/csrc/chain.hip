@@ -377,6 +377,7 @@ unsigned chain_cols(const ephmc_ctx *c, int prec)
 	// columns per workgroup of the forward chain: enough workgroups to cover 256 CUs;
 	// EPHMC_CHAIN_COLS overrides it for tuning sweeps
 	const unsigned ns = c->ns;
+	if(c->chain_cols_set) return c->chain_cols_set;
 	if(const char *e = getenv("EPHMC_CHAIN_COLS")){
 		const unsigned v = (unsigned)atoi(e);
 		if(v == 1 || v == 2 || v == 4 || v == 8) return v;

@@ -1,5 +1,6 @@
 // ctx.hip -- context, memory, C-ABI entry points and the device-resident HMC driver.
 #include "internal.h"
+#include <functional>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -363,8 +364,9 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 		|| (rc = dalloc(&c->d_coef64, (size_t)c->nt * 6 * ns))
 		|| (rc = dalloc(&c->d_coef32, mixed ? (size_t)c->nt * 6 * ns : 0))
 		|| (rc = dalloc((unsigned char **)&c->d_ck, (size_t)c->nseg * c->mat * rsz))
-		|| (rc = dalloc((unsigned char **)&c->d_seg, (size_t)c->nwg * c->cols * ns * c->ckpt * rsz))
-		|| (rc = dalloc(&c->d_partial[0], (size_t)c->span * c->nwg * c->loc))
+		// recompute scratch of every width 1, 2, 4: ceil(ns / C) C <= ns + 3 columns
+		|| (rc = dalloc((unsigned char **)&c->d_seg, ((size_t)ns + 3) * ns * c->ckpt * rsz))
+		|| (rc = dalloc(&c->d_partial[0], c->partial_cap = (size_t)c->span * c->nwg * c->loc))
 		|| (rc = dalloc(&c->d_partial[1], (size_t)c->span * c->nwg * c->loc))
 		|| (rc = dalloc(&c->d_Tfin, c->mat))
 		|| (rc = dalloc(&c->d_Tfin32, mixed ? c->mat : 0))
@@ -738,6 +740,125 @@ int ephmc_gj_fallbacks(ephmc_ctx *c, unsigned long *count)
 	double v;
 	EPHMC_TRY(read_scal(c, SCAL_GJ_FALLBACKS, &v));
 	*count = (unsigned long)v;
+	return EPHMC_OK;
+}
+
+} // extern "C"
+
+// ---- autotuner ----------------------------------------------------------------------------
+// Times the forward chain (coefficients + chain with the context's stored products) for every
+// columns-per-workgroup width that fits LDS, then the force sweep for every width, on a zero
+// field with M_hat^{-1} = 1 (the kernels do the same work for any values), and keeps the
+// fastest of each.  The sweep's partial buffers grow when a narrower width needs more
+// workgroups.  A captured trajectory graph is dropped, to be captured again with the new
+// shapes.  Uses the MD scratch buffers only; call it between trajectories.
+__global__ void zero_kernel(double *a, size_t n)
+{
+	for(size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = 0;
+}
+
+static int set_zero(ephmc_ctx *c, double *a, size_t n)
+{
+	zero_kernel<<<(unsigned)std::min<size_t>((n + 255) / 256, 1024), 256, 0, c->stream>>>(a, n);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+static int time_ms(ephmc_ctx *c, hipEvent_t e0, hipEvent_t e1, int reps, double *ms, const std::function<int()> &body)
+{
+	EPHMC_TRY(body());                                      // warm-up (code load)
+	EPHMC_HIP(hipEventRecord(e0, c->stream));
+	for(int r = 0; r < reps; r++) EPHMC_TRY(body());
+	EPHMC_HIP(hipEventRecord(e1, c->stream));
+	EPHMC_HIP(hipEventSynchronize(e1));
+	float f = 0;
+	EPHMC_HIP(hipEventElapsedTime(&f, e0, e1));
+	*ms = f / reps;
+	return EPHMC_OK;
+}
+
+static int reserve_partials(ephmc_ctx *c, unsigned nwg)
+{
+	const size_t need = (size_t)c->span * nwg * c->loc;
+	if(need <= c->partial_cap) return EPHMC_OK;
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	for(int b = 0; b < 2; b++){
+		(void)hipFree(c->d_partial[b]);
+		c->d_partial[b] = nullptr;
+	}
+	c->partial_cap = 0;
+	EPHMC_TRY(dalloc(&c->d_partial[0], need));
+	EPHMC_TRY(dalloc(&c->d_partial[1], need));
+	c->partial_cap = need;
+	return EPHMC_OK;
+}
+
+extern "C" {
+
+int ephmc_autotune(ephmc_ctx *c, ephmc_tune_result *res)
+{
+	if(!c){ set_error("null context"); return EPHMC_ERR_ARG; }
+	const int prec = c->cfg.precision;
+	const size_t rsz = prec == EPHMC_FP64 ? 8 : 4;
+	const unsigned ns = c->ns;
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	const bool timing = c->timing;
+	c->timing = false;
+	hipEvent_t e0, e1;
+	EPHMC_HIP(hipEventCreate(&e0));
+	EPHMC_HIP(hipEventCreate(&e1));
+	int rc = EPHMC_OK;
+	ephmc_tune_result r{};
+	void *tfin = prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32;
+	auto chain = [&]() -> int {
+		EPHMC_TRY(coefficients(c, c->d_xnew, prec != EPHMC_FP64));
+		return chain_forward(c, prec, nullptr, tfin, c->d_ck, c->ckpt, 0, true);
+	};
+	do {
+		if((rc = set_zero(c, c->d_xnew, c->dim)) != EPHMC_OK) break;
+		if((rc = set_zero(c, c->d_mhat, c->mat)) != EPHMC_OK || (rc = add_identity(c, c->d_mhat)) != EPHMC_OK) break;
+		// forward chain widths: 2 C N values of LDS
+		r.chain_ms = 1e30;
+		for(unsigned C : {1u, 2u, 4u, 8u}){
+			if(2 * (size_t)C * ns * rsz > 160 * 1024 || C > ns) continue;
+			c->chain_cols_set = C;
+			double ms;
+			if((rc = time_ms(c, e0, e1, 3, &ms, chain)) != EPHMC_OK) break;
+			if(ms < r.chain_ms){ r.chain_ms = ms; r.chain_cols = C; }
+		}
+		if(rc != EPHMC_OK) break;
+		c->chain_cols_set = r.chain_cols;
+		if((rc = chain()) != EPHMC_OK) break;
+		// force sweep widths: (4 C + 3) N values of LDS plus the reduction scratch
+		r.adjoint_ms = 1e30;
+		for(unsigned C : {1u, 2u, 4u}){
+			const unsigned bt = ns <= 256 ? 256 : (ns <= 512 ? 512 : 1024);
+			if((4 * (size_t)C * ns + 3 * (size_t)ns) * rsz + bt * 8 > 160 * 1024 || C > ns) continue;
+			const unsigned nwg = (ns + C - 1) / C;
+			if((rc = reserve_partials(c, nwg)) != EPHMC_OK) break;
+			c->cols = C;
+			c->nwg = nwg;
+			double ms;
+			if((rc = time_ms(c, e0, e1, 3, &ms, [&]{ return force_dispatch(c, prec, c->d_mhat, c->d_pdot, nullptr); })) != EPHMC_OK) break;
+			if(ms < r.adjoint_ms){ r.adjoint_ms = ms; r.adjoint_cols = C; }
+		}
+		if(rc != EPHMC_OK) break;
+		c->cols = r.adjoint_cols;
+		c->nwg = (ns + c->cols - 1) / c->cols;
+	} while(0);
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	c->timing = timing;
+	if(rc != EPHMC_OK) return rc;
+	if(c->graph_exec){
+		(void)hipGraphExecDestroy(c->graph_exec);
+		c->graph_exec = nullptr;
+	}
+	if(c->graph){
+		(void)hipGraphDestroy(c->graph);
+		c->graph = nullptr;
+	}
+	if(res) *res = r;
 	return EPHMC_OK;
 }
 

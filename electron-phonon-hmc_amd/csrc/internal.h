@@ -109,8 +109,10 @@ struct ephmc_ctx {
 	unsigned span;                    // slices per launch of the force sweep (k, or 32 when k = 1)
 	unsigned cols;                    // columns per workgroup of the force sweep
 	unsigned nwg;                     // ceil(ns / cols)
+	unsigned chain_cols_set;          // columns per workgroup of the forward chain chosen by ephmc_autotune (0: rule)
+	size_t partial_cap;               // doubles in each d_partial buffer
 	void *d_ck;                       // checkpoints T_{s k}: nseg matrices (R = fp64 or fp32)
-	void *d_seg;                      // per-workgroup segment scratch of the recompute: k * ns * ns (>= ns * ns)
+	void *d_seg;                      // per-workgroup segment scratch of the recompute: (ns + 3) * ns * k values
 	double *d_partial[2];             // [span][nwg][3 ns] force partial sums, double-buffered
 	double *d_Tfin;                   // T_nt (fp64) or scratch for conversions
 	float *d_Tfin32;

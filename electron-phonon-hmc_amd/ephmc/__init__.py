@@ -30,6 +30,10 @@ class Config(C.Structure):
                 ("device", C.c_int), ("stream", C.c_void_p), ("graph", C.c_int)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("chain_cols", C.c_uint), ("adjoint_cols", C.c_uint), ("chain_ms", C.c_double), ("adjoint_ms", C.c_double)]
+
+
 class Record(C.Structure):
     _fields_ = [("accepted", C.c_int), ("boltzmann", C.c_double), ("energy", C.c_double),
                 ("logdet", C.c_double), ("dH", C.c_double)]
@@ -78,6 +82,7 @@ def _load():
         "ephmc_gj_fallbacks": [vp, C.POINTER(C.c_ulong)],
         "ephmc_timing": [vp, C.c_int],
         "ephmc_timing_get": [vp, C.c_char_p, dp, C.POINTER(C.c_ulong)],
+        "ephmc_autotune": [vp, C.POINTER(TuneResult)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -308,6 +313,14 @@ class Context:
         v = C.c_ulong()
         check(LIB.ephmc_gj_fallbacks(self.h, C.byref(v)))
         return v.value
+
+    def autotune(self):
+        """Launch-shape tuning of this context (ephmc_autotune): returns the chosen columns per
+        workgroup of the forward chain and the force sweep and their per-call times (ms)."""
+        r = TuneResult()
+        check(LIB.ephmc_autotune(self.h, C.byref(r)))
+        return {"chain_cols": r.chain_cols, "adjoint_cols": r.adjoint_cols, "chain_ms": r.chain_ms,
+                "adjoint_ms": r.adjoint_ms}
 
     def timing(self, enable=True):
         check(LIB.ephmc_timing(self.h, int(enable)))

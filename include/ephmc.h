@@ -164,6 +164,18 @@ int ephmc_tl_hist(ephmc_ctx *ctx, const double *normals_host, unsigned nsamples,
 		const double *contr_mat_host, double width, unsigned nbins,
 		double *greens_host, double *results_host);
 
+/* ---- launch-shape tuning ----------------------------------------------------------------------
+ * No counterpart in the reference (its shapes are fixed by the CPU loops).  Times the forward
+ * chain and the force sweep of this context's size and precision for every columns-per-
+ * workgroup width that fits LDS, keeps the fastest of each for the context's later calls, and
+ * drops a captured trajectory graph so that it is captured again.  Waits; call it between
+ * trajectories.  res (may be null) receives the choice and its per-call times. */
+typedef struct ephmc_tune_result {
+	unsigned chain_cols, adjoint_cols;
+	double chain_ms, adjoint_ms;
+} ephmc_tune_result;
+int ephmc_autotune(ephmc_ctx *ctx, ephmc_tune_result *res);
+
 /* ---- diagnostics ---------------------------------------------------------------------------- */
 /* Number of inversions that needed row exchanges (a multiplier above 1 in the elimination
  * without exchanges) and ran the pivoted fallback; waits. */
