@@ -119,6 +119,10 @@ static struct organic_engine *engine_get(unsigned l1, unsigned l2, unsigned nt, 
 	cfg.cfa_steps = mode->cfa_steps;
 	cfg.cfa_mass = mode->cfa_mass;
 	GPU(ephmc_create(&cfg, &e->ctx));
+	/* EPHMC_AUTOTUNE=1: measured launch shapes (faster, but the choice follows timing noise and
+	 * moves the force's rounding, so runs are no longer bit-reproducible) */
+	const char *tune = getenv("EPHMC_AUTOTUNE");
+	if(tune && atoi(tune) == 1) GPU(ephmc_autotune(e->ctx, NULL));
 	e->l1 = l1; e->l2 = l2; e->nt = nt; e->ns = l1 * l2;
 	memcpy(e->t, t, sizeof(e->t));
 	memcpy(e->kappa, kappa, sizeof(e->kappa));

@@ -4,7 +4,7 @@
  * Reads the same input format, applies the same parameter scaling and runs HMC (or transient
  * localisation with STATIC_DISORDER) on the MI355X engine.  Extra input token: SEED <n>.
  * Environment: EPHMC_PRECISION=mixed selects FP32 molecular dynamics; EPHMC_CHECKPOINT=k the
- * force checkpoint interval.
+ * interval of stored partial products; EPHMC_AUTOTUNE=1 measured launch shapes.
  */
 #include <math.h>
 #include <stdio.h>
