@@ -62,13 +62,16 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 {
 	constexpr int NB = GJ_NB;
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-#pragma unroll
-	for(int g = 0; g < NB; g += 2){
-		if(g >= (int)nbe) break;
+	for(int g = 0; g < (int)nbe; g += 2){
 		const int h = g + 1, b = (g >> 1) & 1, gq = g & 7;
 		if(wave == (g >> 3)){
-			sh.prow[b][0][lane] = a[gq];
-			sh.prow[b][1][lane] = a[gq + 1];
+			// rows gq, gq + 1 of this wave (selected without indexing a[] at run time)
+#pragma unroll
+			for(int q = 0; q < 8; q += 2)
+				if(q == gq){
+					sh.prow[b][0][lane] = a[q];
+					sh.prow[b][1][lane] = a[q + 1];
+				}
 		}
 		if(lane == g || lane == h){
 			double *dst = &sh.pcs[g >> 1][lane - g][8 * wave];
