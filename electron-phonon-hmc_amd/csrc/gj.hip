@@ -188,14 +188,22 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		// entry the same fma sequence as the tile that holds it (m ascending over all NB columns)
 		if(blockIdx.y != 0 || k1 >= n) return;
 		const unsigned nb1 = min((unsigned)NB, n - k1);
-		double av[NB * NB / 256];
+		// every global load is issued before the first LDS store, so they share one round trip
+		double av[NB * NB / 256], dv[NB * NB / 256], xv[NB * NB / 256], yv[NB * NB / 256];
 #pragma unroll
 		for(int q = 0; q < NB * NB / 256; q++){
 			const int e = tid + 256 * q, i = e % NB, j = e / NB;     // i: row (fastest), j: column / m
-			dinv[e / NB][e % NB] = dinv_cur[e];
-			Art[j][i] = (j < (int)nbe && i < (int)nb1) ? A[(size_t)(k0 + j) * n + k1 + i] : 0.0;
-			Ys[i][j] = (i < (int)nbe && j < (int)nb1) ? A[(size_t)(k1 + j) * n + k0 + i] : 0.0;
+			dv[q] = dinv_cur[e];
+			xv[q] = (j < (int)nbe && i < (int)nb1) ? A[(size_t)(k0 + j) * n + k1 + i] : 0.0;
+			yv[q] = (i < (int)nbe && j < (int)nb1) ? A[(size_t)(k1 + j) * n + k0 + i] : 0.0;
 			av[q] = (i < (int)nb1 && j < (int)nb1) ? A[(size_t)(k1 + j) * n + k1 + i] : 0.0;
+		}
+#pragma unroll
+		for(int q = 0; q < NB * NB / 256; q++){
+			const int e = tid + 256 * q, i = e % NB, j = e / NB;
+			dinv[j][i] = dv[q];
+			Art[j][i] = xv[q];
+			Ys[i][j] = yv[q];
 		}
 		__syncthreads();
 #pragma unroll
@@ -228,24 +236,27 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		return;
 	}
 	const unsigned r0 = blockIdx.x * GJ_RB, c0 = blockIdx.y * GJ_CB;
-	// ---- loads: D^{-1}, L (first column block), panels and tile values
+	// ---- loads: D^{-1}, L (first column block), panels and tile values, all issued before the
+	// first LDS store so that they share one round trip
+	const bool first = blockIdx.y == 0;
+	double dv[NB * NB / 256], lv[NB * NB / 256], ysv[NB * GJ_CB / 256], arv[GJ_RB * NB / 256];
 #pragma unroll
 	for(int i = 0; i < NB * NB / 256; i++){
 		const int e = tid + 256 * i;
-		dinv[e / NB][e % NB] = dinv_cur[e];
-		if(blockIdx.y == 0) lm[e / NB][e % NB] = lm_cur[e];
+		dv[i] = dinv_cur[e];
+		lv[i] = first ? lm_cur[e] : 0.0;
 	}
 #pragma unroll
 	for(int j = 0; j < NB * GJ_CB / 256; j++){
 		const int e = tid + 256 * j, m = e % NB, cc = e / NB;
 		const unsigned col = c0 + cc;
-		Ys[m][cc] = (m < (int)nbe && col < n) ? A[(size_t)col * n + k0 + m] : 0.0;
+		ysv[j] = (m < (int)nbe && col < n) ? A[(size_t)col * n + k0 + m] : 0.0;
 	}
 #pragma unroll
 	for(int j = 0; j < GJ_RB * NB / 256; j++){
 		const int e = tid + 256 * j, rr = e % GJ_RB, m = e / GJ_RB;
 		const unsigned i = r0 + rr;
-		Art[m][rr] = (m < (int)nbe && i < n) ? A[(size_t)(k0 + m) * n + i] : 0.0;
+		arv[j] = (m < (int)nbe && i < n) ? A[(size_t)(k0 + m) * n + i] : 0.0;
 	}
 	// this thread's tile block: rows tr + 16 i, columns 4 tc + j (16 lanes read 16 consecutive rows)
 	const int tr = tid & 15, tc = tid >> 4;
@@ -259,6 +270,22 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 			const unsigned col = c0 + 4 * tc + j;
 			acc[i][j] = (row < n && col < n && !rowK) ? A[(size_t)col * n + row] : 0.0;
 		}
+	}
+#pragma unroll
+	for(int i = 0; i < NB * NB / 256; i++){
+		const int e = tid + 256 * i;
+		dinv[e / NB][e % NB] = dv[i];
+		if(first) lm[e / NB][e % NB] = lv[i];
+	}
+#pragma unroll
+	for(int j = 0; j < NB * GJ_CB / 256; j++){
+		const int e = tid + 256 * j;
+		Ys[e % NB][e / NB] = ysv[j];
+	}
+#pragma unroll
+	for(int j = 0; j < GJ_RB * NB / 256; j++){
+		const int e = tid + 256 * j;
+		Art[e / GJ_RB][e % GJ_RB] = arv[j];
 	}
 	__syncthreads();
 	// ---- transform rows: thread (rows 4 rq + i, columns 2 cq + jj)
