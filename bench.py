@@ -195,6 +195,25 @@ def copy_calibration():
     log("copy calibration: 5 x %d bytes" % (a.numel() * 8))
 
 
+def correlator_leg(E, x_host, reps=5):
+    """cc_correlator at 32x32, Nt=128 on the chain's field (FP64 M_hat^{-1} and log det): ms per
+    measurement (2 Nt slice applications + the 9-component contraction per slice), and its
+    agreement with a second call (the kernels are deterministic)."""
+    t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, NT)
+    c = E.Context(L, L, NT, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=E.FP64)
+    c.chain_set_x(x_host)
+    minv, ld = c.chain_minv(), c.chain_logdet()
+    g0 = c.cc_correlator(x_host, minv, ld)          # warm-up: allocates the transfer matrices
+    c.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g = c.cc_correlator(x_host, minv, ld)
+    el = (time.perf_counter() - t0) / reps
+    c.close()
+    return {"ms_per_measurement": 1e3 * el, "reps": reps, "repeatable": bool(np.array_equal(g, g0)),
+            "g_00_tau0": float(np.asarray(g).ravel()[0])}
+
+
 def tl_leg(E, nsamples):
     """Transient localisation at 32x32 (N=1024), 512 bins, beta=40, seed-7 disorder normals:
     disorder samples per second of the batched eigensolve + current matrix elements + binning."""
@@ -330,6 +349,8 @@ def main():
         r["ctx"].close()
     acc = accuracy(E, x_state, normals_list[0], float(uniforms[0]))
     log("accuracy", acc)
+    corr = correlator_leg(E, x_state)
+    log("correlator", corr)
 
     el64, elmx = res["fp64"]["el"], res["mixed"]["el"]
     tps64, tpsmx = args.steps / el64, args.steps / elmx
@@ -363,6 +384,7 @@ def main():
         "logdet_rel_err_metropolis": acc["logdet_rel_err"],
         "observables": {k[4:]: v for k, v in acc.items() if k.startswith("obs_")},
         "kernel_ms": {p: {g: round(v[0], 4) for g, v in res[p]["groups"].items()} for p in res if res[p]["groups"]},
+        "cc_correlator": corr,
         "roofline": roof,
     }
     if not args.no_graph:

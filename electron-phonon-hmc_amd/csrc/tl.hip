@@ -27,8 +27,8 @@
 //                   applies every reflector (last first)
 //   tl_sort         eigenvalue ranks, e^{-E}, the partition sum
 //   tl_current      V sorted by eigenvalue and W_c = G_c V (gather form of the bond currents)
-//   tl_pairs        64 x 64 tiles of J_c = V^T W_c (VALU FP64) with the binning in the
-//                   epilogue; tiles that cannot hold a pair E_j >= E_i are skipped.  Each tile
+//   tl_pairs        64 x 64 tiles of J_c = V^T W_c (FP64 matrix cores, v_mfma_f64_16x16x4)
+//                   with the binning in the epilogue; tiles that cannot hold a pair E_j >= E_i are skipped.  Each tile
 //                   bins its 4096 pairs in a fixed order (no atomics), so bins are
 //                   reproducible run to run
 //   tl_reduce       fixed-order sum of the tile histograms into the accumulated bins
@@ -571,8 +571,13 @@ __global__ __launch_bounds__(256) void tl_current_kernel(const double *__restric
 }
 
 // ---- pair tiles: J_c(j, i) = v_j . W_c[:, i], binned --------------------------------------
-// grid (i-block, j-block, sample); 256 threads, thread (tj, ti) = (tid / 16, tid % 16) owns
-// j = j0 + 4 tj + a, i = i0 + 4 ti + b.
+// grid (i-block, j-block, sample), 64 x 64 tiles, 4 waves.  The tile is a GEMM over the N rows:
+// wave w computes the 32 x 32 quarter (j: 32 (w / 2), i: 32 (w % 2)) of J_0 and J_1 with
+// v_mfma_f64_16x16x4_f64, 2 x 2 blocks of 16 x 16, from 16-row panels of V, W_0, W_1 staged in
+// LDS.  Operand layout of the instruction: A[m][k] in lane m + 16 k, B[k][n] in lane n + 16 k,
+// D[m][n] in lane n + 16 (m % 4), element m / 4.
+typedef double tl_v4d __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict__ Vall, const double *__restrict__ W0all,
 		const double *__restrict__ W1all, const double *__restrict__ evs_all, const double *__restrict__ ems_all,
 		unsigned n, double width, unsigned nbins, double *__restrict__ part)
@@ -581,8 +586,9 @@ __global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict_
 	__shared__ unsigned ebin[TL_TILE * TL_TILE];
 	__shared__ double eval[TL_TILE * TL_TILE];
 	__shared__ unsigned brange[2];
+	__shared__ uint2 rrange[TL_TILE];
 	const unsigned ib = blockIdx.x, jb = blockIdx.y, s = blockIdx.z, nblk = gridDim.x;
-	const unsigned tid = threadIdx.x, tj = tid >> 4, ti = tid & 15;
+	const unsigned tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 	const unsigned i0 = ib * TL_TILE, j0 = jb * TL_TILE;
 	const double *evs = evs_all + (size_t)s * n, *ems = ems_all + (size_t)s * n;
 	double *out = part + (((size_t)s * nblk + jb) * nblk + ib) * nbins;
@@ -593,13 +599,15 @@ __global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict_
 		return;
 	}
 	const double *V = Vall + (size_t)s * n * n, *W0 = W0all + (size_t)s * n * n, *W1 = W1all + (size_t)s * n * n;
-	double acc0[4][4], acc1[4][4];
+	const unsigned wj = 32 * (wave >> 1), wi = 32 * (wave & 1);      // this wave's quarter
+	const unsigned m16 = lane & 15, k4 = lane >> 4;
+	tl_v4d acc0[2][2], acc1[2][2];
 #pragma unroll
-	for(int a = 0; a < 4; a++)
+	for(int a = 0; a < 2; a++)
 #pragma unroll
-		for(int b = 0; b < 4; b++){
-			acc0[a][b] = 0;
-			acc1[a][b] = 0;
+		for(int b = 0; b < 2; b++){
+			acc0[a][b] = tl_v4d{0, 0, 0, 0};
+			acc1[a][b] = tl_v4d{0, 0, 0, 0};
 		}
 	if(tid == 0){
 		brange[0] = TL_NONE;
@@ -616,21 +624,22 @@ __global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict_
 			W1s[rr][cc] = (ic < n && r < n) ? W1[(size_t)ic * n + r] : 0.0;
 		}
 		__syncthreads();
-#pragma unroll 4
-		for(int rr = 0; rr < TL_KC; rr++){
-			double va[4], w0[4], w1[4];
 #pragma unroll
-			for(int a = 0; a < 4; a++){
-				va[a] = Vs[rr][4 * tj + a];
-				w0[a] = W0s[rr][4 * ti + a];
-				w1[a] = W1s[rr][4 * ti + a];
+		for(int kq = 0; kq < TL_KC / 4; kq++){
+			const unsigned kk = 4 * kq + k4;
+			double av[2], b0[2], b1[2];
+#pragma unroll
+			for(int a = 0; a < 2; a++){
+				av[a] = Vs[kk][wj + 16 * a + m16];
+				b0[a] = W0s[kk][wi + 16 * a + m16];
+				b1[a] = W1s[kk][wi + 16 * a + m16];
 			}
 #pragma unroll
-			for(int a = 0; a < 4; a++)
+			for(int a = 0; a < 2; a++)
 #pragma unroll
-				for(int b = 0; b < 4; b++){
-					acc0[a][b] += va[a] * w0[b];
-					acc1[a][b] += va[a] * w1[b];
+				for(int b = 0; b < 2; b++){
+					acc0[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], b0[b], acc0[a][b], 0, 0, 0);
+					acc1[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], b1[b], acc1[a][b], 0, 0, 0);
 				}
 		}
 		__syncthreads();
@@ -638,36 +647,73 @@ __global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict_
 	const double wn = width * n;
 	unsigned bmin = TL_NONE, bmax = 0;
 #pragma unroll
-	for(int a = 0; a < 4; a++)
+	for(int a = 0; a < 2; a++)
 #pragma unroll
-		for(int b = 0; b < 4; b++){
-			const unsigned j = j0 + 4 * tj + a, i = i0 + 4 * ti + b;
-			unsigned bin = TL_NONE;
-			double val = 0;
-			if(j < n && i < n && evs[j] >= evs[i]){
-				const double fr = (evs[j] - evs[i]) / width + 0.5;
-				if(fr < (double)nbins){
-					bin = (unsigned)fr;
-					const double inm = acc0[a][b] * acc0[a][b] + acc1[a][b] * acc1[a][b];
-					val = (ems[i] + ems[j]) * inm / wn;
-					bmin = min(bmin, bin);
-					bmax = max(bmax, bin);
+		for(int b = 0; b < 2; b++)
+#pragma unroll
+			for(int e = 0; e < 4; e++){
+				const unsigned jl = wj + 16 * a + 4 * e + k4, il = wi + 16 * b + m16;   // tile-local (j, i)
+				const unsigned j = j0 + jl, i = i0 + il;
+				unsigned bin = TL_NONE;
+				double val = 0;
+				if(j < n && i < n && evs[j] >= evs[i]){
+					const double fr = (evs[j] - evs[i]) / width + 0.5;
+					if(fr < (double)nbins){
+						bin = (unsigned)fr;
+						const double x0 = acc0[a][b][e], x1 = acc1[a][b][e];
+						const double inm = x0 * x0 + x1 * x1;
+						val = (ems[i] + ems[j]) * inm / wn;
+						bmin = min(bmin, bin);
+						bmax = max(bmax, bin);
+					}
 				}
+				ebin[jl * TL_TILE + il] = bin;
+				eval[jl * TL_TILE + il] = val;
 			}
-			ebin[tid * 16 + a * 4 + b] = bin;
-			eval[tid * 16 + a * 4 + b] = val;
-		}
 	if(bmin != TL_NONE){
 		atomicMin(&brange[0], bmin);
 		atomicMax(&brange[1], bmax);
+	}
+	__syncthreads();
+	// Along a row (fixed j) the eigenvalues E_i ascend, so the bins are TL_NONE (difference past
+	// the last bin), then non-increasing, then TL_NONE (E_i > E_j or i >= n): each bin is one run
+	// per row.  Row ranges first, then every bin adds its runs row by row, i ascending - the
+	// order of a scan of the whole tile, found by binary search.
+	if(tid < TL_TILE){
+		const unsigned *rb = ebin + tid * TL_TILE;
+		unsigned lo = TL_TILE, hi = 0;
+		for(unsigned il = 0; il < TL_TILE; il++)
+			if(rb[il] != TL_NONE){
+				lo = min(lo, il);
+				hi = il + 1;
+			}
+		rrange[tid] = make_uint2(lo, hi);
 	}
 	__syncthreads();
 	const unsigned blo = brange[0], bhi = brange[1];
 	for(unsigned b = tid; b < nbins; b += 256){
 		double sum = 0;
 		if(blo != TL_NONE && b >= blo && b <= bhi)
-			for(unsigned e = 0; e < TL_TILE * TL_TILE; e++)
-				if(ebin[e] == b) sum += eval[e];
+			for(unsigned jl = 0; jl < TL_TILE; jl++){
+				const uint2 rr = rrange[jl];
+				if(rr.x >= rr.y) continue;
+				const unsigned *rb = ebin + jl * TL_TILE;
+				if(b > rb[rr.x] || b < rb[rr.y - 1]) continue;
+				// first il with bin <= b, then first il with bin < b
+				unsigned p = rr.x, q = rr.y;
+				while(p < q){
+					const unsigned mid = (p + q) >> 1;
+					if(rb[mid] <= b) q = mid;
+					else p = mid + 1;
+				}
+				unsigned t = p, u = rr.y;
+				while(t < u){
+					const unsigned mid = (t + u) >> 1;
+					if(rb[mid] < b) u = mid;
+					else t = mid + 1;
+				}
+				for(unsigned il = p; il < t; il++) sum += eval[jl * TL_TILE + il];
+			}
 		out[b] = sum;
 	}
 }
