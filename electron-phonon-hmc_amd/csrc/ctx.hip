@@ -781,14 +781,17 @@ static int reserve_partials(ephmc_ctx *c, unsigned nwg)
 {
 	const size_t need = (size_t)c->span * nwg * c->loc;
 	if(need <= c->partial_cap) return EPHMC_OK;
+	// the new pair first: a failed allocation leaves the context on its old buffers
+	double *nb[2] = {nullptr, nullptr};
+	if(dalloc(&nb[0], need) != EPHMC_OK || dalloc(&nb[1], need) != EPHMC_OK){
+		if(nb[0]) (void)hipFree(nb[0]);
+		return EPHMC_ERR_ALLOC;
+	}
 	EPHMC_HIP(hipStreamSynchronize(c->stream));
 	for(int b = 0; b < 2; b++){
 		(void)hipFree(c->d_partial[b]);
-		c->d_partial[b] = nullptr;
+		c->d_partial[b] = nb[b];
 	}
-	c->partial_cap = 0;
-	EPHMC_TRY(dalloc(&c->d_partial[0], need));
-	EPHMC_TRY(dalloc(&c->d_partial[1], need));
 	c->partial_cap = need;
 	return EPHMC_OK;
 }
