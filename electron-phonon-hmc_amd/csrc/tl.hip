@@ -43,6 +43,7 @@ namespace ephmc {
 constexpr int TL_BT = 1024;     // tridiagonalisation / sort workgroup
 constexpr int TL_G = 16;        // sweeps per wavefront pass
 constexpr int TL_PF = 8;        // columns loaded ahead by the rotation pass
+constexpr int TL_QC = 64;       // rotation steps staged in LDS at a time
 constexpr int TL_TILE = 64;
 constexpr int TL_KC = 16;       // rows of V / W per LDS stage of the pair tiles
 constexpr unsigned TL_NONE = 0xffffffffu;
@@ -367,13 +368,15 @@ __global__ __launch_bounds__(64) void tl_ql_scalar_kernel(const double *__restri
 
 // ---- rotations applied to Z = I (thread = row, column-major Z) ------------------------------
 // G sweeps per pass: sweep k acts at position q + k while the window steps q downwards, so the
-// pass holds G+1 columns in registers.  Column q-1-j is loaded PF steps ahead and the G
-// rotations of step q-1 are loaded during step q; a sweep outside its range at a step gets the
-// identity rotation (c, s) = (1, 0), which leaves both columns unchanged.
-template <int G, int PF>
+// pass holds G+1 columns in registers.  Column q-1-j is loaded PF steps ahead.  The rotations of
+// the next QC steps of all G sweeps are staged in LDS by the whole workgroup (every row of the
+// sample applies the same rotations); a sweep outside its range at a step gets the identity
+// rotation (c, s) = (1, 0), which leaves both columns unchanged.
+template <int G, int PF, int QC>
 __global__ __launch_bounds__(256) void tl_ql_apply_kernel(double *__restrict__ Zall, unsigned n, const double2 *__restrict__ rotall,
 		size_t rot_cap, const int4 *__restrict__ swall, unsigned sw_cap, const unsigned *__restrict__ nsw_all)
 {
+	__shared__ double2 rs[QC][G];
 	const unsigned s = blockIdx.y;
 	const unsigned r = blockIdx.x * 256 + threadIdx.x;
 	const bool act = r < n;
@@ -401,15 +404,6 @@ __global__ __launch_bounds__(256) void tl_ql_apply_kernel(double *__restrict__ Z
 				hi[k] = 0;
 			}
 		}
-		auto fetch = [&](int q, double2 *cs) {
-#pragma unroll
-			for(int k = 0; k < G; k++){
-				const int pos = q + k;
-				const bool in = pos >= lo[k] && pos <= hi[k];
-				const double2 v = rot[in ? base[k] + (hi[k] - pos) : 0];
-				cs[k] = in ? v : make_double2(1.0, 0.0);
-			}
-		};
 		double W[G + 1], pf[PF];
 #pragma unroll
 		for(int k = 0; k <= G; k++){
@@ -421,34 +415,41 @@ __global__ __launch_bounds__(256) void tl_ql_apply_kernel(double *__restrict__ Z
 			const int col = P - 1 - j;
 			pf[j] = (act && col >= 0) ? Z[(size_t)col * n + r] : 0.0;
 		}
-		double2 cs[G], csn[G];
-		fetch(P, cs);
-		int q = P;
-		for(;;){
-			if(q > qend) fetch(q - 1, csn);
-#pragma unroll
-			for(int k = 0; k < G; k++){
-				const double a = W[k], f = W[k + 1];
-				W[k + 1] = cs[k].y * a + cs[k].x * f;
-				W[k] = cs[k].x * a - cs[k].y * f;
+		// steps q = P, P-1, ..., qend in chunks of QC
+		for(int qc = P; qc >= qend; qc -= QC){
+			__syncthreads();                 // the previous chunk's rotations are consumed
+			for(int e = threadIdx.x; e < QC * G; e += 256){
+				const int t = e / G, k = e % G, pos = qc - t + k;
+				const bool in = qc - t >= qend && pos >= lo[k] && pos <= hi[k];
+				rs[t][k] = in ? rot[base[k] + (hi[k] - pos)] : make_double2(1.0, 0.0);
 			}
-			if(q <= qend) break;
-			const int col = q + G;
-			if(act && col < (int)n) Z[(size_t)col * n + r] = W[G];
+			__syncthreads();
+			const int tn = min(QC, qc - qend + 1);
+			for(int t = 0; t < tn; t++){
+				const int q = qc - t;
 #pragma unroll
-			for(int k = G; k > 0; k--) W[k] = W[k - 1];
-			q--;
-			W[0] = pf[0];
+				for(int k = 0; k < G; k++){
+					const double2 cs = rs[t][k];
+					const double a = W[k], f = W[k + 1];
+					W[k + 1] = cs.y * a + cs.x * f;
+					W[k] = cs.x * a - cs.y * f;
+				}
+				if(q <= qend) break;
+				const int col = q + G;
+				if(act && col < (int)n) Z[(size_t)col * n + r] = W[G];
 #pragma unroll
-			for(int j = 0; j + 1 < PF; j++) pf[j] = pf[j + 1];
-			const int pcol = q - PF;
-			pf[PF - 1] = (act && pcol >= 0) ? Z[(size_t)pcol * n + r] : 0.0;
+				for(int k = G; k > 0; k--) W[k] = W[k - 1];
+				W[0] = pf[0];
 #pragma unroll
-			for(int k = 0; k < G; k++) cs[k] = csn[k];
+				for(int j = 0; j + 1 < PF; j++) pf[j] = pf[j + 1];
+				const int pcol = q - 1 - PF;
+				pf[PF - 1] = (act && pcol >= 0) ? Z[(size_t)pcol * n + r] : 0.0;
+			}
 		}
+		// the window ends at q = qend: columns qend .. qend + G
 #pragma unroll
 		for(int k = 0; k <= G; k++){
-			const int col = q + k;
+			const int col = qend + k;
 			if(act && col >= 0 && col < (int)n) Z[(size_t)col * n + r] = W[k];
 		}
 	}
@@ -852,7 +853,7 @@ extern "C" int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamp
 		tl_ql_scalar_kernel<<<nb_, 64, 2 * (size_t)n * 8, c->stream>>>(w.d, w.e, n, w.ev, w.rot, w.rot_cap, w.sw, w.sw_cap, w.nsw,
 			w.status);
 		EPHMC_LAUNCHED();
-		tl_ql_apply_kernel<TL_G, TL_PF><<<dim3((n + 255) / 256, nb_), 256, 0, c->stream>>>(w.Z, n, w.rot, w.rot_cap, w.sw, w.sw_cap, w.nsw);
+		tl_ql_apply_kernel<TL_G, TL_PF, TL_QC><<<dim3((n + 255) / 256, nb_), 256, 0, c->stream>>>(w.Z, n, w.rot, w.rot_cap, w.sw, w.sw_cap, w.nsw);
 		EPHMC_LAUNCHED();
 		EPHMC_TRY(timer_end(c, "tl_ql"));
 		EPHMC_TRY(timer_begin(c, "tl_back"));
