@@ -293,7 +293,7 @@ def main():
     ap.add_argument("--no-tl", action="store_true", help="skip the transient-localisation leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-chain leg")
     ap.add_argument("--batch-traj", type=int, default=3)
-    ap.add_argument("--tl-samples", type=int, default=512)
+    ap.add_argument("--tl-samples", type=int, default=1024)
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the chains with the trajectory body replayed as a hipGraph (the headline value "
                          "then comes from the eager chain)")

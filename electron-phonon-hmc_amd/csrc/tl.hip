@@ -820,11 +820,11 @@ extern "C" int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamp
 	if(n > 2 * TL_BT){ set_error("transient localisation supports N <= %d sites", 2 * TL_BT); return EPHMC_ERR_ARG; }
 	const size_t mat = (size_t)n * n;
 	const unsigned nblk = (n + TL_TILE - 1) / TL_TILE;
-	// samples per batch: about 24 GB of workspace (5.5 N^2 doubles per sample), at most 512, so
-	// that the one-wave-per-sample QL recurrence has many samples in flight
+	// samples per batch: about 48 GB of workspace (5.5 N^2 doubles per sample), at most 1024, so
+	// that the one-wave-per-sample QL recurrence has a wave on every SIMD
 	const size_t per = (size_t)(5.5 * mat * 8) + (size_t)nblk * nblk * nbins * 8;
-	unsigned B = (unsigned)std::max<size_t>(1, ((size_t)24 << 30) / per);
-	B = std::min(B, 512u);
+	unsigned B = (unsigned)std::max<size_t>(1, ((size_t)48 << 30) / per);
+	B = std::min(B, 1024u);
 	B = std::min(B, nsamples);
 	EPHMC_TRY(tl_reserve(c, B, nbins));
 	TlWork &w = c->tl;
