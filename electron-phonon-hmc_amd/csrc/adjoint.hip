@@ -34,6 +34,7 @@ struct AdjArgs {
 	const R *ck;            // checkpoint T_{tau_b} of this segment (column-major N x N); with
 	                        // direct, T_tau for every tau of the segment at ck + (tau - tau_b) N^2
 	int direct;             // every slice stored: no recompute
+	int w0_early;           // W_0 written at the end of the previous slice (no barrier of its own)
 	R *seg;                 // scratch: blockIdx.x * (k * C * ns) + (tau - tau_b) * C * ns + c * ns + site
 	unsigned seg_stride;    // k * C * ns
 	const double *w_init;   // M_hat^{-1} (first segment only), else null
@@ -250,6 +251,16 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 #pragma unroll
 				for(int k = 0; k < 3; k++) out[(size_t)site[j] * 3 + k] = hb[j][k] + hx[(size_t)nbr[j][2 * k] * 3 + k];
 	};
+	// W_0 of a slice goes to wbs[0] as soon as it is known (here, then at the end of each slice);
+	// the T-power barriers make it visible, so the first W phase needs no barrier of its own
+	auto put_w0 = [&](){
+#pragma unroll
+		for(int j = 0; j < S; j++)
+			if(act[j])
+#pragma unroll
+				for(int c = 0; c < C; c++) wbs[0][(size_t)site[j] * C + c] = wv[j][c];
+	};
+	if(a.w0_early) put_w0();
 	load_slice(a.tau_e - 1);
 	for(int tau = a.tau_e - 1; tau >= a.tau_b; tau--){
 		R t[S][C];
@@ -278,6 +289,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 #pragma unroll
 					for(int c = 0; c < C; c++) bb[(size_t)site[j] * C + c] = t[j][c];
 			__syncthreads();
+			if(FOLD && a.w0_early && b == 1 && tau_out >= 0) flush_bonds();
 #pragma unroll
 			for(int j = 0; j < S; j++){
 				R tn[C];
@@ -315,13 +327,15 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 		for(int q = 0; q < MAXD; q++){
 			if(q >= deg) break;
 			R *wb = wbs[q & 1];
+			if(q > 0 || !a.w0_early){
 #pragma unroll
-			for(int j = 0; j < S; j++)
-				if(act[j])
+				for(int j = 0; j < S; j++)
+					if(act[j])
 #pragma unroll
-					for(int c = 0; c < C; c++) wb[(size_t)site[j] * C + c] = wa[j][c];
-			__syncthreads();
-			if(FOLD && q == 0 && tau_out >= 0) flush_bonds();
+						for(int c = 0; c < C; c++) wb[(size_t)site[j] * C + c] = wa[j][c];
+				__syncthreads();
+				if(FOLD && !a.w0_early && q == 0 && tau_out >= 0) flush_bonds();
+			}
 #pragma unroll
 			for(int j = 0; j < S; j++){
 				if(!act[j]) continue;
@@ -367,6 +381,11 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 		for(int j = 0; j < S; j++)
 #pragma unroll
 			for(int c = 0; c < C; c++) wv[j][c] = acc[j][c];
+		// wbs[0] was last read in W phase deg - 2 (deg is even), before the barrier of phase deg - 1
+		if(a.w0_early){
+			if((deg - 1) % 2 == 0) __syncthreads();
+			put_w0();
+		}
 	}
 	__syncthreads();
 	if(tau_out >= 0) flush_bonds();
@@ -470,6 +489,7 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 		a.tau_e = std::min<int>((s + 1) * (int)span, (int)nt);
 		a.ck = ck + (size_t)(a.tau_b / (int)k) * mat;
 		a.direct = k == 1;
+		a.w0_early = c->adj_w0_early;
 		a.seg = (R *)c->d_seg;
 		a.w_init = (s == (int)nseg - 1) ? minv : nullptr;
 		a.partial = (R *)(void *)c->d_partial[s & 1];

@@ -347,6 +347,8 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 	c->nseg = (c->nt + c->ckpt - 1) / c->ckpt;
 	c->span = c->ckpt == 1 ? std::min(32u, c->nt) : c->ckpt;
 	c->cols = adjoint_cols_auto(ns, rsz);
+	c->adj_w0_early = 1;
+	if(const char *e = getenv("EPHMC_ADJOINT_W0EARLY")) c->adj_w0_early = atoi(e) != 0;
 	c->nwg = (ns + c->cols - 1) / c->cols;
 	c->red_blocks = (unsigned)std::min<size_t>((c->dim + 255) / 256, 1024);
 	const size_t fft_blocks = (c->loc + 31) / 32 + 1;
@@ -832,8 +834,10 @@ int ephmc_autotune(ephmc_ctx *c, ephmc_tune_result *res)
 		if(rc != EPHMC_OK) break;
 		c->chain_cols_set = r.chain_cols;
 		if((rc = chain()) != EPHMC_OK) break;
-		// force sweep widths: (4 C + 3) N values of LDS plus the reduction scratch
+		// force sweep widths: (4 C + 3) N values of LDS plus the reduction scratch; both orders of
+		// the first W phase (its own barrier, or W_0 written ahead)
 		r.adjoint_ms = 1e30;
+		int best_early = 1;
 		for(unsigned C : {1u, 2u, 4u}){
 			const unsigned bt = ns <= 256 ? 256 : (ns <= 512 ? 512 : 1024);
 			if((4 * (size_t)C * ns + 3 * (size_t)ns) * rsz + bt * 8 > 160 * 1024 || C > ns) continue;
@@ -841,13 +845,19 @@ int ephmc_autotune(ephmc_ctx *c, ephmc_tune_result *res)
 			if((rc = reserve_partials(c, nwg)) != EPHMC_OK) break;
 			c->cols = C;
 			c->nwg = nwg;
-			double ms;
-			if((rc = time_ms(c, e0, e1, 3, &ms, [&]{ return force_dispatch(c, prec, c->d_mhat, c->d_pdot, nullptr); })) != EPHMC_OK) break;
-			if(ms < r.adjoint_ms){ r.adjoint_ms = ms; r.adjoint_cols = C; }
+			for(int early = 0; early < 2 && rc == EPHMC_OK; early++){
+				c->adj_w0_early = early;
+				double ms;
+				if((rc = time_ms(c, e0, e1, 3, &ms, [&]{ return force_dispatch(c, prec, c->d_mhat, c->d_pdot, nullptr); })) != EPHMC_OK) break;
+				if(ms < r.adjoint_ms){ r.adjoint_ms = ms; r.adjoint_cols = C; best_early = early; }
+			}
+			if(rc != EPHMC_OK) break;
 		}
 		if(rc != EPHMC_OK) break;
 		c->cols = r.adjoint_cols;
 		c->nwg = (ns + c->cols - 1) / c->cols;
+		c->adj_w0_early = best_early;
+		r.adjoint_w0_early = best_early;
 	} while(0);
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);

@@ -110,6 +110,7 @@ struct ephmc_ctx {
 	unsigned cols;                    // columns per workgroup of the force sweep
 	unsigned nwg;                     // ceil(ns / cols)
 	unsigned chain_cols_set;          // columns per workgroup of the forward chain chosen by ephmc_autotune (0: rule)
+	int adj_w0_early;                 // force sweep: W_0 of a slice written ahead (one barrier fewer per slice)
 	size_t partial_cap;               // doubles in each d_partial buffer
 	void *d_ck;                       // checkpoints T_{s k}: nseg matrices (R = fp64 or fp32)
 	void *d_seg;                      // per-workgroup segment scratch of the recompute: (ns + 3) * ns * k values

@@ -31,7 +31,8 @@ class Config(C.Structure):
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("chain_cols", C.c_uint), ("adjoint_cols", C.c_uint), ("chain_ms", C.c_double), ("adjoint_ms", C.c_double)]
+    _fields_ = [("chain_cols", C.c_uint), ("adjoint_cols", C.c_uint), ("chain_ms", C.c_double), ("adjoint_ms", C.c_double),
+                ("adjoint_w0_early", C.c_int)]
 
 
 class Record(C.Structure):
@@ -320,7 +321,7 @@ class Context:
         r = TuneResult()
         check(LIB.ephmc_autotune(self.h, C.byref(r)))
         return {"chain_cols": r.chain_cols, "adjoint_cols": r.adjoint_cols, "chain_ms": r.chain_ms,
-                "adjoint_ms": r.adjoint_ms}
+                "adjoint_ms": r.adjoint_ms, "adjoint_w0_early": r.adjoint_w0_early}
 
     def timing(self, enable=True):
         check(LIB.ephmc_timing(self.h, int(enable)))

@@ -173,6 +173,7 @@ int ephmc_tl_hist(ephmc_ctx *ctx, const double *normals_host, unsigned nsamples,
 typedef struct ephmc_tune_result {
 	unsigned chain_cols, adjoint_cols;
 	double chain_ms, adjoint_ms;
+	int adjoint_w0_early;    /* 1: the sweep writes each slice's W_0 ahead of its first phase */
 } ephmc_tune_result;
 int ephmc_autotune(ephmc_ctx *ctx, ephmc_tune_result *res);
 

@@ -152,6 +152,23 @@ def test_force_every_slice_matches_recompute(precision, l1, l2, nt):
     np.testing.assert_array_equal(F[1], F[8])
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("l1,l2,nt", [(6, 6, 40), (32, 32, 8)])
+def test_force_w0_order_bit_identical(monkeypatch, precision, l1, l2, nt):
+    """The force sweep's two orders of the first W phase (its own barrier, or W_0 written at the
+    end of the previous slice) do the same arithmetic in the same order: same bits."""
+    say("force w0 order", precision, l1, l2, nt)
+    om = OracleModel(l1, l2, nt, beta=nt * 40 / 32)
+    x = field(om, 8)
+    ref = gpu_context(om)
+    _, _, Minv = ref.logdet_inv(ref.M_hat(x))
+    F = {}
+    for early in ("0", "1"):
+        monkeypatch.setenv("EPHMC_ADJOINT_W0EARLY", early)
+        F[early] = gpu_context(om, precision=precision, checkpoint=1).force(x, Minv, 0.0)
+    np.testing.assert_array_equal(F["0"], F["1"])
+
+
 def test_force_mixed():
     say("force mixed 6x6x32")
     om = OracleModel(6, 6, 32)
