@@ -9,56 +9,73 @@
 //   g_ab(tau) = 1/N sum_{i,j} chi_ia chi_jb [R_t0[j*N+i] R_0t[jb*N+ia] + R_t0[jb*N+ia] R_0t[j*N+i]
 //                                        - R_t0[jb*N+i] R_0t[j*N+ia] - R_t0[j*N+ia] R_0t[jb*N+i]]
 // with ia = nb(i, 2a), jb = nb(j, 2b), chi_ia = t_a - kappa_a x[0][i][a], chi_jb from slice tau.
-// The (i, j) sum is split over workgroups (partials per workgroup, then a fixed-order pass).
+// One launch covers every tau: thread = i (R read down its columns), a workgroup walks a chunk of j;
+// block sums per (tau, j-chunk, i-block), then a fixed-order pass per tau.
 #include "internal.h"
 
 namespace ephmc {
 
 constexpr int CC_BT = 256;
-constexpr int CC_ROWS = 8;     // values of j per thread
+constexpr unsigned CC_J = 8;   // values of j per workgroup
 
-__global__ __launch_bounds__(CC_BT) void cc_contract_kernel(const double *__restrict__ Rt0, const double *__restrict__ R0t,
-		const double *__restrict__ x, const int *__restrict__ nb, double t0, double t1, double t2,
-		double k0, double k1, double k2, unsigned ns, unsigned tau, double *__restrict__ partial)
+// block sums of one correlator call: [nt][ceil(N/CC_J)][ceil(N/CC_BT)][9]
+static size_t cc_partials(unsigned ns, unsigned nt)
+{
+	return (size_t)nt * ((ns + CC_J - 1) / CC_J) * ((ns + CC_BT - 1) / CC_BT) * 9;
+}
+
+// grid (i-blocks of CC_BT, j-chunks of cj, tau): thread = i, so every R read runs down a column
+// (R[j*N + i] with i consecutive); the workgroup walks its cj values of j and writes the block
+// sums of the 9 components for its (tau, j-chunk, i-block).
+__global__ __launch_bounds__(CC_BT) void cc_contract_kernel(const double *__restrict__ Tfin, const double *__restrict__ Rt0,
+		const double *__restrict__ R0t, const double *__restrict__ x, const int *__restrict__ nb, double t0, double t1, double t2,
+		double k0, double k1, double k2, unsigned ns, unsigned nt, unsigned cj, double *__restrict__ partial)
 {
 	__shared__ double red[9][CC_BT];
 	const double tk[3] = {t0, t1, t2}, kk[3] = {k0, k1, k2};
-	double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-	const unsigned i = blockIdx.y;                      // one i per grid row
+	const unsigned tau = blockIdx.z;
+	const size_t mat = (size_t)ns * ns;
+	const double *rt = tau == 0 ? Tfin : Rt0 + (size_t)(nt - tau) * mat;
+	const double *r0 = R0t + (size_t)tau * mat;
 	const double *xt = x + (size_t)tau * 3 * ns;
+	const unsigned i = blockIdx.x * CC_BT + threadIdx.x;
+	const bool act = i < ns;
+	double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 	unsigned ia[3];
 	double chia[3];
 #pragma unroll
 	for(int a = 0; a < 3; a++){
-		ia[a] = (unsigned)nb[(2 * a) * ns + i];
-		chia[a] = tk[a] - kk[a] * x[(size_t)i * 3 + a];
+		ia[a] = act ? (unsigned)nb[(2 * a) * ns + i] : 0u;
+		chia[a] = act ? tk[a] - kk[a] * x[(size_t)i * 3 + a] : 0.0;
 	}
-	for(unsigned jj = 0; jj < CC_ROWS; jj++){
-		const unsigned j = (blockIdx.x * CC_ROWS + jj) * CC_BT + threadIdx.x;
-		if(j >= ns) break;
-		const double rt_ji = Rt0[(size_t)j * ns + i], r0_ji = R0t[(size_t)j * ns + i];
-		double rt_jia[3], r0_jia[3];
-#pragma unroll
-		for(int a = 0; a < 3; a++){
-			rt_jia[a] = Rt0[(size_t)j * ns + ia[a]];
-			r0_jia[a] = R0t[(size_t)j * ns + ia[a]];
-		}
-#pragma unroll
-		for(int b = 0; b < 3; b++){
-			const unsigned jb = (unsigned)nb[(2 * b) * ns + j];
-			const double chijb = tk[b] - kk[b] * xt[(size_t)j * 3 + b];
-			const double rt_jbi = Rt0[(size_t)jb * ns + i], r0_jbi = R0t[(size_t)jb * ns + i];
+	const unsigned j0 = blockIdx.y * cj, j1 = min(j0 + cj, ns);
+	if(act)
+		for(unsigned j = j0; j < j1; j++){
+			const double *rtj = rt + (size_t)j * ns, *r0j = r0 + (size_t)j * ns;
+			const double rt_ji = rtj[i], r0_ji = r0j[i];
+			double rt_jia[3], r0_jia[3];
 #pragma unroll
 			for(int a = 0; a < 3; a++){
-				const double rt_jbia = Rt0[(size_t)jb * ns + ia[a]], r0_jbia = R0t[(size_t)jb * ns + ia[a]];
-				double corr = rt_ji * r0_jbia;
-				corr += rt_jbia * r0_ji;
-				corr -= rt_jbi * r0_jia[a];
-				corr -= rt_jia[a] * r0_jbi;
-				g[a * 3 + b] += chia[a] * chijb * corr;
+				rt_jia[a] = rtj[ia[a]];
+				r0_jia[a] = r0j[ia[a]];
+			}
+#pragma unroll
+			for(int b = 0; b < 3; b++){
+				const unsigned jb = (unsigned)nb[(2 * b) * ns + j];
+				const double chijb = tk[b] - kk[b] * xt[(size_t)j * 3 + b];
+				const double *rtb = rt + (size_t)jb * ns, *r0b = r0 + (size_t)jb * ns;
+				const double rt_jbi = rtb[i], r0_jbi = r0b[i];
+#pragma unroll
+				for(int a = 0; a < 3; a++){
+					const double rt_jbia = rtb[ia[a]], r0_jbia = r0b[ia[a]];
+					double corr = rt_ji * r0_jbia;
+					corr += rt_jbia * r0_ji;
+					corr -= rt_jbi * r0_jia[a];
+					corr -= rt_jia[a] * r0_jbi;
+					g[a * 3 + b] += chia[a] * chijb * corr;
+				}
 			}
 		}
-	}
 #pragma unroll
 	for(int q = 0; q < 9; q++) red[q][threadIdx.x] = g[q];
 	__syncthreads();
@@ -68,16 +85,21 @@ __global__ __launch_bounds__(CC_BT) void cc_contract_kernel(const double *__rest
 			for(int q = 0; q < 9; q++) red[q][threadIdx.x] += red[q][threadIdx.x + off];
 		__syncthreads();
 	}
-	if(threadIdx.x < 9) partial[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 9 + threadIdx.x] = red[threadIdx.x][0];
+	if(threadIdx.x < 9)
+		partial[(((size_t)tau * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * 9 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void cc_sum_kernel(const double *__restrict__ partial, unsigned nblocks, double scale, double *__restrict__ out)
+// out[tau][q] = scale * sum of the block sums of slice tau, in block order
+__global__ void cc_sum_kernel(const double *__restrict__ partial, unsigned nblocks, unsigned nt, double scale,
+		double *__restrict__ out)
 {
-	const unsigned q = threadIdx.x;
-	if(q >= 9) return;
+	const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+	if(e >= nt * 9) return;
+	const unsigned tau = e / 9, q = e % 9;
+	const double *p = partial + (size_t)tau * nblocks * 9 + q;
 	double s = 0;
-	for(unsigned b = 0; b < nblocks; b++) s += partial[(size_t)b * 9 + q];
-	out[q] = s * scale;
+	for(unsigned b = 0; b < nblocks; b++) s += p[(size_t)b * 9];
+	out[e] = s * scale;
 }
 
 } // namespace ephmc
@@ -95,7 +117,7 @@ extern "C" int ephmc_cc_correlator(ephmc_ctx *c, const double *x, const double *
 	if(!c->d_Rt0){
 		if(hipMalloc((void **)&c->d_Rt0, nt * mat * sizeof(double)) != hipSuccess
 			|| hipMalloc((void **)&c->d_R0t, nt * mat * sizeof(double)) != hipSuccess
-			|| hipMalloc((void **)&c->d_ccg, nt * 9 * sizeof(double)) != hipSuccess){
+			|| hipMalloc((void **)&c->d_ccg, (nt * 9 + cc_partials(ns, nt)) * sizeof(double)) != hipSuccess){
 			(void)hipGetLastError();
 			set_error("correlator buffers (%zu MB) could not be allocated", 2 * nt * mat * 8 >> 20);
 			return EPHMC_ERR_ALLOC;
@@ -112,18 +134,14 @@ extern "C" int ephmc_cc_correlator(ephmc_ctx *c, const double *x, const double *
 	// R_0t: transposed inverse, then slices 0 .. nt-1 ascending; stored entry tau = state at tau
 	if(!ident) EPHMC_TRY(transpose_plus_identity(c, EPHMC_FP64, minv, c->d_gj[0], 0));
 	EPHMC_TRY(chain_forward(c, EPHMC_FP64, c->d_gj[0], c->d_gj[1], c->d_R0t, 1, 0, ident));
-	const unsigned bx = (ns + CC_BT * CC_ROWS - 1) / (CC_BT * CC_ROWS);
-	const dim3 grid(bx, ns);
-	double *partial = c->d_gj[1];                   // ns * bx * 9 doubles <= N^2
-	for(unsigned tau = 0; tau < nt; tau++){
-		const double *rt = (tau == 0) ? c->d_Tfin : c->d_Rt0 + (size_t)(nt - tau) * mat;
-		cc_contract_kernel<<<grid, CC_BT, 0, c->stream>>>(rt, c->d_R0t + (size_t)tau * mat, x, c->d_nb,
-			c->cfg.t[0], c->cfg.t[1], c->cfg.t[2], c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], ns, tau, partial);
-		EPHMC_LAUNCHED();
-		cc_sum_kernel<<<1, 64, 0, c->stream>>>(partial, bx * ns, c->cfg.single_particle ? exp(-logdet) : 1.0 / ns,
-			c->d_ccg + (size_t)tau * 9);
-		EPHMC_LAUNCHED();
-	}
+	const unsigned bx = (ns + CC_BT - 1) / CC_BT, by = (ns + CC_J - 1) / CC_J, cj = CC_J;
+	double *partial = c->d_ccg + (size_t)nt * 9;     // cc_partials(ns, nt) block sums
+	cc_contract_kernel<<<dim3(bx, by, nt), CC_BT, 0, c->stream>>>(c->d_Tfin, c->d_Rt0, c->d_R0t, x, c->d_nb,
+		c->cfg.t[0], c->cfg.t[1], c->cfg.t[2], c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], ns, nt, cj, partial);
+	EPHMC_LAUNCHED();
+	cc_sum_kernel<<<(nt * 9 + 255) / 256, 256, 0, c->stream>>>(partial, bx * by, nt, c->cfg.single_particle ? exp(-logdet) : 1.0 / ns,
+		c->d_ccg);
+	EPHMC_LAUNCHED();
 	EPHMC_TRY(timer_end(c, "correlator"));
 	EPHMC_HIP(hipMemcpyAsync(greens_host, c->d_ccg, nt * 9 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
 	EPHMC_HIP(hipStreamSynchronize(c->stream));
