@@ -77,7 +77,16 @@ __device__ void reduce_links(const P *__restrict__ part, unsigned nwg, unsigned 
 				const unsigned tl = lk / loc, link = lk - tl * loc;
 				const unsigned w0 = (unsigned)((size_t)nwg * ch / CH), w1 = (unsigned)((size_t)nwg * (ch + 1) / CH);
 				const P *src = part + (size_t)tl * nwg * loc + link;
-				for(unsigned w = w0; w < w1; w++) s += (double)src[(size_t)w * loc];
+				// 16 loads in flight, then added in w order (the same sum as one at a time)
+				unsigned w = w0;
+				for(; w + 16 <= w1; w += 16){
+					P v[16];
+#pragma unroll
+					for(int u = 0; u < 16; u++) v[u] = src[(size_t)(w + u) * loc];
+#pragma unroll
+					for(int u = 0; u < 16; u++) s += (double)v[u];
+				}
+				for(; w < w1; w++) s += (double)src[(size_t)w * loc];
 			}
 			scratch[ch * G + ll] = s;
 		}
