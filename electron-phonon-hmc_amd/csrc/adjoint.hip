@@ -290,8 +290,11 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			for(int j = 0; j < S; j++)
 #pragma unroll
 				for(int c = 0; c < C; c++) sa[q][j][c] = (q < deg) ? a.cm[q + 1] * t[j][c] : (R)0;
-		for(int b = 1; b < deg; b++){
-			R *bb = bufs[(b - 1) & 1];
+		// unrolled over the degree bound: b is a constant, so bb is a known LDS buffer
+#pragma unroll
+		for(int b = 1; b < MAXD; b++){
+			if(b >= deg) break;
+			R *bb = ((b - 1) & 1) ? buf1 : buf0;
 #pragma unroll
 			for(int j = 0; j < S; j++)
 				if(act[j])
@@ -349,19 +352,21 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			for(int j = 0; j < S; j++){
 				if(!act[j]) continue;
 				R wn[C];
+				// all six neighbour reads are issued before any of them is used
+				R nv[6][C];
+#pragma unroll
+				for(int d = 0; d < 6; d++)
+#pragma unroll
+					for(int c = 0; c < C; c++) nv[d][c] = wb[(size_t)nbr[j][d] * C + c];
 #pragma unroll
 				for(int c = 0; c < C; c++) wn[c] = a.mu * wa[j][c];
 #pragma unroll
 				for(int d = 0; d < 6; d++){
-					const R *qn = wb + (size_t)nbr[j][d] * C;
-					R nv[C];
-#pragma unroll
-					for(int c = 0; c < C; c++) nv[c] = qn[c];
 					R hs = 0;
 #pragma unroll
 					for(int c = 0; c < C; c++){
-						hs = fmac(sa[q][j][c], nv[c], hs);
-						wn[c] = fmac(cf[j][d], nv[c], wn[c]);
+						hs = fmac(sa[q][j][c], nv[d][c], hs);
+						wn[c] = fmac(cf[j][d], nv[d][c], wn[c]);
 					}
 					h[j][d] += hs;
 				}
