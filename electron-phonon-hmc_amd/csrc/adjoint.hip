@@ -111,14 +111,18 @@ template <typename R, int C>
 __device__ __forceinline__ void gather6(const R *__restrict__ buf, const int (&nbr)[6], const R (&cf)[6], R mu,
 		const R (&own)[C], R (&out)[C])
 {
+	// the six neighbour reads are issued together; the sum is still taken in the order d = 0..5
+	R nv[6][C];
+#pragma unroll
+	for(int d = 0; d < 6; d++)
+#pragma unroll
+		for(int c = 0; c < C; c++) nv[d][c] = buf[(size_t)nbr[d] * C + c];
 #pragma unroll
 	for(int c = 0; c < C; c++) out[c] = mu * own[c];
 #pragma unroll
-	for(int d = 0; d < 6; d++){
-		const R *q = buf + (size_t)nbr[d] * C;
+	for(int d = 0; d < 6; d++)
 #pragma unroll
-		for(int c = 0; c < C; c++) out[c] = fmac(cf[d], q[c], out[c]);
-	}
+		for(int c = 0; c < C; c++) out[c] = fmac(cf[d], nv[d][c], out[c]);
 }
 
 template <typename R, int C, int BT, int S, int MAXD>

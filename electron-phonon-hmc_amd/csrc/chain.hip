@@ -67,14 +67,18 @@ template <typename R, int C>
 __device__ __forceinline__ void stencil_gather(const R *__restrict__ buf, const int (&nbr)[6], const R (&cf)[6],
 		R mu, const R (&own)[C], R (&out)[C])
 {
+	// the six neighbour reads are issued together; the sum is still taken in the order d = 0..5
+	R nv[6][C];
+#pragma unroll
+	for(int d = 0; d < 6; d++)
+#pragma unroll
+		for(int c = 0; c < C; c++) nv[d][c] = buf[(size_t)nbr[d] * C + c];
 #pragma unroll
 	for(int c = 0; c < C; c++) out[c] = mu * own[c];
 #pragma unroll
-	for(int d = 0; d < 6; d++){
-		const R *p = buf + (size_t)nbr[d] * C;
+	for(int d = 0; d < 6; d++)
 #pragma unroll
-		for(int c = 0; c < C; c++) out[c] = fmac(cf[d], p[c], out[c]);
-	}
+		for(int c = 0; c < C; c++) out[c] = fmac(cf[d], nv[d][c], out[c]);
 }
 
 template <typename R, int C, int BT, int S>
@@ -132,7 +136,9 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 			for(int d = 0; d < 6; d++) cf[j][d] = cfn[j][d];
 		if(i + 1 < a.ntau) load_cf(i + 1);
 
-		for(int p = 0; p < a.np; p++){
+#pragma unroll
+		for(int p = 0; p < MAX_PAIRS; p++){
+			if(p >= a.np) break;
 #pragma unroll
 			for(int j = 0; j < S; j++)
 				if(act[j])
