@@ -111,18 +111,28 @@ template <typename R, int C>
 __device__ __forceinline__ void gather6(const R *__restrict__ buf, const int (&nbr)[6], const R (&cf)[6], R mu,
 		const R (&own)[C], R (&out)[C])
 {
-	// the six neighbour reads are issued together; the sum is still taken in the order d = 0..5
-	R nv[6][C];
-#pragma unroll
-	for(int d = 0; d < 6; d++)
-#pragma unroll
-		for(int c = 0; c < C; c++) nv[d][c] = buf[(size_t)nbr[d] * C + c];
 #pragma unroll
 	for(int c = 0; c < C; c++) out[c] = mu * own[c];
+	if constexpr(sizeof(R) == 4){
+		// fp32: the six neighbour reads are issued together (fp64 keeps them interleaved with the
+		// sums, which holds fewer registers); the sum is taken in the order d = 0..5 either way
+		R nv[6][C];
 #pragma unroll
-	for(int d = 0; d < 6; d++)
+		for(int d = 0; d < 6; d++)
 #pragma unroll
-		for(int c = 0; c < C; c++) out[c] = fmac(cf[d], nv[d][c], out[c]);
+			for(int c = 0; c < C; c++) nv[d][c] = buf[(size_t)nbr[d] * C + c];
+#pragma unroll
+		for(int d = 0; d < 6; d++)
+#pragma unroll
+			for(int c = 0; c < C; c++) out[c] = fmac(cf[d], nv[d][c], out[c]);
+	}else{
+#pragma unroll
+		for(int d = 0; d < 6; d++){
+			const R *q = buf + (size_t)nbr[d] * C;
+#pragma unroll
+			for(int c = 0; c < C; c++) out[c] = fmac(cf[d], q[c], out[c]);
+		}
+	}
 }
 
 template <typename R, int C, int BT, int S, int MAXD>
@@ -356,21 +366,30 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			for(int j = 0; j < S; j++){
 				if(!act[j]) continue;
 				R wn[C];
-				// all six neighbour reads are issued before any of them is used
-				R nv[6][C];
+				// fp32: all six neighbour reads are issued before any of them is used (fp64 reads
+				// each just before its sums, which holds fewer registers; same arithmetic)
+				constexpr int NV = sizeof(R) == 4 ? 6 : 1;
+				R nv[NV][C];
+				if constexpr(NV == 6){
 #pragma unroll
-				for(int d = 0; d < 6; d++)
+					for(int d = 0; d < 6; d++)
 #pragma unroll
-					for(int c = 0; c < C; c++) nv[d][c] = wb[(size_t)nbr[j][d] * C + c];
+						for(int c = 0; c < C; c++) nv[d][c] = wb[(size_t)nbr[j][d] * C + c];
+				}
 #pragma unroll
 				for(int c = 0; c < C; c++) wn[c] = a.mu * wa[j][c];
 #pragma unroll
 				for(int d = 0; d < 6; d++){
+					const int dv = NV == 6 ? d : 0;
+					if constexpr(NV == 1){
+#pragma unroll
+						for(int c = 0; c < C; c++) nv[0][c] = wb[(size_t)nbr[j][d] * C + c];
+					}
 					R hs = 0;
 #pragma unroll
 					for(int c = 0; c < C; c++){
-						hs = fmac(sa[q][j][c], nv[d][c], hs);
-						wn[c] = fmac(cf[j][d], nv[d][c], wn[c]);
+						hs = fmac(sa[q][j][c], nv[dv][c], hs);
+						wn[c] = fmac(cf[j][d], nv[dv][c], wn[c]);
 					}
 					h[j][d] += hs;
 				}

@@ -67,18 +67,28 @@ template <typename R, int C>
 __device__ __forceinline__ void stencil_gather(const R *__restrict__ buf, const int (&nbr)[6], const R (&cf)[6],
 		R mu, const R (&own)[C], R (&out)[C])
 {
-	// the six neighbour reads are issued together; the sum is still taken in the order d = 0..5
-	R nv[6][C];
-#pragma unroll
-	for(int d = 0; d < 6; d++)
-#pragma unroll
-		for(int c = 0; c < C; c++) nv[d][c] = buf[(size_t)nbr[d] * C + c];
 #pragma unroll
 	for(int c = 0; c < C; c++) out[c] = mu * own[c];
+	if constexpr(sizeof(R) == 4){
+		// fp32: the six neighbour reads are issued together (fp64 keeps them interleaved with the
+		// sums, which holds fewer registers); the sum is taken in the order d = 0..5 either way
+		R nv[6][C];
 #pragma unroll
-	for(int d = 0; d < 6; d++)
+		for(int d = 0; d < 6; d++)
 #pragma unroll
-		for(int c = 0; c < C; c++) out[c] = fmac(cf[d], nv[d][c], out[c]);
+			for(int c = 0; c < C; c++) nv[d][c] = buf[(size_t)nbr[d] * C + c];
+#pragma unroll
+		for(int d = 0; d < 6; d++)
+#pragma unroll
+			for(int c = 0; c < C; c++) out[c] = fmac(cf[d], nv[d][c], out[c]);
+	}else{
+#pragma unroll
+		for(int d = 0; d < 6; d++){
+			const R *q = buf + (size_t)nbr[d] * C;
+#pragma unroll
+			for(int c = 0; c < C; c++) out[c] = fmac(cf[d], q[c], out[c]);
+		}
+	}
 }
 
 template <typename R, int C, int BT, int S>
@@ -262,13 +272,16 @@ __global__ __launch_bounds__(LB) void chain_strip_kernel(ChainArgs<R> a)
 	}
 }
 
-// Strip width for this lattice (0: site-per-thread form).  The strip form halves the LDS reads
-// but its register footprint leaves one wave per SIMD; measured at 32x32 it is slower, so it is
-// off unless EPHMC_CHAIN_STRIP=2/4 asks for it.
-unsigned strip_width(const ephmc_ctx *c)
+// Strip width for this lattice and arithmetic (0: site-per-thread form): the width chosen by
+// ephmc_autotune, else EPHMC_CHAIN_STRIP=2/4, else 0.  The strip form cuts the LDS reads by a
+// third (M = 2) at a larger register footprint; at 32x32 it is the faster fp64 chain (0.55
+// against 0.77 ms per 128 slices) and the slower fp32 one (0.34 against 0.31 ms).
+unsigned strip_width(const ephmc_ctx *c, int prec)
 {
+	const int slot = prec == EPHMC_FP64 ? 0 : 1;
 	unsigned m = 0;
-	if(const char *e = getenv("EPHMC_CHAIN_STRIP")) m = (unsigned)atoi(e);
+	if(c->chain_strip_set[slot] >= 0) m = (unsigned)c->chain_strip_set[slot];
+	else if(const char *e = getenv("EPHMC_CHAIN_STRIP")) m = (unsigned)atoi(e);
 	if(m != 2 && m != 4) return 0;
 	if(c->l1 % m != 0) m = (m == 4 && c->l1 % 2 == 0) ? 2 : 0;
 	if(m == 0 || c->ns / m > 1024) return 0;
@@ -336,7 +349,7 @@ static int launch_chain(ephmc_ctx *c, const ChainArgs<R> &a, unsigned cols)
 {
 	// LDS budget: 2 * C * ns * sizeof(R) <= 160 KiB
 	while(cols > 1 && 2 * (size_t)cols * a.ns * sizeof(R) > 160 * 1024) cols >>= 1;
-	const unsigned m = strip_width(c);
+	const unsigned m = strip_width(c, sizeof(R) == 8 ? EPHMC_FP64 : EPHMC_MIXED);
 	if(m == 4){
 		switch(cols){
 			case 1: return launch_strip<R, 1, 4>(c, a);
@@ -383,7 +396,8 @@ unsigned chain_cols(const ephmc_ctx *c, int prec)
 	// columns per workgroup of the forward chain: enough workgroups to cover 256 CUs;
 	// EPHMC_CHAIN_COLS overrides it for tuning sweeps
 	const unsigned ns = c->ns;
-	if(c->chain_cols_set) return c->chain_cols_set;
+	const int slot = prec == EPHMC_FP64 ? 0 : 1;
+	if(c->chain_cols_set[slot]) return c->chain_cols_set[slot];
 	if(const char *e = getenv("EPHMC_CHAIN_COLS")){
 		const unsigned v = (unsigned)atoi(e);
 		if(v == 1 || v == 2 || v == 4 || v == 8) return v;

@@ -323,6 +323,7 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 	if(ns > 2048){ set_error("lattice of %u sites exceeds 2048", ns); return EPHMC_ERR_UNSUPPORTED; }
 	ephmc_ctx *c = new ephmc_ctx();
 	c->cfg = *cfg;
+	c->chain_strip_set[0] = c->chain_strip_set[1] = -1;
 	c->l1 = cfg->l1; c->l2 = cfg->l2; c->ns = ns; c->nt = cfg->nt; c->loc = 3 * ns;
 	c->dim = (size_t)c->nt * c->loc;
 	c->mat = (size_t)ns * ns;
@@ -815,25 +816,49 @@ int ephmc_autotune(ephmc_ctx *c, ephmc_tune_result *res)
 	int rc = EPHMC_OK;
 	ephmc_tune_result r{};
 	void *tfin = prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32;
-	auto chain = [&]() -> int {
+	// the molecular-dynamics chain (context precision, stores its checkpoints) and, in mixed
+	// precision, the fp64 chain of the Metropolis log det (no checkpoints)
+	auto md_chain = [&]() -> int {
 		EPHMC_TRY(coefficients(c, c->d_xnew, prec != EPHMC_FP64));
 		return chain_forward(c, prec, nullptr, tfin, c->d_ck, c->ckpt, 0, true);
 	};
+	auto ld_chain = [&]() -> int {
+		EPHMC_TRY(coefficients(c, c->d_xnew, false));
+		return chain_forward(c, EPHMC_FP64, nullptr, c->d_Tfin, nullptr, 0, 1, true);
+	};
+	// widths: 2 C N values of LDS; strips of M sites need l1 % M == 0 and N / M <= 1024 threads
+	auto tune_chain = [&](int cprec, auto &&run, unsigned &cols_out, unsigned &strip_out, double &ms_out) -> int {
+		const int slot = cprec == EPHMC_FP64 ? 0 : 1;
+		const size_t csz = cprec == EPHMC_FP64 ? 8 : 4;
+		ms_out = 1e30;
+		for(unsigned M : {0u, 2u, 4u}){
+			if(M && (c->l1 % M != 0 || ns / M > 1024)) continue;
+			for(unsigned C : {1u, 2u, 4u, 8u}){
+				if(2 * (size_t)C * ns * csz > 160 * 1024 || C > ns) continue;
+				c->chain_cols_set[slot] = C;
+				c->chain_strip_set[slot] = (int)M;
+				double ms;
+				EPHMC_TRY(time_ms(c, e0, e1, 3, &ms, run));
+				if(ms < ms_out){ ms_out = ms; cols_out = C; strip_out = M; }
+			}
+		}
+		c->chain_cols_set[slot] = cols_out;
+		c->chain_strip_set[slot] = (int)strip_out;
+		return EPHMC_OK;
+	};
 	do {
 		if((rc = set_zero(c, c->d_xnew, c->dim)) != EPHMC_OK) break;
-		if((rc = set_zero(c, c->d_mhat, c->mat)) != EPHMC_OK || (rc = add_identity(c, c->d_mhat)) != EPHMC_OK) break;
-		// forward chain widths: 2 C N values of LDS
-		r.chain_ms = 1e30;
-		for(unsigned C : {1u, 2u, 4u, 8u}){
-			if(2 * (size_t)C * ns * rsz > 160 * 1024 || C > ns) continue;
-			c->chain_cols_set = C;
-			double ms;
-			if((rc = time_ms(c, e0, e1, 3, &ms, chain)) != EPHMC_OK) break;
-			if(ms < r.chain_ms){ r.chain_ms = ms; r.chain_cols = C; }
+		if((rc = tune_chain(prec, md_chain, r.chain_cols, r.chain_strip, r.chain_ms)) != EPHMC_OK) break;
+		if(prec != EPHMC_FP64){
+			if((rc = tune_chain(EPHMC_FP64, ld_chain, r.chain64_cols, r.chain64_strip, r.chain64_ms)) != EPHMC_OK) break;
+		}else{
+			r.chain64_cols = r.chain_cols;
+			r.chain64_strip = r.chain_strip;
+			r.chain64_ms = r.chain_ms;
 		}
-		if(rc != EPHMC_OK) break;
-		c->chain_cols_set = r.chain_cols;
-		if((rc = chain()) != EPHMC_OK) break;
+		// the force sweep below runs on the chain's checkpoints and M_hat^{-1} = 1
+		if((rc = md_chain()) != EPHMC_OK) break;
+		if((rc = set_zero(c, c->d_mhat, c->mat)) != EPHMC_OK || (rc = add_identity(c, c->d_mhat)) != EPHMC_OK) break;
 		// force sweep widths: (4 C + 3) N values of LDS plus the reduction scratch; both orders of
 		// the first W phase (its own barrier, or W_0 written ahead)
 		r.adjoint_ms = 1e30;

@@ -109,7 +109,10 @@ struct ephmc_ctx {
 	unsigned span;                    // slices per launch of the force sweep (k, or 32 when k = 1)
 	unsigned cols;                    // columns per workgroup of the force sweep
 	unsigned nwg;                     // ceil(ns / cols)
-	unsigned chain_cols_set;          // columns per workgroup of the forward chain chosen by ephmc_autotune (0: rule)
+	// forward-chain launch shape chosen by ephmc_autotune, per arithmetic ([0] fp64, [1] fp32):
+	// columns per workgroup (0: rule) and sites per thread of a lattice row (-1: rule, 0: one)
+	unsigned chain_cols_set[2];
+	int chain_strip_set[2];
 	int adj_w0_early;                 // force sweep: W_0 of a slice written ahead (one barrier fewer per slice)
 	size_t partial_cap;               // doubles in each d_partial buffer
 	void *d_ck;                       // checkpoints T_{s k}: nseg matrices (R = fp64 or fp32)
@@ -182,6 +185,7 @@ int timer_end(ephmc_ctx *c, const char *group);
 
 // chain.hip
 unsigned sites_per_thread(const char *var);
+unsigned strip_width(const ephmc_ctx *c, int prec);
 int coefficients(ephmc_ctx *c, const double *x, bool want32);
 int chain_forward(ephmc_ctx *c, int prec, const void *src, void *dst, void *ck, unsigned ck_every,
 		int descending, bool from_identity);

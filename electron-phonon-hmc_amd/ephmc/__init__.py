@@ -32,7 +32,8 @@ class Config(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("chain_cols", C.c_uint), ("adjoint_cols", C.c_uint), ("chain_ms", C.c_double), ("adjoint_ms", C.c_double),
-                ("adjoint_w0_early", C.c_int)]
+                ("adjoint_w0_early", C.c_int), ("chain_strip", C.c_uint), ("chain64_cols", C.c_uint),
+                ("chain64_strip", C.c_uint), ("chain64_ms", C.c_double)]
 
 
 class Record(C.Structure):
@@ -317,11 +318,13 @@ class Context:
 
     def autotune(self):
         """Launch-shape tuning of this context (ephmc_autotune): returns the chosen columns per
-        workgroup of the forward chain and the force sweep and their per-call times (ms)."""
+        workgroup (and strip width) of the forward chain and the force sweep, the fp64 chain of
+        the Metropolis log det, and their per-call times (ms)."""
         r = TuneResult()
         check(LIB.ephmc_autotune(self.h, C.byref(r)))
-        return {"chain_cols": r.chain_cols, "adjoint_cols": r.adjoint_cols, "chain_ms": r.chain_ms,
-                "adjoint_ms": r.adjoint_ms, "adjoint_w0_early": r.adjoint_w0_early}
+        return {"chain_cols": r.chain_cols, "chain_strip": r.chain_strip, "adjoint_cols": r.adjoint_cols,
+                "chain_ms": r.chain_ms, "adjoint_ms": r.adjoint_ms, "adjoint_w0_early": r.adjoint_w0_early,
+                "chain64_cols": r.chain64_cols, "chain64_strip": r.chain64_strip, "chain64_ms": r.chain64_ms}
 
     def timing(self, enable=True):
         check(LIB.ephmc_timing(self.h, int(enable)))

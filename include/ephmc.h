@@ -166,14 +166,19 @@ int ephmc_tl_hist(ephmc_ctx *ctx, const double *normals_host, unsigned nsamples,
 
 /* ---- launch-shape tuning ----------------------------------------------------------------------
  * No counterpart in the reference (its shapes are fixed by the CPU loops).  Times the forward
- * chain and the force sweep of this context's size and precision for every columns-per-
- * workgroup width that fits LDS, keeps the fastest of each for the context's later calls, and
+ * chain (columns per workgroup x sites per thread of a lattice row) and the force sweep (columns
+ * per workgroup) of this context's size and precision for every shape that fits LDS, and in
+ * mixed precision also the fp64 chain of the Metropolis log det; keeps the fastest of each for
+ * the context's later calls, and
  * drops a captured trajectory graph so that it is captured again.  Waits; call it between
  * trajectories.  res (may be null) receives the choice and its per-call times. */
 typedef struct ephmc_tune_result {
 	unsigned chain_cols, adjoint_cols;
 	double chain_ms, adjoint_ms;
 	int adjoint_w0_early;    /* 1: the sweep writes each slice's W_0 ahead of its first phase */
+	unsigned chain_strip;    /* sites per thread of a lattice row in the chain (0: one site) */
+	unsigned chain64_cols, chain64_strip;   /* fp64 chain of the Metropolis log det (mixed precision; */
+	double chain64_ms;                      /* in fp64 contexts the same as chain_*)               */
 } ephmc_tune_result;
 int ephmc_autotune(ephmc_ctx *ctx, ephmc_tune_result *res);
 

@@ -18,9 +18,16 @@ def test_autotune_keeps_force(l1, l2, nt, k, precision):
     x = np.random.default_rng(l1 + nt).standard_normal(om.dim) * 0.5
     ld, _, Minv = om.logdet_inv(om.M_hat(x))
     res = ctx.autotune()
+    # the fp64 log det after tuning (the Metropolis chain's shape) against the oracle
+    ld_t, _, _ = ctx.logdet_inv(ctx.M_hat(x))
+    assert abs(ld_t - ld) <= 1e-10 * max(1.0, abs(ld))
     say("autotune", l1, l2, nt, k, precision, res)
     assert res["chain_cols"] in (1, 2, 4, 8) and res["adjoint_cols"] in (1, 2, 4)
-    assert res["chain_ms"] > 0 and res["adjoint_ms"] > 0
+    assert res["chain_strip"] in (0, 2, 4) and res["chain64_strip"] in (0, 2, 4)
+    assert res["chain_strip"] == 0 or l1 % res["chain_strip"] == 0
+    assert res["chain_ms"] > 0 and res["adjoint_ms"] > 0 and res["chain64_ms"] > 0
+    if precision == 0:
+        assert (res["chain64_cols"], res["chain64_strip"]) == (res["chain_cols"], res["chain_strip"])
     assert rel(ctx.force(x, Minv, ld), om.force(x, Minv, ld)) < (1e-10 if precision == 0 else 1e-5)
 
 
