@@ -273,13 +273,14 @@ __global__ __launch_bounds__(LB) void chain_strip_kernel(ChainArgs<R> a)
 }
 
 // Strip width for this lattice and arithmetic (0: site-per-thread form): the width chosen by
-// ephmc_autotune, else EPHMC_CHAIN_STRIP=2/4, else 0.  The strip form cuts the LDS reads by a
-// third (M = 2) at a larger register footprint; at 32x32 it is the faster fp64 chain (0.55
-// against 0.77 ms per 128 slices) and the slower fp32 one (0.34 against 0.31 ms).
+// ephmc_autotune, else EPHMC_CHAIN_STRIP=0/2/4, else 2 in fp64 and 0 in fp32.  The strip form
+// cuts the LDS reads by a third (M = 2) at a larger register footprint; at 32x32 it is the
+// faster fp64 chain (0.53 against 0.76 ms per 128 slices) and the slower fp32 one (0.32 against
+// 0.31 ms).
 unsigned strip_width(const ephmc_ctx *c, int prec)
 {
 	const int slot = prec == EPHMC_FP64 ? 0 : 1;
-	unsigned m = 0;
+	unsigned m = prec == EPHMC_FP64 ? 2 : 0;
 	if(c->chain_strip_set[slot] >= 0) m = (unsigned)c->chain_strip_set[slot];
 	else if(const char *e = getenv("EPHMC_CHAIN_STRIP")) m = (unsigned)atoi(e);
 	if(m != 2 && m != 4) return 0;
@@ -393,8 +394,8 @@ static ChainArgs<R> make_args(ephmc_ctx *c, const R *coef)
 
 unsigned chain_cols(const ephmc_ctx *c, int prec)
 {
-	// columns per workgroup of the forward chain: enough workgroups to cover 256 CUs;
-	// EPHMC_CHAIN_COLS overrides it for tuning sweeps
+	// columns per workgroup of the forward chain: the widest that gives at least 512 workgroups
+	// (two per CU); EPHMC_CHAIN_COLS overrides it for tuning sweeps
 	const unsigned ns = c->ns;
 	const int slot = prec == EPHMC_FP64 ? 0 : 1;
 	if(c->chain_cols_set[slot]) return c->chain_cols_set[slot];
@@ -402,8 +403,8 @@ unsigned chain_cols(const ephmc_ctx *c, int prec)
 		const unsigned v = (unsigned)atoi(e);
 		if(v == 1 || v == 2 || v == 4 || v == 8) return v;
 	}
-	unsigned cols = prec == EPHMC_FP64 ? 4 : 8;
-	while(cols > 1 && (ns + cols - 1) / cols < 256) cols >>= 1;
+	unsigned cols = 8;
+	while(cols > 1 && (ns + cols - 1) / cols < 512) cols >>= 1;
 	return cols;
 }
 

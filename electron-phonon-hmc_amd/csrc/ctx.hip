@@ -145,15 +145,15 @@ int timer_end(ephmc_ctx *c, const char *group)
 }
 
 // Columns per workgroup of the adjoint sweep: the widest that fits LDS and keeps at least one
-// workgroup per CU, at most 2 in fp64 (4 fp64 columns spill registers: 0.43 against 0.30 ms per
-// 32-slice launch at 32x32).  EPHMC_ADJOINT_COLS overrides it for tuning sweeps.
+// workgroup per CU (at 32x32: 4 columns, 0.14 ms per 32-slice launch in fp32 and 0.32 ms in
+// fp64).  EPHMC_ADJOINT_COLS overrides it for tuning sweeps.
 static unsigned adjoint_cols_auto(unsigned ns, size_t rsize)
 {
 	if(const char *e = getenv("EPHMC_ADJOINT_COLS")){
 		const unsigned v = (unsigned)atoi(e);
 		if(v == 1 || v == 2 || v == 4) return v;
 	}
-	for(unsigned C = rsize == 8 ? 2 : 4; C > 1; C >>= 1){
+	for(unsigned C = 4; C > 1; C >>= 1){
 		const size_t lds = (4 * (size_t)C * ns + 3 * (size_t)ns) * rsize;
 		if(lds <= 160 * 1024 && (ns + C - 1) / C >= 256) return C;
 	}

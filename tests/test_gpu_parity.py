@@ -46,15 +46,16 @@ def test_gj_trailing_blocks(n):
     assert ctx.gj_fallbacks() == 0
 
 
+@pytest.mark.parametrize("strip", ["2", "4"])
 @pytest.mark.parametrize("l1,l2,nt", [(2, 3, 4), (6, 6, 8), (16, 16, 8), (32, 32, 4)])
-def test_strip_chain_bit_identical(monkeypatch, l1, l2, nt):
+def test_strip_chain_bit_identical(monkeypatch, l1, l2, nt, strip):
     """The strip form of the slice stencil sums in the same order as the site-per-thread form:
     M_hat must agree to the last bit (EPHMC_CHAIN_STRIP=0 selects the site-per-thread form)."""
-    say("strip vs site chain", l1, l2, nt)
+    say("strip vs site chain", l1, l2, nt, strip)
     om = OracleModel(l1, l2, nt, beta=nt * 40 / 32)
     ctx = gpu_context(om)
     x = field(om, 5)
-    monkeypatch.setenv("EPHMC_CHAIN_STRIP", "4")
+    monkeypatch.setenv("EPHMC_CHAIN_STRIP", strip)
     Ms = ctx.M_hat(x)
     monkeypatch.setenv("EPHMC_CHAIN_STRIP", "0")
     Mg = ctx.M_hat(x)
