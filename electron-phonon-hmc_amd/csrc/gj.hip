@@ -42,94 +42,190 @@ __device__ inline double rcp_nr(double x)
 }
 
 // Diagonal-block inversion.  D^{-1} by Gauss-Jordan on [D | 1] (32 x 64) without exchanges,
-// two pivots (g, g+1) per step eliminated together through their 2 x 2 block
-// P = [[a_gg, a_g,g+1], [a_g+1,g, a_g+1,g+1]]: pivot rows <- P^{-1} pivot rows, every other row
-// r -= [a_rg a_r,g+1] (new pivot rows).  Wave w holds rows 8w..8w+7 in a[], lane c column c
-// (lanes 32..63 end with D^{-1}); one barrier per step publishes the two pivot rows and the two
-// pivot columns.  The pivot columns of every step are kept, and the LU multipliers of D (and the
-// pivots U_gg) are formed from them afterwards: l[r][g] = a_rg / a_gg,
-// U_hh = a_hh - l[h][g] a_gh, l[r][h] = (a_rh - l[r][g] a_gh) / U_hh.
+// four pivots g..g+3 per step eliminated together through their 4 x 4 block P (rows and
+// columns g..g+3): pivot rows <- P^{-1} pivot rows, every other row r -= a_r,[g..g+3] (new pivot
+// rows).  Wave w holds rows 8w..8w+7 in a[], lane c column c (lanes 32..63 end with D^{-1}); one
+// barrier per step publishes the four pivot rows and the four pivot columns.  The pivot columns
+// of every step are kept, and the LU multipliers of D (and the pivots U_gg) are formed from them
+// afterwards: with P = L_P U_P (Doolittle), row r's multipliers of the step are
+// l[r][g+i] = (a_r,g+i - sum_{q<i} l[r][g+q] U_P[q][i]) / U_P[i][i] (the division by a refined
+// reciprocal; the multipliers only feed the |l| <= 1 check).
 // Writes dinv_out[r][c] = D^{-1}[r][c] and lm_out[m][j] = L[m][j] (unit lower triangular, zero
 // elsewhere, NB x NB row-major), the pivots of rows kb.. to diag/piv, and raises the flag when a
 // multiplier exceeds 1.  Called by all 256 threads of a workgroup.
 struct GjInvLds {
-	__align__(16) double prow[2][2][2 * GJ_NB];     // pivot rows of the step, by step parity
-	__align__(16) double pcs[GJ_NB / 2][2][GJ_NB];  // pivot columns of every step
+	__align__(16) double prow[2][4][2 * GJ_NB];     // pivot rows of the step, by step parity
+	__align__(16) double pcs[GJ_NB / 4][4][GJ_NB];  // pivot columns of every step
+	double ublk[GJ_NB / 4][4][4];                   // U of every step's pivot block, 1 / U_qq on its diagonal
 };
+
+// P[m][i] = row g+m, column g+i of the pivot block of step s
+__device__ __forceinline__ void gj_pivot_block(const GjInvLds &sh, int s, double (&P)[4][4])
+{
+	const int g = 4 * s;
+#pragma unroll
+	for(int i = 0; i < 4; i++){
+		const double2 x0 = *(const double2 *)&sh.pcs[s][i][g], x1 = *(const double2 *)&sh.pcs[s][i][g + 2];
+		P[0][i] = x0.x;
+		P[1][i] = x0.y;
+		P[2][i] = x1.x;
+		P[3][i] = x1.y;
+	}
+}
+
+// Q = P^{-1} by Gauss-Jordan without exchanges on the 4 x 4 block
+__device__ __forceinline__ void gj_inv4(const double (&P)[4][4], double (&Q)[4][4])
+{
+	double M[4][4];
+#pragma unroll
+	for(int m = 0; m < 4; m++)
+#pragma unroll
+		for(int i = 0; i < 4; i++){
+			M[m][i] = P[m][i];
+			Q[m][i] = m == i ? 1.0 : 0.0;
+		}
+#pragma unroll
+	for(int k = 0; k < 4; k++){
+		const double r = rcp_nr(M[k][k]);
+#pragma unroll
+		for(int i = 0; i < 4; i++){
+			M[k][i] *= r;
+			Q[k][i] *= r;
+		}
+#pragma unroll
+		for(int m = 0; m < 4; m++){
+			if(m == k) continue;
+			const double f = M[m][k];
+#pragma unroll
+			for(int i = 0; i < 4; i++){
+				M[m][i] = fmac(-f, M[k][i], M[m][i]);
+				Q[m][i] = fmac(-f, Q[k][i], Q[m][i]);
+			}
+		}
+	}
+}
+
+// U of P = L_P U_P (Doolittle, no exchanges): U[q][i] for q <= i
+__device__ __forceinline__ void gj_upper4(const double (&P)[4][4], double (&U)[4][4])
+{
+	double L[4][4];
+#pragma unroll
+	for(int q = 0; q < 4; q++){
+#pragma unroll
+		for(int i = q; i < 4; i++){
+			double u = P[q][i];
+#pragma unroll
+			for(int p = 0; p < q; p++) u = fmac(-L[q][p], U[p][i], u);
+			U[q][i] = u;
+		}
+#pragma unroll
+		for(int m = q + 1; m < 4; m++){
+			double l = P[m][q];
+#pragma unroll
+			for(int p = 0; p < q; p++) l = fmac(-L[m][p], U[p][q], l);
+			L[m][q] = l / U[q][q];
+		}
+	}
+}
 
 __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, GjInvLds &sh, double *__restrict__ dinv_out,
 		double *__restrict__ lm_out, double *__restrict__ diag, int *__restrict__ piv, unsigned kb, double *__restrict__ scal)
 {
 	constexpr int NB = GJ_NB;
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-	for(int g = 0; g < (int)nbe; g += 2){
-		const int h = g + 1, b = (g >> 1) & 1, gq = g & 7;
+	for(int s = 0; s < NB / 4; s++){
+		const int g = 4 * s, b = s & 1, gq = g & 7;
+		if(g >= (int)nbe) break;
 		if(wave == (g >> 3)){
-			// rows gq, gq + 1 of this wave (selected without indexing a[] at run time)
+			// rows gq .. gq + 3 of this wave (selected without indexing a[] at run time)
 #pragma unroll
-			for(int q = 0; q < 8; q += 2)
+			for(int q = 0; q < 8; q += 4)
 				if(q == gq){
-					sh.prow[b][0][lane] = a[q];
-					sh.prow[b][1][lane] = a[q + 1];
+#pragma unroll
+					for(int i = 0; i < 4; i++) sh.prow[b][i][lane] = a[q + i];
 				}
 		}
-		if(lane == g || lane == h){
-			double *dst = &sh.pcs[g >> 1][lane - g][8 * wave];
+		if(lane >= g && lane < g + 4){
+			double *dst = &sh.pcs[s][lane - g][8 * wave];
 #pragma unroll
 			for(int q = 0; q < 8; q += 2) *(double2 *)(dst + q) = make_double2(a[q], a[q + 1]);
 		}
 		__syncthreads();
-		const double *pg = sh.pcs[g >> 1][0], *ph = sh.pcs[g >> 1][1];
-		const double pgg = pg[g], phg = pg[h], pgh = ph[g], phh = ph[h];
-		const double idet = rcp_nr(fmac(pgg, phh, -pgh * phg));
-		const double u = sh.prow[b][0][lane], v = sh.prow[b][1][lane];
-		const double u2 = fmac(phh, u, -pgh * v) * idet, v2 = fmac(pgg, v, -phg * u) * idet;
-		double cg[8], ch[8];
+		double P[4][4], Q[4][4];
+		gj_pivot_block(sh, s, P);
+		gj_inv4(P, Q);
+		double u[4], nr[4];
 #pragma unroll
-		for(int q = 0; q < 8; q += 2){
-			const double2 x = *(const double2 *)(pg + 8 * wave + q), y = *(const double2 *)(ph + 8 * wave + q);
-			cg[q] = x.x;
-			cg[q + 1] = x.y;
-			ch[q] = y.x;
-			ch[q + 1] = y.y;
+		for(int i = 0; i < 4; i++) u[i] = sh.prow[b][i][lane];
+#pragma unroll
+		for(int m = 0; m < 4; m++){
+			double x = Q[m][0] * u[0];
+#pragma unroll
+			for(int i = 1; i < 4; i++) x = fmac(Q[m][i], u[i], x);
+			nr[m] = x;
 		}
+		double c[8][4];
+#pragma unroll
+		for(int i = 0; i < 4; i++)
+#pragma unroll
+			for(int q = 0; q < 8; q += 2){
+				const double2 x = *(const double2 *)&sh.pcs[s][i][8 * wave + q];
+				c[q][i] = x.x;
+				c[q + 1][i] = x.y;
+			}
 #pragma unroll
 		for(int q = 0; q < 8; q++){
 			const int r = 8 * wave + q;
-			a[q] = r == g ? u2 : (r == h ? v2 : fmac(-ch[q], v2, fmac(-cg[q], u2, a[q])));
+			double x = a[q];
+#pragma unroll
+			for(int i = 0; i < 4; i++) x = fmac(-c[q][i], nr[i], x);
+			a[q] = r == g ? nr[0] : (r == g + 1 ? nr[1] : (r == g + 2 ? nr[2] : (r == g + 3 ? nr[3] : x)));
 		}
 	}
 	if(lane >= NB){
 #pragma unroll
 		for(int q = 0; q < 8; q++) dinv_out[(8 * wave + q) * NB + lane - NB] = a[q];
 	}
-	// L, entry (r, j) by thread tid + 256 i
-	bool bad = false;
+	// U of every step's pivot block and the reciprocals of its diagonal (thread s, in LDS; the
+	// pivots U_gg go to diag), then L, entries (r, j) with r = (tid + 256 i) / NB and j = tid % NB
+	if(tid < NB / 4 && 4 * tid < (int)nbe){
+		const int g = 4 * tid;
+		double P[4][4], U[4][4];
+		gj_pivot_block(sh, tid, P);
+		gj_upper4(P, U);
 #pragma unroll
-	for(int i = 0; i < NB * NB / 256; i++){
-		const int e = tid + 256 * i, r = e / NB, j = e % NB;
-		double l = r == j ? 1.0 : 0.0;
-		if(r > j && r < (int)nbe){
-			const int st = j >> 1, g = 2 * st;
-			const double *pg = sh.pcs[st][0], *ph = sh.pcs[st][1];
-			const double pgg = pg[g], lg = pg[r] / pgg;
-			if(j == g) l = lg;
-			else{
-				const double uhh = ph[j] - (pg[j] / pgg) * ph[g];
-				l = (ph[r] - lg * ph[g]) / uhh;
+		for(int q = 0; q < 4; q++){
+#pragma unroll
+			for(int i = 0; i < 4; i++) sh.ublk[tid][q][i] = q == i ? rcp_nr(U[q][q]) : (q < i ? U[q][i] : 0.0);
+			if(g + q < (int)nbe){
+				diag[kb + g + q] = U[q][q];
+				piv[kb + g + q] = (int)(kb + g + q);
 			}
-			bad |= !(fabs(l) <= 1.0);
 		}
-		lm_out[e] = l;
 	}
-	if(tid < NB / 2 && 2 * tid < (int)nbe){
-		const int g = 2 * tid, h = g + 1;
-		const double *pg = sh.pcs[tid][0], *ph = sh.pcs[tid][1];
-		diag[kb + g] = pg[g];
-		piv[kb + g] = (int)(kb + g);
-		if(h < (int)nbe){
-			diag[kb + h] = ph[h] - (pg[h] / pg[g]) * ph[g];
-			piv[kb + h] = (int)(kb + h);
+	__syncthreads();
+	bool bad = false;
+	{
+		const int j = tid % NB, s = j >> 2, ii = j & 3;
+#pragma unroll
+		for(int i = 0; i < NB * NB / 256; i++){
+			const int e = tid + 256 * i, r = e / NB;
+			double l = r == j ? 1.0 : 0.0;
+			if(r > j && r < (int)nbe){
+				// l[r][g+q] = (a_r,g+q - sum_{p<q} l[r][g+p] U[p][q]) / U[q][q]
+				double lr[4];
+#pragma unroll
+				for(int q = 0; q < 4; q++){
+					if(q > ii) break;
+					double x = sh.pcs[s][q][r];
+#pragma unroll
+					for(int p = 0; p < q; p++) x = fmac(-lr[p], sh.ublk[s][p][q], x);
+					lr[q] = x * sh.ublk[s][q][q];
+				}
+				l = ii == 0 ? lr[0] : (ii == 1 ? lr[1] : (ii == 2 ? lr[2] : lr[3]));
+				bad |= !(fabs(l) <= 1.0);
+			}
+			lm_out[e] = l;
 		}
 	}
 	if(bad) scal[SCAL_GJ_FLAG] = 1;
@@ -159,8 +255,9 @@ __global__ __launch_bounds__(256) void gj_first_kernel(const double *__restrict_
 // Tile workgroups: load their operands (A[K, column block], A[row block, K], the tile), form the
 // transform rows E[i,:] = -A[i,K] D^{-1} (rows inside K: the rows of D^{-1}) as 4 x 2 register
 // blocks into a transposed LDS panel Et[m][row], and update 4 rows (16 apart) x 4 columns each:
-// A'[rows, cols] = [rows not in K] A + E A[K, cols] (columns inside K: E itself).  Workgroups of
-// the first column block check the LU multipliers of their rows below K (l = -E L, |l| <= 1).
+// A'[rows, cols] = [rows not in K] A + E A[K, cols] (columns inside K: E itself).  The tiles of
+// a row block share the check of its LU multipliers below K (l = -E L, |l| <= 1): tile y checks
+// the columns j = y, y + ncb, ... of L.
 //
 // Lookahead workgroup (blockIdx = (row tiles, 0)): loads only the 32 x 32 operands of the next
 // diagonal block D' = A'[K', K'], K' = [k0+nbe, ...), forms it with the same operations in the
@@ -179,7 +276,11 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	__shared__ __align__(16) double Art[NB][GJ_LD];         // A[row block, K], transposed
 	__shared__ __align__(16) double Et[NB][GJ_LD];          // transform rows, transposed
 	__shared__ __align__(16) double Ys[NB][GJ_LD];          // A[K, column block]
-	__shared__ GjInvLds sh;
+	// the lookahead's inversion scratch reuses Et (free once D' is formed), which keeps the
+	// kernel at about 69 KB of LDS: two workgroups per CU, so the 257 workgroups of a 1024 x 1024
+	// step all start at once
+	static_assert(sizeof(GjInvLds) <= sizeof(double) * NB * GJ_LD, "inversion scratch exceeds Et");
+	GjInvLds &sh = *reinterpret_cast<GjInvLds *>(&Et[0][0]);
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 	const unsigned nrb = (n + GJ_RB - 1) / GJ_RB;
 	const unsigned k1 = k0 + nbe;
@@ -238,13 +339,16 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	const unsigned r0 = blockIdx.x * GJ_RB, c0 = blockIdx.y * GJ_CB;
 	// ---- loads: D^{-1}, L (first column block), panels and tile values, all issued before the
 	// first LDS store so that they share one round trip
-	const bool first = blockIdx.y == 0;
+	// the LU multipliers of a row block are checked by all its tiles: tile y takes the columns
+	// j = y, y + ncb, ... of L (lm[m][jj] holds column y + ncb jj)
+	const unsigned ncb = gridDim.y;
 	double dv[NB * NB / 256], lv[NB * NB / 256], ysv[NB * GJ_CB / 256], arv[GJ_RB * NB / 256];
 #pragma unroll
 	for(int i = 0; i < NB * NB / 256; i++){
-		const int e = tid + 256 * i;
+		const int e = tid + 256 * i, m = e % NB, jj = e / NB;
+		const unsigned j = blockIdx.y + ncb * (unsigned)jj;
 		dv[i] = dinv_cur[e];
-		lv[i] = first ? lm_cur[e] : 0.0;
+		lv[i] = j < (unsigned)NB ? lm_cur[m * NB + j] : 0.0;
 	}
 #pragma unroll
 	for(int j = 0; j < NB * GJ_CB / 256; j++){
@@ -275,7 +379,7 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	for(int i = 0; i < NB * NB / 256; i++){
 		const int e = tid + 256 * i;
 		dinv[e / NB][e % NB] = dv[i];
-		if(first) lm[e / NB][e % NB] = lv[i];
+		lm[e % NB][e / NB] = lv[i];
 	}
 #pragma unroll
 	for(int j = 0; j < NB * GJ_CB / 256; j++){
@@ -313,23 +417,19 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		}
 	}
 	__syncthreads();
-	// LU multipliers of the rows below K (first column block only): thread (row, columns
-	// jg + 4 t), y_j = sum_m E[row][m] L[m][j]
-	if(blockIdx.y == 0){
+	// LU multipliers of the rows below K: thread (row, columns y + ncb (jg + 4 t)),
+	// y_j = sum_m E[row][m] L[m][j]
+	{
 		const int row = tid & 63, jg = tid >> 6;
 		const unsigned i = r0 + row;
-		if(i < n && i >= k1){
-			double er[NB];
-#pragma unroll
-			for(int m = 0; m < NB; m++) er[m] = Et[m][row];
+		if(i < n && i >= k1 && blockIdx.y + ncb * (unsigned)jg < (unsigned)NB){
 			bool bad = false;
-#pragma unroll
-			for(int t = 0; t < NB / 4; t++){
-				const int j = jg + 4 * t;
+			for(int jj = jg; blockIdx.y + ncb * (unsigned)jj < (unsigned)NB; jj += 4){
+				const unsigned j = blockIdx.y + ncb * (unsigned)jj;
 				double y = 0;
-#pragma unroll
-				for(int m = 0; m < NB; m++) y = fmac(er[m], lm[m][j], y);
-				bad |= j < (int)nbe && !(fabs(y) <= 1.0);
+#pragma unroll 8
+				for(int m = 0; m < NB; m++) y = fmac(Et[m][row], lm[m][jj], y);
+				bad |= j < nbe && !(fabs(y) <= 1.0);
 			}
 			if(bad) scal[SCAL_GJ_FLAG] = 1;
 		}
