@@ -260,9 +260,9 @@ __global__ __launch_bounds__(256) void gj_first_kernel(const double *__restrict_
 // the columns j = y, y + ncb, ... of L.
 //
 // Lookahead workgroup (blockIdx = (row tiles, 0)): loads only the 32 x 32 operands of the next
-// diagonal block D' = A'[K', K'], K' = [k0+nbe, ...), forms it with the same operations in the
-// same order as the tile that holds it (so both hold the same bits), inverts it and writes
-// D'^{-1}, L' and its pivots for the next step, so that no step waits for an inversion of its own.
+// diagonal block D' = A'[K', K'], K' = [k0+nbe, ...), forms it on the matrix cores (equal to
+// the tile that holds it up to rounding), inverts it and writes D'^{-1}, L' and its pivots for
+// the next step, so that no step waits for an inversion of its own.
 constexpr int GJ_RB = 64, GJ_CB = 64, GJ_LD = 68;
 
 __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__ A, double *__restrict__ An,
@@ -285,19 +285,29 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	const unsigned nrb = (n + GJ_RB - 1) / GJ_RB;
 	const unsigned k1 = k0 + nbe;
 	if(blockIdx.x == nrb){
-		// ---- lookahead: D' = A[K',K'] + E[K',:] A[K,K'] with E[K',:] = -A[K',K] D^{-1}, entry by
-		// entry the same fma sequence as the tile that holds it (m ascending over all NB columns)
+		// ---- lookahead: D' = A[K',K'] + E[K',:] A[K,K'] with E[K',:] = -A[K',K] D^{-1}, both
+		// products on the FP64 matrix cores (v_mfma_f64_16x16x4): wave w forms the 16 x 16 quadrant
+		// rows 16 qi + m, columns 16 qj + n (qi = w / 2, qj = w % 2) in 8 steps of k = 4.  The tile
+		// that holds A'[K',K'] sums in another order, so D' may differ from it in the last bits: the
+		// inversion uses D', the next step's panels use the tile's values (rounding-level difference)
 		if(blockIdx.y != 0 || k1 >= n) return;
 		const unsigned nb1 = min((unsigned)NB, n - k1);
-		// every global load is issued before the first LDS store, so they share one round trip
-		double av[NB * NB / 256], dv[NB * NB / 256], xv[NB * NB / 256], yv[NB * NB / 256];
+		typedef double gj_v4d __attribute__((ext_vector_type(4)));
+		const int m16 = lane & 15, k4 = lane >> 4, qi = wave >> 1, qj = wave & 1;
+		// every global load is issued before the first LDS store, so they share one round trip;
+		// A[K',K'] is read straight into the accumulator layout (row 16 qi + 4 e + k4, column 16 qj + m16)
+		double dv[NB * NB / 256], xv[NB * NB / 256], yv[NB * NB / 256], avm[4];
 #pragma unroll
 		for(int q = 0; q < NB * NB / 256; q++){
 			const int e = tid + 256 * q, i = e % NB, j = e / NB;     // i: row (fastest), j: column / m
 			dv[q] = dinv_cur[e];
 			xv[q] = (j < (int)nbe && i < (int)nb1) ? A[(size_t)(k0 + j) * n + k1 + i] : 0.0;
 			yv[q] = (i < (int)nbe && j < (int)nb1) ? A[(size_t)(k1 + j) * n + k0 + i] : 0.0;
-			av[q] = (i < (int)nb1 && j < (int)nb1) ? A[(size_t)(k1 + j) * n + k1 + i] : 0.0;
+		}
+#pragma unroll
+		for(int e = 0; e < 4; e++){
+			const int i = 16 * qi + 4 * e + k4, j = 16 * qj + m16;
+			avm[e] = (i < (int)nb1 && j < (int)nb1) ? A[(size_t)(k1 + j) * n + k1 + i] : 0.0;
 		}
 #pragma unroll
 		for(int q = 0; q < NB * NB / 256; q++){
@@ -307,24 +317,24 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 			Ys[i][j] = yv[q];
 		}
 		__syncthreads();
+		gj_v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-		for(int q = 0; q < NB * NB / 256; q++){
-			const int e = tid + 256 * q, i = e % NB, j = e / NB;
-			double x = 0;
-#pragma unroll 8
-			for(int m = 0; m < NB; m++) x = fmac(Art[m][i], dinv[m][j], x);
-			Et[j][i] = -x;
+		for(int kb = 0; kb < NB / 4; kb++){
+			const int k = 4 * kb + k4;
+			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Art[k][16 * qi + m16], dinv[k][16 * qj + m16], acc, 0, 0, 0);
 		}
+#pragma unroll
+		for(int e = 0; e < 4; e++) Et[16 * qj + m16][16 * qi + 4 * e + k4] = -acc[e];
 		__syncthreads();
 		double (*Dn)[GJ_LD] = Art;           // free once E' is formed
+		gj_v4d dac = {avm[0], avm[1], avm[2], avm[3]};
 #pragma unroll
-		for(int q = 0; q < NB * NB / 256; q++){
-			const int e = tid + 256 * q, i = e % NB, j = e / NB;
-			double x = av[q];
-#pragma unroll 8
-			for(int m = 0; m < NB; m++) x = fmac(Et[m][i], Ys[m][j], x);
-			Dn[i][j] = x;
+		for(int kb = 0; kb < NB / 4; kb++){
+			const int k = 4 * kb + k4;
+			dac = __builtin_amdgcn_mfma_f64_16x16x4f64(Et[k][16 * qi + m16], Ys[k][16 * qj + m16], dac, 0, 0, 0);
 		}
+#pragma unroll
+		for(int e = 0; e < 4; e++) Dn[16 * qi + 4 * e + k4][16 * qj + m16] = dac[e];
 		__syncthreads();
 		double a[8];
 #pragma unroll
