@@ -46,10 +46,10 @@ __device__ inline double rcp_nr(double x)
 // columns g..g+3): pivot rows <- P^{-1} pivot rows, every other row r -= a_r,[g..g+3] (new pivot
 // rows).  Wave w holds rows 8w..8w+7 in a[], lane c column c (lanes 32..63 end with D^{-1}); one
 // barrier per step publishes the four pivot rows and the four pivot columns.  The pivot columns
-// of every step are kept, and the LU multipliers of D (and the pivots U_gg) are formed from them
-// afterwards: with P = L_P U_P (Doolittle), row r's multipliers of the step are
-// l[r][g+i] = (a_r,g+i - sum_{q<i} l[r][g+q] U_P[q][i]) / U_P[i][i] (the division by a refined
-// reciprocal; the multipliers only feed the |l| <= 1 check).
+// of every step are kept, and the LU multipliers of D are formed from them afterwards: with
+// P = L_P U_P (U_P and the pivots U_gg come out of the block's own elimination), row r's
+// multipliers of the step are l[r][g+i] = (a_r,g+i - sum_{q<i} l[r][g+q] U_P[q][i]) / U_P[i][i]
+// (the division by a refined reciprocal; the multipliers only feed the |l| <= 1 check).
 // Writes dinv_out[r][c] = D^{-1}[r][c] and lm_out[m][j] = L[m][j] (unit lower triangular, zero
 // elsewhere, NB x NB row-major), the pivots of rows kb.. to diag/piv, and raises the flag when a
 // multiplier exceeds 1.  Called by all 256 threads of a workgroup.
@@ -73,8 +73,11 @@ __device__ __forceinline__ void gj_pivot_block(const GjInvLds &sh, int s, double
 	}
 }
 
-// Q = P^{-1} by Gauss-Jordan without exchanges on the 4 x 4 block
-__device__ __forceinline__ void gj_inv4(const double (&P)[4][4], double (&Q)[4][4])
+// Q = P^{-1} by Gauss-Jordan without exchanges on the 4 x 4 block.  Row k just before it is
+// scaled is row k of U in P = L_P U_P (only the rows below have been subtracted from it):
+// U[k][i] (i > k) and the reciprocal of the pivot U[k][k] are returned in U (its diagonal holds
+// the reciprocals) and the pivots in piv.
+__device__ __forceinline__ void gj_inv4(const double (&P)[4][4], double (&Q)[4][4], double (&U)[4][4], double (&pv)[4])
 {
 	double M[4][4];
 #pragma unroll
@@ -87,6 +90,9 @@ __device__ __forceinline__ void gj_inv4(const double (&P)[4][4], double (&Q)[4][
 #pragma unroll
 	for(int k = 0; k < 4; k++){
 		const double r = rcp_nr(M[k][k]);
+		pv[k] = M[k][k];
+#pragma unroll
+		for(int i = 0; i < 4; i++) U[k][i] = i == k ? r : (i > k ? M[k][i] : 0.0);
 #pragma unroll
 		for(int i = 0; i < 4; i++){
 			M[k][i] *= r;
@@ -101,29 +107,6 @@ __device__ __forceinline__ void gj_inv4(const double (&P)[4][4], double (&Q)[4][
 				M[m][i] = fmac(-f, M[k][i], M[m][i]);
 				Q[m][i] = fmac(-f, Q[k][i], Q[m][i]);
 			}
-		}
-	}
-}
-
-// U of P = L_P U_P (Doolittle, no exchanges): U[q][i] for q <= i
-__device__ __forceinline__ void gj_upper4(const double (&P)[4][4], double (&U)[4][4])
-{
-	double L[4][4];
-#pragma unroll
-	for(int q = 0; q < 4; q++){
-#pragma unroll
-		for(int i = q; i < 4; i++){
-			double u = P[q][i];
-#pragma unroll
-			for(int p = 0; p < q; p++) u = fmac(-L[q][p], U[p][i], u);
-			U[q][i] = u;
-		}
-#pragma unroll
-		for(int m = q + 1; m < 4; m++){
-			double l = P[m][q];
-#pragma unroll
-			for(int p = 0; p < q; p++) l = fmac(-L[m][p], U[p][q], l);
-			L[m][q] = l / U[q][q];
 		}
 	}
 }
@@ -151,9 +134,21 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 			for(int q = 0; q < 8; q += 2) *(double2 *)(dst + q) = make_double2(a[q], a[q + 1]);
 		}
 		__syncthreads();
-		double P[4][4], Q[4][4];
+		double P[4][4], Q[4][4], U[4][4], pv[4];
 		gj_pivot_block(sh, s, P);
-		gj_inv4(P, Q);
+		gj_inv4(P, Q, U, pv);
+		if(tid == 0){
+			// U of the pivot block (1 / U_qq on its diagonal) for the multipliers, and the pivots
+#pragma unroll
+			for(int q = 0; q < 4; q++){
+#pragma unroll
+				for(int i = 0; i < 4; i++) sh.ublk[s][q][i] = U[q][i];
+				if(g + q < (int)nbe){
+					diag[kb + g + q] = pv[q];
+					piv[kb + g + q] = (int)(kb + g + q);
+				}
+			}
+		}
 		double u[4], nr[4];
 #pragma unroll
 		for(int i = 0; i < 4; i++) u[i] = sh.prow[b][i][lane];
@@ -186,23 +181,7 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 #pragma unroll
 		for(int q = 0; q < 8; q++) dinv_out[(8 * wave + q) * NB + lane - NB] = a[q];
 	}
-	// U of every step's pivot block and the reciprocals of its diagonal (thread s, in LDS; the
-	// pivots U_gg go to diag), then L, entries (r, j) with r = (tid + 256 i) / NB and j = tid % NB
-	if(tid < NB / 4 && 4 * tid < (int)nbe){
-		const int g = 4 * tid;
-		double P[4][4], U[4][4];
-		gj_pivot_block(sh, tid, P);
-		gj_upper4(P, U);
-#pragma unroll
-		for(int q = 0; q < 4; q++){
-#pragma unroll
-			for(int i = 0; i < 4; i++) sh.ublk[tid][q][i] = q == i ? rcp_nr(U[q][q]) : (q < i ? U[q][i] : 0.0);
-			if(g + q < (int)nbe){
-				diag[kb + g + q] = U[q][q];
-				piv[kb + g + q] = (int)(kb + g + q);
-			}
-		}
-	}
+	// the multipliers from the pivot columns and the blocks' U (written by thread 0 in the loop)
 	__syncthreads();
 	bool bad = false;
 	{
