@@ -196,18 +196,20 @@ def copy_calibration():
 
 
 def correlator_leg(E, x_host, reps=5):
-    """cc_correlator at 32x32, Nt=128 on the chain's field (FP64 M_hat^{-1} and log det): ms per
-    measurement (2 Nt slice applications + the 9-component contraction per slice), and its
-    agreement with a second call (the kernels are deterministic)."""
+    """cc_correlator at 32x32, Nt=128 on the chain's field (FP64 M_hat^{-1} and log det, resident in
+    HBM): ms per measurement (2 Nt slice applications + the 9-component contraction per slice, and
+    the 9 Nt values copied back), and its agreement with a second call (the kernels are deterministic)."""
     t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, NT)
     c = E.Context(L, L, NT, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=E.FP64)
     c.chain_set_x(x_host)
     minv, ld = c.chain_minv(), c.chain_logdet()
-    g0 = c.cc_correlator(x_host, minv, ld)          # warm-up: allocates the transfer matrices
+    # the field and M_hat^{-1} stay in HBM, as in the host's trajectory loop
+    dx, dm = c.array(c.dim, x_host), c.array(c.mat, minv)
+    g0 = c.cc_correlator(dx, dm, ld)                # warm-up: allocates the transfer matrices
     c.sync()
     t0 = time.perf_counter()
     for _ in range(reps):
-        g = c.cc_correlator(x_host, minv, ld)
+        g = c.cc_correlator(dx, dm, ld)
     el = (time.perf_counter() - t0) / reps
     c.close()
     return {"ms_per_measurement": 1e3 * el, "reps": reps, "repeatable": bool(np.array_equal(g, g0)),

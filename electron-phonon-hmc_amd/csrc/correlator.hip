@@ -9,61 +9,71 @@
 //   g_ab(tau) = 1/N sum_{i,j} chi_ia chi_jb [R_t0[j*N+i] R_0t[jb*N+ia] + R_t0[jb*N+ia] R_0t[j*N+i]
 //                                        - R_t0[jb*N+i] R_0t[j*N+ia] - R_t0[j*N+ia] R_0t[jb*N+i]]
 // with ia = nb(i, 2a), jb = nb(j, 2b), chi_ia = t_a - kappa_a x[0][i][a], chi_jb from slice tau.
-// One launch covers every tau: thread = i (R read down its columns), a workgroup walks a chunk of j;
-// block sums per (tau, j-chunk, i-block), then a fixed-order pass per tau.
+// One launch covers every tau: a workgroup takes a chunk of j for one tau and, for each j, stages
+// the rows j and jb (b = 0..2) of both propagators in LDS, from where its threads gather the
+// entries at i and ia; block sums per (tau, j-chunk), then a fixed-order pass per tau.
 #include "internal.h"
 
 namespace ephmc {
 
-constexpr int CC_BT = 256;
+constexpr int CC_BT = 1024;
 constexpr unsigned CC_J = 8;   // values of j per workgroup
 
-// block sums of one correlator call: [nt][ceil(N/CC_J)][ceil(N/CC_BT)][9]
+// block sums of one correlator call: [nt][ceil(N/CC_J)][9]
 static size_t cc_partials(unsigned ns, unsigned nt)
 {
-	return (size_t)nt * ((ns + CC_J - 1) / CC_J) * ((ns + CC_BT - 1) / CC_BT) * 9;
+	return (size_t)nt * ((ns + CC_J - 1) / CC_J) * 9;
 }
 
-// grid (i-blocks of CC_BT, j-chunks of cj, tau): thread = i, so every R read runs down a column
-// (R[j*N + i] with i consecutive); the workgroup walks its cj values of j and writes the block
-// sums of the 9 components for its (tau, j-chunk, i-block).
+// grid (j-chunks of CC_J, tau).  rows (dynamic LDS, 8 N doubles): R_t0[j], R_0t[j], R_t0[jb]
+// (b = 0..2), R_0t[jb] (b = 0..2); thread t takes the sites i = t, t + CC_BT, ...
 __global__ __launch_bounds__(CC_BT) void cc_contract_kernel(const double *__restrict__ Tfin, const double *__restrict__ Rt0,
 		const double *__restrict__ R0t, const double *__restrict__ x, const int *__restrict__ nb, double t0, double t1, double t2,
-		double k0, double k1, double k2, unsigned ns, unsigned nt, unsigned cj, double *__restrict__ partial)
+		double k0, double k1, double k2, unsigned ns, unsigned nt, double *__restrict__ partial)
 {
-	__shared__ double red[9][CC_BT];
+	extern __shared__ double rows[];
+	__shared__ double red[9][CC_BT / 64];
 	const double tk[3] = {t0, t1, t2}, kk[3] = {k0, k1, k2};
-	const unsigned tau = blockIdx.z;
+	const unsigned tau = blockIdx.y;
 	const size_t mat = (size_t)ns * ns;
 	const double *rt = tau == 0 ? Tfin : Rt0 + (size_t)(nt - tau) * mat;
 	const double *r0 = R0t + (size_t)tau * mat;
 	const double *xt = x + (size_t)tau * 3 * ns;
-	const unsigned i = blockIdx.x * CC_BT + threadIdx.x;
-	const bool act = i < ns;
 	double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-	unsigned ia[3];
-	double chia[3];
+	const unsigned j0 = blockIdx.x * CC_J, j1 = min(j0 + CC_J, ns);
+	for(unsigned j = j0; j < j1; j++){
+		unsigned jb[3];
+		double chijb[3];
 #pragma unroll
-	for(int a = 0; a < 3; a++){
-		ia[a] = act ? (unsigned)nb[(2 * a) * ns + i] : 0u;
-		chia[a] = act ? tk[a] - kk[a] * x[(size_t)i * 3 + a] : 0.0;
-	}
-	const unsigned j0 = blockIdx.y * cj, j1 = min(j0 + cj, ns);
-	if(act)
-		for(unsigned j = j0; j < j1; j++){
-			const double *rtj = rt + (size_t)j * ns, *r0j = r0 + (size_t)j * ns;
-			const double rt_ji = rtj[i], r0_ji = r0j[i];
-			double rt_jia[3], r0_jia[3];
+		for(int b = 0; b < 3; b++){
+			jb[b] = (unsigned)nb[(2 * b) * ns + j];
+			chijb[b] = tk[b] - kk[b] * xt[(size_t)j * 3 + b];
+		}
+		__syncthreads();                 // the previous j's rows are no longer read
+		for(unsigned i = threadIdx.x; i < ns; i += CC_BT){
+			rows[i] = rt[(size_t)j * ns + i];
+			rows[ns + i] = r0[(size_t)j * ns + i];
+#pragma unroll
+			for(int b = 0; b < 3; b++){
+				rows[(2 + b) * ns + i] = rt[(size_t)jb[b] * ns + i];
+				rows[(5 + b) * ns + i] = r0[(size_t)jb[b] * ns + i];
+			}
+		}
+		__syncthreads();
+		for(unsigned i = threadIdx.x; i < ns; i += CC_BT){
+			const double rt_ji = rows[i], r0_ji = rows[ns + i];
+			unsigned ia[3];
+			double chia[3], rt_jia[3], r0_jia[3];
 #pragma unroll
 			for(int a = 0; a < 3; a++){
-				rt_jia[a] = rtj[ia[a]];
-				r0_jia[a] = r0j[ia[a]];
+				ia[a] = (unsigned)nb[(2 * a) * ns + i];
+				chia[a] = tk[a] - kk[a] * x[(size_t)i * 3 + a];
+				rt_jia[a] = rows[ia[a]];
+				r0_jia[a] = rows[ns + ia[a]];
 			}
 #pragma unroll
 			for(int b = 0; b < 3; b++){
-				const unsigned jb = (unsigned)nb[(2 * b) * ns + j];
-				const double chijb = tk[b] - kk[b] * xt[(size_t)j * 3 + b];
-				const double *rtb = rt + (size_t)jb * ns, *r0b = r0 + (size_t)jb * ns;
+				const double *rtb = rows + (2 + b) * ns, *r0b = rows + (5 + b) * ns;
 				const double rt_jbi = rtb[i], r0_jbi = r0b[i];
 #pragma unroll
 				for(int a = 0; a < 3; a++){
@@ -72,34 +82,40 @@ __global__ __launch_bounds__(CC_BT) void cc_contract_kernel(const double *__rest
 					corr += rt_jbia * r0_ji;
 					corr -= rt_jbi * r0_jia[a];
 					corr -= rt_jia[a] * r0_jbi;
-					g[a * 3 + b] += chia[a] * chijb * corr;
+					g[a * 3 + b] += chia[a] * chijb[b] * corr;
 				}
 			}
 		}
-#pragma unroll
-	for(int q = 0; q < 9; q++) red[q][threadIdx.x] = g[q];
-	__syncthreads();
-	for(int off = CC_BT / 2; off > 0; off >>= 1){
-		if((int)threadIdx.x < off)
-#pragma unroll
-			for(int q = 0; q < 9; q++) red[q][threadIdx.x] += red[q][threadIdx.x + off];
-		__syncthreads();
 	}
-	if(threadIdx.x < 9)
-		partial[(((size_t)tau * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * 9 + threadIdx.x] = red[threadIdx.x][0];
+	// fixed-order sums: a butterfly within each wave, then the waves in order
+	const unsigned lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+	for(int q = 0; q < 9; q++){
+		double v = g[q];
+#pragma unroll
+		for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+		if(lane == 0) red[q][wv] = v;
+	}
+	__syncthreads();
+	if(threadIdx.x < 9){
+		double t = 0;
+		for(int w = 0; w < CC_BT / 64; w++) t += red[threadIdx.x][w];
+		partial[((size_t)tau * gridDim.x + blockIdx.x) * 9 + threadIdx.x] = t;
+	}
 }
 
-// out[tau][q] = scale * sum of the block sums of slice tau, in block order
-__global__ void cc_sum_kernel(const double *__restrict__ partial, unsigned nblocks, unsigned nt, double scale,
+// out[tau][q] = scale * sum of the block sums of slice tau: workgroup tau, wave q; lane l adds the
+// blocks l, l + 64, ... and a butterfly adds the lanes (a fixed order)
+__global__ __launch_bounds__(576) void cc_sum_kernel(const double *__restrict__ partial, unsigned nblocks, double scale,
 		double *__restrict__ out)
 {
-	const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
-	if(e >= nt * 9) return;
-	const unsigned tau = e / 9, q = e % 9;
+	const unsigned tau = blockIdx.x, q = threadIdx.x >> 6, lane = threadIdx.x & 63;
 	const double *p = partial + (size_t)tau * nblocks * 9 + q;
 	double s = 0;
-	for(unsigned b = 0; b < nblocks; b++) s += p[(size_t)b * 9];
-	out[e] = s * scale;
+	for(unsigned b = lane; b < nblocks; b += 64) s += p[(size_t)b * 9];
+#pragma unroll
+	for(int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+	if(lane == 0) out[tau * 9 + q] = s * scale;
 }
 
 } // namespace ephmc
@@ -134,12 +150,15 @@ extern "C" int ephmc_cc_correlator(ephmc_ctx *c, const double *x, const double *
 	// R_0t: transposed inverse, then slices 0 .. nt-1 ascending; stored entry tau = state at tau
 	if(!ident) EPHMC_TRY(transpose_plus_identity(c, EPHMC_FP64, minv, c->d_gj[0], 0));
 	EPHMC_TRY(chain_forward(c, EPHMC_FP64, c->d_gj[0], c->d_gj[1], c->d_R0t, 1, 0, ident));
-	const unsigned bx = (ns + CC_BT - 1) / CC_BT, by = (ns + CC_J - 1) / CC_J, cj = CC_J;
+	const unsigned by = (ns + CC_J - 1) / CC_J;
+	const size_t lds = 8 * (size_t)ns * sizeof(double);
+	if(lds + 9 * (CC_BT / 64) * sizeof(double) > 160 * 1024){ set_error("correlator: %u sites exceed the LDS", ns); return EPHMC_ERR_UNSUPPORTED; }
 	double *partial = c->d_ccg + (size_t)nt * 9;     // cc_partials(ns, nt) block sums
-	cc_contract_kernel<<<dim3(bx, by, nt), CC_BT, 0, c->stream>>>(c->d_Tfin, c->d_Rt0, c->d_R0t, x, c->d_nb,
-		c->cfg.t[0], c->cfg.t[1], c->cfg.t[2], c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], ns, nt, cj, partial);
+	(void)hipFuncSetAttribute((const void *)cc_contract_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	cc_contract_kernel<<<dim3(by, nt), CC_BT, lds, c->stream>>>(c->d_Tfin, c->d_Rt0, c->d_R0t, x, c->d_nb,
+		c->cfg.t[0], c->cfg.t[1], c->cfg.t[2], c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], ns, nt, partial);
 	EPHMC_LAUNCHED();
-	cc_sum_kernel<<<(nt * 9 + 255) / 256, 256, 0, c->stream>>>(partial, bx * by, nt, c->cfg.single_particle ? exp(-logdet) : 1.0 / ns,
+	cc_sum_kernel<<<nt, 576, 0, c->stream>>>(partial, by, c->cfg.single_particle ? exp(-logdet) : 1.0 / ns,
 		c->d_ccg);
 	EPHMC_LAUNCHED();
 	EPHMC_TRY(timer_end(c, "correlator"));

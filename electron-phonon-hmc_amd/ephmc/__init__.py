@@ -294,8 +294,10 @@ class Context:
 
     # -- measurement ---------------------------------------------------------------------
     def cc_correlator(self, x, Minv, logdet=0.0):
-        """g_ab(tau), shape (nt, 3, 3), of field x with M_inv = (1 + B..B)^{-1}."""
-        dx, dm = self.array(self.dim, x), self.array(self.mat, Minv)
+        """g_ab(tau), shape (nt, 3, 3), of field x with M_inv = (1 + B..B)^{-1}; x and Minv are
+        host arrays or DeviceArrays already in HBM (the state organic_main hands over)."""
+        dx = x if isinstance(x, DeviceArray) else self.array(self.dim, x)
+        dm = Minv if isinstance(Minv, DeviceArray) else self.array(self.mat, Minv)
         out = np.zeros(self.nt * 9)
         check(LIB.ephmc_cc_correlator(self.h, dx.ptr, dm.ptr, logdet, _dp(out)))
         return out.reshape(self.nt, 3, 3)
