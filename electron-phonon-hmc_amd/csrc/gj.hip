@@ -26,10 +26,6 @@ namespace ephmc {
 
 constexpr int GJ_NB = 32;
 
-__global__ void gj_clear_kernel(double *scal)
-{
-	scal[SCAL_GJ_FLAG] = 0;
-}
 
 // 1/x: hardware reciprocal refined by two Newton steps
 __device__ inline double rcp_nr(double x)
@@ -218,6 +214,8 @@ __global__ __launch_bounds__(256) void gj_first_kernel(const double *__restrict_
 	constexpr int NB = GJ_NB;
 	__shared__ GjInvLds sh;
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	if(threadIdx.x == 0) scal[SCAL_GJ_FLAG] = 0;     // no multiplier above 1 seen yet
+	__syncthreads();
 	double a[8];
 #pragma unroll
 	for(int q = 0; q < 8; q++){
@@ -247,7 +245,7 @@ constexpr int GJ_RB = 64, GJ_CB = 64, GJ_LD = 68;
 __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__ A, double *__restrict__ An,
 		unsigned n, unsigned k0, unsigned nbe, const double *__restrict__ dinv_cur, const double *__restrict__ lm_cur,
 		double *__restrict__ dinv_next, double *__restrict__ lm_next, double *__restrict__ diag, int *__restrict__ piv,
-		double *__restrict__ scal)
+		double *__restrict__ scal, double *__restrict__ out_final)
 {
 	constexpr int NB = GJ_NB;
 	__shared__ __align__(16) double dinv[NB][NB + 2];
@@ -269,7 +267,35 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		// rows 16 qi + m, columns 16 qj + n (qi = w / 2, qj = w % 2) in 8 steps of k = 4.  The tile
 		// that holds A'[K',K'] sums in another order, so D' may differ from it in the last bits: the
 		// inversion uses D', the next step's panels use the tile's values (rounding-level difference)
-		if(blockIdx.y != 0 || k1 >= n) return;
+		if(blockIdx.y != 0) return;
+		if(k1 >= n){
+			// last step: no next block; with out_final this workgroup sums log|det| and its sign
+			// from the pivots (the flag is final: the last block's multipliers were checked when it
+			// was inverted)
+			if(out_final && scal[SCAL_GJ_FLAG] == 0){
+				double *part = &Art[0][0];
+				int *negs = reinterpret_cast<int *>(&Ys[0][0]);
+				double sl = 0;
+				int neg = 0;
+				for(unsigned i = tid; i < n; i += 256){
+					const double d = diag[i];
+					sl += log(fabs(d));
+					neg += d < 0;
+				}
+				part[tid] = sl;
+				negs[tid] = neg;
+				__syncthreads();
+				for(int off = 128; off > 0; off >>= 1){
+					if(tid < off){ part[tid] += part[tid + off]; negs[tid] += negs[tid + off]; }
+					__syncthreads();
+				}
+				if(tid == 0){
+					scal[SCAL_LOGDET] = part[0];
+					scal[SCAL_SIGN] = (negs[0] & 1) ? -1.0 : 1.0;
+				}
+			}
+			return;
+		}
 		const unsigned nb1 = min((unsigned)NB, n - k1);
 		typedef double gj_v4d __attribute__((ext_vector_type(4)));
 		const int m16 = lane & 15, k4 = lane >> 4, qi = wave >> 1, qj = wave & 1;
@@ -326,6 +352,9 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		return;
 	}
 	const unsigned r0 = blockIdx.x * GJ_RB, c0 = blockIdx.y * GJ_CB;
+	// the last step writes the inverse straight to out_final unless the pivoted fallback will run
+	// (it inverts the untouched input there)
+	double *__restrict__ dst = (out_final && scal[SCAL_GJ_FLAG] == 0) ? out_final : An;
 	// ---- loads: D^{-1}, L (first column block), panels and tile values, all issued before the
 	// first LDS store so that they share one round trip
 	// the LU multipliers of a row block are checked by all its tiles: tile y takes the columns
@@ -442,7 +471,7 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 #pragma unroll
 		for(int i = 0; i < 4; i++){
 			const unsigned row = r0 + tr + 16 * i;
-			if(row < n) An[(size_t)col * n + row] = colK ? Et[col - k0][tr + 16 * i] : acc[i][j];
+			if(row < n) dst[(size_t)col * n + row] = colK ? Et[col - k0][tr + 16 * i] : acc[i][j];
 		}
 	}
 }
@@ -560,8 +589,6 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M)
 {
 	const unsigned n = c->ns;
 	if(n > 4096){ set_error("gj: N = %u exceeds 4096", n); return EPHMC_ERR_UNSUPPORTED; }
-	gj_clear_kernel<<<1, 1, 0, c->stream>>>(c->d_scal);
-	EPHMC_LAUNCHED();
 	const double *cur = M;
 	double *buf[2] = {c->d_gj[0], c->d_gj[1]};
 	int which = 0;
@@ -571,20 +598,26 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M)
 	EPHMC_LAUNCHED();
 	const dim3 sgrid((n + GJ_RB - 1) / GJ_RB + 1, (n + GJ_CB - 1) / GJ_CB);
 	int par = 0;
+	// from the second step on the input M is no longer read by the steps, so the last step writes
+	// the inverse into it and sums the log det itself; a single-step inversion (N <= 32) copies
+	const bool direct = n > GJ_NB;
 	for(unsigned k0 = 0; k0 < n; k0 += GJ_NB){
 		const unsigned nbe = std::min<unsigned>(GJ_NB, n - k0);
 		double *next = buf[which];
+		double *final_out = (direct && k0 + nbe >= n) ? M : nullptr;
 		gj_step_kernel<<<sgrid, 256, 0, c->stream>>>(cur, next, n, k0, nbe, dinv_g[par], lm_g[par], dinv_g[par ^ 1], lm_g[par ^ 1],
-			c->d_diag, c->d_piv, c->d_scal);
+			c->d_diag, c->d_piv, c->d_scal, final_out);
 		EPHMC_LAUNCHED();
 		cur = next;
 		which ^= 1;
 		par ^= 1;
 	}
-	gj_logdet_kernel<<<1, 1024, 0, c->stream>>>(c->d_diag, n, c->d_scal);
-	EPHMC_LAUNCHED();
-	gj_copy_kernel<<<1024, 256, 0, c->stream>>>(cur, M, c->mat, c->d_scal);
-	EPHMC_LAUNCHED();
+	if(!direct){
+		gj_logdet_kernel<<<1, 1024, 0, c->stream>>>(c->d_diag, n, c->d_scal);
+		EPHMC_LAUNCHED();
+		gj_copy_kernel<<<1024, 256, 0, c->stream>>>(cur, M, c->mat, c->d_scal);
+		EPHMC_LAUNCHED();
+	}
 	const size_t lds = (size_t)n * (sizeof(double) + sizeof(int));
 	(void)hipFuncSetAttribute((const void *)gj_fallback_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 	gj_fallback_kernel<<<1, 1024, lds, c->stream>>>(M, n, c->d_scal);
