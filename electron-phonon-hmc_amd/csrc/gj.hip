@@ -52,7 +52,6 @@ __device__ inline double rcp_nr(double x)
 struct GjInvLds {
 	__align__(16) double prow[2][4][2 * GJ_NB];     // pivot rows of the step, by step parity
 	__align__(16) double pcs[GJ_NB / 4][4][GJ_NB];  // pivot columns of every step
-	double ublk[GJ_NB / 4][4][4];                   // U of every step's pivot block, 1 / U_qq on its diagonal
 };
 
 // P[m][i] = row g+m, column g+i of the pivot block of step s
@@ -112,6 +111,7 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 {
 	constexpr int NB = GJ_NB;
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+	double my_u[4][4] = {}, my_pv[4] = {};
 	for(int s = 0; s < NB / 4; s++){
 		const int g = 4 * s, b = s & 1, gq = g & 7;
 		if(g >= (int)nbe) break;
@@ -133,18 +133,16 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 		double P[4][4], Q[4][4], U[4][4], pv[4];
 		gj_pivot_block(sh, s, P);
 		gj_inv4(P, Q, U, pv);
-		if(tid == 0){
-			// U of the pivot block (1 / U_qq on its diagonal) for the multipliers, and the pivots
+		// every thread keeps U of the step its multipliers belong to (column tid % NB), and thread
+		// s the pivots of step s (register copies only: no memory traffic inside the loop)
+		if(((tid % NB) >> 2) == s)
 #pragma unroll
-			for(int q = 0; q < 4; q++){
+			for(int q = 0; q < 4; q++)
 #pragma unroll
-				for(int i = 0; i < 4; i++) sh.ublk[s][q][i] = U[q][i];
-				if(g + q < (int)nbe){
-					diag[kb + g + q] = pv[q];
-					piv[kb + g + q] = (int)(kb + g + q);
-				}
-			}
-		}
+				for(int i = 0; i < 4; i++) my_u[q][i] = U[q][i];
+		if(tid == s)
+#pragma unroll
+			for(int q = 0; q < 4; q++) my_pv[q] = pv[q];
 		double u[4], nr[4];
 #pragma unroll
 		for(int i = 0; i < 4; i++) u[i] = sh.prow[b][i][lane];
@@ -177,8 +175,14 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 #pragma unroll
 		for(int q = 0; q < 8; q++) dinv_out[(8 * wave + q) * NB + lane - NB] = a[q];
 	}
-	// the multipliers from the pivot columns and the blocks' U (written by thread 0 in the loop)
-	__syncthreads();
+	// the pivots, and the multipliers from the pivot columns and the blocks' U
+	if(tid < NB / 4 && 4 * tid < (int)nbe)
+#pragma unroll
+		for(int q = 0; q < 4; q++)
+			if(4 * tid + q < (int)nbe){
+				diag[kb + 4 * tid + q] = my_pv[q];
+				piv[kb + 4 * tid + q] = (int)(kb + 4 * tid + q);
+			}
 	bool bad = false;
 	{
 		const int j = tid % NB, s = j >> 2, ii = j & 3;
@@ -194,8 +198,8 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 					if(q > ii) break;
 					double x = sh.pcs[s][q][r];
 #pragma unroll
-					for(int p = 0; p < q; p++) x = fmac(-lr[p], sh.ublk[s][p][q], x);
-					lr[q] = x * sh.ublk[s][q][q];
+					for(int p = 0; p < q; p++) x = fmac(-lr[p], my_u[p][q], x);
+					lr[q] = x * my_u[q][q];
 				}
 				l = ii == 0 ? lr[0] : (ii == 1 ? lr[1] : (ii == 2 ? lr[2] : lr[3]));
 				bad |= !(fabs(l) <= 1.0);
