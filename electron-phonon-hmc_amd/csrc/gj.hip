@@ -384,19 +384,25 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		const unsigned i = r0 + rr;
 		arv[j] = (m < (int)nbe && i < n) ? A[(size_t)(k0 + m) * n + i] : 0.0;
 	}
-	// this thread's tile block: rows tr + 16 i, columns 4 tc + j (16 lanes read 16 consecutive rows)
-	const int tr = tid & 15, tc = tid >> 4;
-	double acc[4][4];
+	// the tile on the FP64 matrix cores in transposed form, T^T[col][row] += sum_m Y[m][col] E[row][m]
+	// (v_mfma_f64_16x16x4): wave w holds the 16 x 16 blocks of columns 16 (2 (w >> 1) + cb) and
+	// rows 16 (2 (w & 1) + rb); lane l has rows 16 rb' + l % 16 (16 lanes read 16 consecutive
+	// rows) and columns 16 cb' + 4 e + l / 16 in element e
+	typedef double gj_v4d __attribute__((ext_vector_type(4)));
+	const int m16 = lane & 15, k4 = lane >> 4, wcb = 2 * (wave >> 1), wrb = 2 * (wave & 1);
+	gj_v4d acc[2][2];
 #pragma unroll
-	for(int i = 0; i < 4; i++){
-		const unsigned row = r0 + tr + 16 * i;
-		const bool rowK = row >= k0 && row < k1;
+	for(int cb = 0; cb < 2; cb++)
 #pragma unroll
-		for(int j = 0; j < 4; j++){
-			const unsigned col = c0 + 4 * tc + j;
-			acc[i][j] = (row < n && col < n && !rowK) ? A[(size_t)col * n + row] : 0.0;
+		for(int rb = 0; rb < 2; rb++){
+			const unsigned row = r0 + 16 * (wrb + rb) + m16;
+			const bool rowK = row >= k0 && row < k1;
+#pragma unroll
+			for(int e = 0; e < 4; e++){
+				const unsigned col = c0 + 16 * (wcb + cb) + 4 * e + k4;
+				acc[cb][rb][e] = (row < n && col < n && !rowK) ? A[(size_t)col * n + row] : 0.0;
+			}
 		}
-	}
 #pragma unroll
 	for(int i = 0; i < NB * NB / 256; i++){
 		const int e = tid + 256 * i;
@@ -414,28 +420,30 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		Art[e / GJ_RB][e % GJ_RB] = arv[j];
 	}
 	__syncthreads();
-	// ---- transform rows: thread (rows 4 rq + i, columns 2 cq + jj)
+	// ---- transform rows on the matrix cores, transposed: E^T[j][i] = sum_m D^{-1}[m][j] X[i][m]
+	// (X[i][m] = Art[m][i]); wave w forms columns j of block w & 1 for the rows of blocks
+	// 2 (w >> 1) + ib; rows inside K take the rows of D^{-1}
 	{
-		const int rq = tid & 15, cq = tid >> 4;
-		double e[4][2] = {};
-#pragma unroll 8
-		for(int m = 0; m < NB; m++){
-			const double2 x0 = *(const double2 *)&Art[m][4 * rq], x1 = *(const double2 *)&Art[m][4 * rq + 2];
-			const double2 dv = *(const double2 *)&dinv[m][2 * cq];
-			const double xr[4] = {x0.x, x0.y, x1.x, x1.y};
+		const int jb = wave & 1, ib0 = 2 * (wave >> 1);
+		gj_v4d ea[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
-			for(int i = 0; i < 4; i++){
-				e[i][0] = fmac(xr[i], dv.x, e[i][0]);
-				e[i][1] = fmac(xr[i], dv.y, e[i][1]);
-			}
+		for(int kb = 0; kb < NB / 4; kb++){
+			const int k = 4 * kb + k4;
+			const double av = dinv[k][16 * jb + m16];
+#pragma unroll
+			for(int ib = 0; ib < 2; ib++)
+				ea[ib] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Art[k][16 * (ib0 + ib) + m16], ea[ib], 0, 0, 0);
 		}
 #pragma unroll
-		for(int i = 0; i < 4; i++){
-			const unsigned row = r0 + 4 * rq + i;
+		for(int ib = 0; ib < 2; ib++){
+			const int il = 16 * (ib0 + ib) + m16;
+			const unsigned row = r0 + il;
 			const bool inK = row >= k0 && row < k1;
 #pragma unroll
-			for(int jj = 0; jj < 2; jj++)
-				Et[2 * cq + jj][4 * rq + i] = inK ? dinv[row - k0][2 * cq + jj] : -e[i][jj];
+			for(int e = 0; e < 4; e++){
+				const int j = 16 * jb + 4 * e + k4;
+				Et[j][il] = inK ? dinv[row - k0][j] : -ea[ib][e];
+			}
 		}
 	}
 	__syncthreads();
@@ -456,28 +464,37 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 			if(bad) scal[SCAL_GJ_FLAG] = 1;
 		}
 	}
-	// ---- tile update: acc += E[rows, :] A[K, cols]
-#pragma unroll 8
-	for(int m = 0; m < NB; m++){
-		const double2 y0 = *(const double2 *)&Ys[m][4 * tc], y1 = *(const double2 *)&Ys[m][4 * tc + 2];
-		const double ev[4] = {Et[m][tr], Et[m][tr + 16], Et[m][tr + 32], Et[m][tr + 48]};
-		const double yv[4] = {y0.x, y0.y, y1.x, y1.y};
+	// ---- tile update: T^T[col][row] += sum_m Y[m][col] E[row][m] (Ys[m][col], Et[m][row])
 #pragma unroll
-		for(int i = 0; i < 4; i++)
+	for(int kb = 0; kb < NB / 4; kb++){
+		const int k = 4 * kb + k4;
+		double yv[2], ev[2];
 #pragma unroll
-			for(int j = 0; j < 4; j++) acc[i][j] = fmac(ev[i], yv[j], acc[i][j]);
-	}
-#pragma unroll
-	for(int j = 0; j < 4; j++){
-		const unsigned col = c0 + 4 * tc + j;
-		if(col >= n) continue;
-		const bool colK = col >= k0 && col < k1;
-#pragma unroll
-		for(int i = 0; i < 4; i++){
-			const unsigned row = r0 + tr + 16 * i;
-			if(row < n) dst[(size_t)col * n + row] = colK ? Et[col - k0][tr + 16 * i] : acc[i][j];
+		for(int q = 0; q < 2; q++){
+			yv[q] = Ys[k][16 * (wcb + q) + m16];
+			ev[q] = Et[k][16 * (wrb + q) + m16];
 		}
+#pragma unroll
+		for(int cb = 0; cb < 2; cb++)
+#pragma unroll
+			for(int rb = 0; rb < 2; rb++)
+				acc[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(yv[cb], ev[rb], acc[cb][rb], 0, 0, 0);
 	}
+#pragma unroll
+	for(int cb = 0; cb < 2; cb++)
+#pragma unroll
+		for(int e = 0; e < 4; e++){
+			const int cl = 16 * (wcb + cb) + 4 * e + k4;
+			const unsigned col = c0 + cl;
+			if(col >= n) continue;
+			const bool colK = col >= k0 && col < k1;
+#pragma unroll
+			for(int rb = 0; rb < 2; rb++){
+				const int rl = 16 * (wrb + rb) + m16;
+				const unsigned row = r0 + rl;
+				if(row < n) dst[(size_t)col * n + row] = colK ? Et[col - k0][rl] : acc[cb][rb][e];
+			}
+		}
 }
 
 // log|det| and sign from the pivots of the fast path (no row exchanges).
