@@ -135,7 +135,9 @@ __device__ __forceinline__ void gather6(const R *__restrict__ buf, const int (&n
 	}
 }
 
-template <typename R, int C, int BT, int S, int MAXD>
+// DEG is the polynomial degree (a.deg), a template constant so that in fp32 the phase structure and
+// the S_a recursion need no run-time tests
+template <typename R, int C, int BT, int S, int DEG>
 __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -243,7 +245,9 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			wv[j][c] = val;
 		}
 
-	const int deg = a.deg;
+	// fp64 tests the degree at run time (equal to DEG): the tests keep fewer values live, and the
+	// fp64 sweep at C = 4 would spill registers with the constant
+	const int deg = sizeof(R) == 4 ? DEG : a.deg;
 	// coefficients and T of slice tau-1 are loaded while slice tau is processed (each thread
 	// reads back only the scratch entries it wrote itself)
 	R cfn[S][6], tn0[S][C];
@@ -297,16 +301,16 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 		if(tau > a.tau_b) load_slice(tau - 1);
 
 		// S_a = sum_b c_{a+b+1} T_b, built while the powers T_b = A^b T are formed
-		R sa[MAXD][S][C];
+		R sa[DEG][S][C];
 #pragma unroll
-		for(int q = 0; q < MAXD; q++)
+		for(int q = 0; q < DEG; q++)
 #pragma unroll
 			for(int j = 0; j < S; j++)
 #pragma unroll
 				for(int c = 0; c < C; c++) sa[q][j][c] = (q < deg) ? a.cm[q + 1] * t[j][c] : (R)0;
-		// unrolled over the degree bound: b is a constant, so bb is a known LDS buffer
+		// unrolled: b is a constant, so bb is a known LDS buffer
 #pragma unroll
-		for(int b = 1; b < MAXD; b++){
+		for(int b = 1; b < DEG; b++){
 			if(b >= deg) break;
 			R *bb = ((b - 1) & 1) ? buf1 : buf0;
 #pragma unroll
@@ -318,8 +322,11 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			if(FOLD && a.w0_early && b == 1 && tau_out >= 0) flush_bonds();
 #pragma unroll
 			for(int j = 0; j < S; j++){
+				// fp32: a thread past the last site gathers too (neighbour 0, zero coefficients and T,
+				// all in bounds) and stores nothing, so its lanes need no separate zero values; fp64
+				// keeps the guard, which holds fewer registers live
 				R tn[C];
-				if(act[j]) gather6<R, C>(bb, nbr[j], cf[j], a.mu, t[j], tn);
+				if(sizeof(R) == 4 || act[j]) gather6<R, C>(bb, nbr[j], cf[j], a.mu, t[j], tn);
 				else
 #pragma unroll
 					for(int c = 0; c < C; c++) tn[c] = 0;
@@ -327,7 +334,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 				for(int c = 0; c < C; c++) t[j][c] = tn[c];
 			}
 #pragma unroll
-			for(int q = 0; q < MAXD; q++)
+			for(int q = 0; q < DEG; q++)
 				if(q + b < deg)
 #pragma unroll
 					for(int j = 0; j < S; j++)
@@ -350,7 +357,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			}
 		}
 #pragma unroll
-		for(int q = 0; q < MAXD; q++){
+		for(int q = 0; q < DEG; q++){
 			if(q >= deg) break;
 			R *wb = wbs[q & 1];
 			if(q > 0 || !a.w0_early){
@@ -465,14 +472,23 @@ static int launch_adjoint_c(ephmc_ctx *c, AdjArgs<R> a)
 	const size_t lds = (((4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7) + bt * sizeof(double);
 	if(lds > 160 * 1024){ set_error("adjoint: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
 	const unsigned blocks = a.nwg;
-#define EPHMC_ADJ_LAUNCH2(BT, S, MAXD)                                                            \
+#define EPHMC_ADJ_LAUNCH2(BT, S, DEG)                                                             \
 	do {                                                                                          \
-		(void)hipFuncSetAttribute((const void *)adjoint_kernel<R, C, BT, S, MAXD>,                      \
+		(void)hipFuncSetAttribute((const void *)adjoint_kernel<R, C, BT, S, DEG>,                       \
 			hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                \
-		adjoint_kernel<R, C, BT, S, MAXD><<<blocks, BT, lds, c->stream>>>(a);                    \
+		adjoint_kernel<R, C, BT, S, DEG><<<blocks, BT, lds, c->stream>>>(a);                     \
 	} while(0)
 #define EPHMC_ADJ_LAUNCH(BT, S)                                                                   \
-	do { if(a.deg <= 4) EPHMC_ADJ_LAUNCH2(BT, S, 4); else EPHMC_ADJ_LAUNCH2(BT, S, 8); } while(0)
+	do {                                                                                          \
+		switch(a.deg){                                                                            \
+			case 2: EPHMC_ADJ_LAUNCH2(BT, S, 2); break;                                               \
+			case 4: EPHMC_ADJ_LAUNCH2(BT, S, 4); break;                                               \
+			case 6: EPHMC_ADJ_LAUNCH2(BT, S, 6); break;                                               \
+			case 8: EPHMC_ADJ_LAUNCH2(BT, S, 8); break;                                               \
+			default: set_error("adjoint: polynomial degree %d (Taylor orders 2, 4, 6, 8)", a.deg);     \
+				return EPHMC_ERR_UNSUPPORTED;                                                         \
+		}                                                                                         \
+	} while(0)
 	if(ns <= 256) EPHMC_ADJ_LAUNCH(256, 1);
 	else if(ns <= 512) EPHMC_ADJ_LAUNCH(512, 1);
 	else if(ns <= 1024 && sites_per_thread("EPHMC_ADJOINT_SITES") == 2) EPHMC_ADJ_LAUNCH(512, 2);
