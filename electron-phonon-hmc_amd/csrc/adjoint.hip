@@ -135,6 +135,18 @@ __device__ __forceinline__ void gather6(const R *__restrict__ buf, const int (&n
 	}
 }
 
+template <typename R, int C, int BT, int S>
+__host__ __device__ constexpr size_t adj_buf_stride(unsigned ns)
+{
+	return sizeof(R) == 4 ? (size_t)C * BT * S : (size_t)C * ns;
+}
+
+template <typename R, int C, int BT, int S>
+__host__ __device__ constexpr size_t adj_lds_bytes(unsigned ns)
+{
+	return (((4 * adj_buf_stride<R, C, BT, S>(ns) + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7) + BT * sizeof(double);
+}
+
 // DEG is the polynomial degree (a.deg), a template constant so that in fp32 the phase structure and
 // the S_a recursion need no run-time tests
 template <typename R, int C, int BT, int S, int DEG>
@@ -142,13 +154,17 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	const unsigned ns = a.ns;
+	// LDS: buf0, buf1, wb0, wb1 of bs values each, then hx [site][3] and the reduction scratch; in
+	// fp32 bs is the constant C * BT * S, so the four buffers sit at fixed offsets from one
+	// neighbour address and a read needs no address add (the offsets fit the 16-bit immediate
+	// when C * BT * S * 4 B <= 16 KiB)
+	const size_t bs = adj_buf_stride<R, C, BT, S>(ns);
 	R *buf0 = reinterpret_cast<R *>(smem_raw);
-	R *buf1 = buf0 + (size_t)C * ns;
-	R *wb0 = buf1 + (size_t)C * ns;
-	R *wb1 = wb0 + (size_t)C * ns;
-	R *hx = wb1 + (size_t)C * ns;          // [site][3]
-	double *rscratch = reinterpret_cast<double *>(smem_raw + (((4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7));
-	R *bufs[2] = {buf0, buf1};
+	R *buf1 = buf0 + bs;
+	R *wb0 = buf0 + 2 * bs;
+	R *wb1 = buf0 + 3 * bs;
+	R *hx = buf0 + 4 * bs;                 // [site][3]
+	double *rscratch = reinterpret_cast<double *>(smem_raw + adj_lds_bytes<R, C, BT, S>(ns) - BT * sizeof(double));
 	if(a.red_src){
 		const unsigned total = (unsigned)a.red_ntau * 3 * ns;
 		const unsigned per = (total + a.nwg - 1) / a.nwg;
@@ -257,10 +273,22 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 		                         : seg + (size_t)(tau - a.tau_b) * C * ns;
 #pragma unroll
 		for(int j = 0; j < S; j++){
+			if constexpr(sizeof(R) == 4){
+				// fp32: every lane loads, so no lane needs a zero of its own. A lane past the last site
+				// reads site ns-1 and stores nothing. A column past the last (last workgroup only)
+				// reads column ns-1; its W column is zero throughout the sweep, so its products add
+				// zeros to the bond sums.
+				const unsigned sl = act[j] ? site[j] : ns - 1;
 #pragma unroll
-			for(int d = 0; d < 6; d++) cfn[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
+				for(int d = 0; d < 6; d++) cfn[j][d] = cft[d * ns + sl];
 #pragma unroll
-			for(int c = 0; c < C; c++) tn0[j][c] = (act[j] && colok(c)) ? tsrc[(size_t)c * ns + site[j]] : (R)0;
+				for(int c = 0; c < C; c++) tn0[j][c] = tsrc[(size_t)(colok(c) ? c : ns - 1 - col0) * ns + sl];
+			}else{
+#pragma unroll
+				for(int d = 0; d < 6; d++) cfn[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
+#pragma unroll
+				for(int c = 0; c < C; c++) tn0[j][c] = (act[j] && colok(c)) ? tsrc[(size_t)c * ns + site[j]] : (R)0;
+			}
 		}
 	};
 	// bond(i, 2k) = H[i][2k] + H[nb(i,2k)][2k+1]: the H[.][2k+1] of a slice go to LDS (hx) at
@@ -322,9 +350,9 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			if(FOLD && a.w0_early && b == 1 && tau_out >= 0) flush_bonds();
 #pragma unroll
 			for(int j = 0; j < S; j++){
-				// fp32: a thread past the last site gathers too (neighbour 0, zero coefficients and T,
-				// all in bounds) and stores nothing, so its lanes need no separate zero values; fp64
-				// keeps the guard, which holds fewer registers live
+				// fp32: a thread past the last site gathers too (neighbour 0, in bounds) and stores
+				// nothing, so its lanes need no zero values; fp64 keeps the guard, which holds fewer
+				// registers live
 				R tn[C];
 				if(sizeof(R) == 4 || act[j]) gather6<R, C>(bb, nbr[j], cf[j], a.mu, t[j], tn);
 				else
@@ -468,12 +496,11 @@ template <typename R, int C>
 static int launch_adjoint_c(ephmc_ctx *c, AdjArgs<R> a)
 {
 	const unsigned ns = a.ns;
-	const unsigned bt = ns <= 256 ? 256 : (ns <= 512 ? 512 : 1024);
-	const size_t lds = (((4 * (size_t)C * ns + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7) + bt * sizeof(double);
-	if(lds > 160 * 1024){ set_error("adjoint: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
 	const unsigned blocks = a.nwg;
 #define EPHMC_ADJ_LAUNCH2(BT, S, DEG)                                                             \
 	do {                                                                                          \
+		const size_t lds = adj_lds_bytes<R, C, BT, S>(ns);                                        \
+		if(lds > 160 * 1024){ set_error("adjoint: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; } \
 		(void)hipFuncSetAttribute((const void *)adjoint_kernel<R, C, BT, S, DEG>,                       \
 			hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                \
 		adjoint_kernel<R, C, BT, S, DEG><<<blocks, BT, lds, c->stream>>>(a);                     \
