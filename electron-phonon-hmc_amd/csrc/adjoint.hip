@@ -147,8 +147,8 @@ __host__ __device__ constexpr size_t adj_lds_bytes(unsigned ns)
 	return (((4 * adj_buf_stride<R, C, BT, S>(ns) + 3 * (size_t)ns) * sizeof(R) + 7) & ~(size_t)7) + BT * sizeof(double);
 }
 
-// DEG is the polynomial degree (a.deg), a template constant so that in fp32 the phase structure and
-// the S_a recursion need no run-time tests
+// DEG is the polynomial degree (a.deg), a template constant so that in the fp32 one-site form the
+// phase structure and the S_a recursion need no run-time tests
 template <typename R, int C, int BT, int S, int DEG>
 __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 {
@@ -261,9 +261,11 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			wv[j][c] = val;
 		}
 
-	// fp64 tests the degree at run time (equal to DEG): the tests keep fewer values live, and the
-	// fp64 sweep at C = 4 would spill registers with the constant
-	const int deg = sizeof(R) == 4 ? DEG : a.deg;
+	// fp32 with one site per thread: every lane loads and gathers, so no lane needs zero values
+	// of its own, and the degree is the constant DEG (with two sites per thread, or in fp64, the
+	// guarded form and the run-time degree hold fewer registers)
+	constexpr bool ALL_LANES = sizeof(R) == 4 && S == 1;
+	const int deg = ALL_LANES ? DEG : a.deg;
 	// coefficients and T of slice tau-1 are loaded while slice tau is processed (each thread
 	// reads back only the scratch entries it wrote itself)
 	R cfn[S][6], tn0[S][C];
@@ -273,11 +275,10 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 		                         : seg + (size_t)(tau - a.tau_b) * C * ns;
 #pragma unroll
 		for(int j = 0; j < S; j++){
-			if constexpr(sizeof(R) == 4){
-				// fp32: every lane loads, so no lane needs a zero of its own. A lane past the last site
-				// reads site ns-1 and stores nothing. A column past the last (last workgroup only)
-				// reads column ns-1; its W column is zero throughout the sweep, so its products add
-				// zeros to the bond sums.
+			if constexpr(ALL_LANES){
+				// A lane past the last site reads site ns-1 and stores nothing. A column past the
+				// last (last workgroup only) reads column ns-1; its W column is zero throughout the
+				// sweep, so its products add zeros to the bond sums.
 				const unsigned sl = act[j] ? site[j] : ns - 1;
 #pragma unroll
 				for(int d = 0; d < 6; d++) cfn[j][d] = cft[d * ns + sl];
@@ -350,11 +351,10 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			if(FOLD && a.w0_early && b == 1 && tau_out >= 0) flush_bonds();
 #pragma unroll
 			for(int j = 0; j < S; j++){
-				// fp32: a thread past the last site gathers too (neighbour 0, in bounds) and stores
-				// nothing, so its lanes need no zero values; fp64 keeps the guard, which holds fewer
-				// registers live
+				// with ALL_LANES a thread past the last site gathers too (neighbour 0, in bounds)
+				// and stores nothing
 				R tn[C];
-				if(sizeof(R) == 4 || act[j]) gather6<R, C>(bb, nbr[j], cf[j], a.mu, t[j], tn);
+				if(ALL_LANES || act[j]) gather6<R, C>(bb, nbr[j], cf[j], a.mu, t[j], tn);
 				else
 #pragma unroll
 					for(int c = 0; c < C; c++) tn[c] = 0;
@@ -426,7 +426,9 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 						hs = fmac(sa[q][j][c], nv[dv][c], hs);
 						wn[c] = fmac(cf[j][d], nv[dv][c], wn[c]);
 					}
-					h[j][d] += hs;
+					// (with ALL_LANES the first phase's sum is taken as is; the guarded form adds
+					// it to zero, which holds fewer registers)
+					h[j][d] = (ALL_LANES && q == 0) ? hs : h[j][d] + hs;
 				}
 #pragma unroll
 				for(int c = 0; c < C; c++){
