@@ -91,13 +91,27 @@ __device__ __forceinline__ void stencil_gather(const R *__restrict__ buf, const 
 	}
 }
 
+// fp32 with one site per thread: every lane loads and gathers (a lane past the last site reads
+// site ns-1 or neighbour 0, all in bounds, and stores nothing), so no lane needs zero values of
+// its own, and the second LDS buffer sits at the constant offset C * BT, which a read takes as an
+// immediate offset
+template <typename R, int S>
+constexpr bool chain_all_lanes(){ return sizeof(R) == 4 && S == 1; }
+
+template <typename R, int C, int BT, int S>
+constexpr size_t chain_lds_bytes(unsigned ns)
+{
+	return 2 * (chain_all_lanes<R, S>() ? (size_t)C * BT : (size_t)C * ns) * sizeof(R);
+}
+
 template <typename R, int C, int BT, int S>
 __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
-	R *V = reinterpret_cast<R *>(smem_raw);
-	R *W = V + (size_t)C * a.ns;
+	constexpr bool ALL_LANES = chain_all_lanes<R, S>();
 	const unsigned ns = a.ns;
+	R *V = reinterpret_cast<R *>(smem_raw);
+	R *W = V + (ALL_LANES ? (size_t)C * BT : (size_t)C * ns);
 	const unsigned col0 = blockIdx.x * C;
 	const size_t mat = (size_t)ns * ns;
 
@@ -126,9 +140,11 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 	auto load_cf = [&](int i){
 		const R *cft = a.coef + (size_t)(a.tau0 + a.dir * i) * 6 * ns;
 #pragma unroll
-		for(int j = 0; j < S; j++)
+		for(int j = 0; j < S; j++){
+			const unsigned sl = ALL_LANES && !act[j] ? ns - 1 : site[j];
 #pragma unroll
-			for(int d = 0; d < 6; d++) cfn[j][d] = act[j] ? cft[d * ns + site[j]] : (R)0;
+			for(int d = 0; d < 6; d++) cfn[j][d] = (ALL_LANES || act[j]) ? cft[d * ns + sl] : (R)0;
+		}
 	};
 	if(a.ntau > 0) load_cf(0);
 	for(int i = 0; i < a.ntau; i++){
@@ -157,7 +173,7 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 			__syncthreads();
 #pragma unroll
 			for(int j = 0; j < S; j++)
-				if(act[j]) stencil_gather<R, C>(V, nbr[j], cf[j], a.mu, v[j], w[j]);
+				if(ALL_LANES || act[j]) stencil_gather<R, C>(V, nbr[j], cf[j], a.mu, v[j], w[j]);
 			if(a.cplx[p]){
 				// W is free: its last readers passed the barrier above.  The barrier below also
 				// retires every read of V before the next pair overwrites it.
@@ -169,7 +185,7 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 				__syncthreads();
 #pragma unroll
 				for(int j = 0; j < S; j++){
-					if(!act[j]) continue;
+					if(!ALL_LANES && !act[j]) continue;
 					R z[C];
 					stencil_gather<R, C>(W, nbr[j], cf[j], a.mu, w[j], z);
 #pragma unroll
@@ -325,10 +341,10 @@ static int launch_chain_c(ephmc_ctx *c, const ChainArgs<R> &a)
 {
 	const unsigned ns = a.ns;
 	const unsigned blocks = (ns + C - 1) / C;
-	const size_t lds = 2 * (size_t)C * ns * sizeof(R);
-	if(lds > 160 * 1024){ set_error("chain: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
 #define EPHMC_CHAIN_LAUNCH(BT, S)                                                                  \
 	do {                                                                                           \
+		const size_t lds = chain_lds_bytes<R, C, BT, S>(ns);                                       \
+		if(lds > 160 * 1024){ set_error("chain: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; } \
 		(void)hipFuncSetAttribute((const void *)chain_kernel<R, C, BT, S>,                               \
 			hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                 \
 		chain_kernel<R, C, BT, S><<<blocks, BT, lds, c->stream>>>(a);                             \
