@@ -182,6 +182,29 @@ def test_force_mixed():
     assert rel(Fg, Fo) < 1e-5
 
 
+@pytest.mark.parametrize("checkpoint", [1, 4])
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-5)])
+@pytest.mark.parametrize("cols", ["1", "2", "4"])
+@pytest.mark.parametrize("l1,l2,nt", [(7, 5, 8), (13, 11, 4), (23, 23, 4)])
+def test_force_ragged(monkeypatch, l1, l2, nt, cols, precision, tol, checkpoint):
+    """Force where the last workgroup's columns run past N (N = 35, 143, 529 are odd) and most
+    lanes of a workgroup lie past the last site (N = 35 on 256 threads, 529 on 1024). In FP32
+    those lanes load and gather but store nothing, and the columns past N read column N-1 and
+    must add only zeros to the bond sums. Every slice stored (k = 1) and recomputed from every
+    4th slice. beta = 40 Nt / 32 (slice width 1.25, as in the other force tests): with beta = 40 at
+    Nt = 4 the slices are 10 wide and the FP32 chain keeps only about 3 digits."""
+    say("force ragged", l1, l2, nt, "cols", cols, "precision", precision, "checkpoint", checkpoint)
+    monkeypatch.setenv("EPHMC_ADJOINT_COLS", cols)
+    om = OracleModel(l1, l2, nt, beta=nt * 40 / 32)
+    ctx = gpu_context(om, precision=precision, checkpoint=checkpoint)
+    x = field(om, 7)
+    ld, _, Minv = om.logdet_inv(om.M_hat(x))
+    Fo = om.force(x, Minv, ld)
+    Fg = ctx.force(x, Minv, ld)
+    say("rel err", rel(Fg, Fo))
+    assert rel(Fg, Fo) < tol
+
+
 @pytest.mark.parametrize("nt", [1, 2, 8, 12, 32, 48])
 def test_boson_kernels(nt):
     say("bosons", nt)
