@@ -182,6 +182,24 @@ def test_force_mixed():
     assert rel(Fg, Fo) < 1e-5
 
 
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-5)])
+@pytest.mark.parametrize("taylor", [2, 6, 8])
+def test_taylor_orders(taylor, precision, tol):
+    """Slice exponentials of Taylor orders 2, 6 and 8 (polynomial degree 2, 6, 8: one force-sweep
+    instance per degree): M_hat and the force against the oracle built with the same order."""
+    say("taylor order", taylor, "precision", precision)
+    om = OracleModel(7, 5, 16, beta=16 * 40 / 32, taylor=taylor)
+    ctx = gpu_context(om, precision=precision, taylor_order=taylor)
+    x = field(om, 11)
+    Mo = om.M_hat(x)
+    assert rel(ctx.M_hat(x), Mo) < 1e-13
+    ld, _, Minv = om.logdet_inv(Mo)
+    Fo = om.force(x, Minv, ld)
+    Fg = ctx.force(x, Minv, ld)
+    say("rel err", rel(Fg, Fo))
+    assert rel(Fg, Fo) < tol
+
+
 @pytest.mark.parametrize("checkpoint", [1, 4])
 @pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-5)])
 @pytest.mark.parametrize("cols", ["1", "2", "4"])
