@@ -19,7 +19,7 @@
 
 namespace ephmc {
 
-constexpr int FFT_LW = 16;     // complex series per workgroup
+constexpr int FFT_LW = 8;      // complex series per workgroup
 constexpr int FFT_BT = 256;
 
 enum FftMode { FFT_EVOLVE = 0, FFT_FILTER = 1, FFT_ENERGY = 2 };
