@@ -291,8 +291,8 @@ __global__ __launch_bounds__(LB) void chain_strip_kernel(ChainArgs<R> a)
 // Strip width for this lattice and arithmetic (0: site-per-thread form): the width chosen by
 // ephmc_autotune, else EPHMC_CHAIN_STRIP=0/2/4, else 2 in fp64 and 0 in fp32.  The strip form
 // cuts the LDS reads by a third (M = 2) at a larger register footprint; at 32x32 it is the
-// faster fp64 chain (0.53 against 0.76 ms per 128 slices) and the slower fp32 one (0.32 against
-// 0.31 ms).
+// faster fp64 chain (0.53 against 0.76 ms per 128 slices) and the slower fp32 one (0.31 against
+// 0.28 ms).
 unsigned strip_width(const ephmc_ctx *c, int prec)
 {
 	const int slot = prec == EPHMC_FP64 ? 0 : 1;

@@ -146,7 +146,7 @@ int timer_end(ephmc_ctx *c, const char *group)
 }
 
 // Columns per workgroup of the adjoint sweep: the widest that fits LDS and keeps at least one
-// workgroup per CU (at 32x32: 4 columns, 0.14 ms per 32-slice launch in fp32 and 0.32 ms in
+// workgroup per CU (at 32x32: 4 columns, 0.13 ms per 32-slice launch in fp32 and 0.32 ms in
 // fp64).  EPHMC_ADJOINT_COLS overrides it for tuning sweeps.
 static unsigned adjoint_cols_auto(unsigned ns, size_t rsize)
 {
