@@ -107,15 +107,22 @@ __device__ void reduce_links(const P *__restrict__ part, unsigned nwg, unsigned 
 	}
 }
 
+// A site's C values fit one 16-byte LDS read (fp32 up to 4 columns, fp64 up to 2): with one site
+// per thread the register footprint allows the lean form of the sweep (all lanes active, six
+// neighbour reads issued together, constant LDS offsets)
 template <typename R, int C>
+constexpr bool adj_lean(){ return C * sizeof(R) <= 16; }
+
+// BATCH: the six neighbour reads are issued before any of them is used; otherwise each is read
+// just before its sums, which holds fewer registers. The sum is taken in the order d = 0..5
+// either way.
+template <typename R, int C, bool BATCH>
 __device__ __forceinline__ void gather6(const R *__restrict__ buf, const int (&nbr)[6], const R (&cf)[6], R mu,
 		const R (&own)[C], R (&out)[C])
 {
 #pragma unroll
 	for(int c = 0; c < C; c++) out[c] = mu * own[c];
-	if constexpr(sizeof(R) == 4){
-		// fp32: the six neighbour reads are issued together (fp64 keeps them interleaved with the
-		// sums, which holds fewer registers); the sum is taken in the order d = 0..5 either way
+	if constexpr(BATCH){
 		R nv[6][C];
 #pragma unroll
 		for(int d = 0; d < 6; d++)
@@ -138,7 +145,7 @@ __device__ __forceinline__ void gather6(const R *__restrict__ buf, const int (&n
 template <typename R, int C, int BT, int S>
 __host__ __device__ constexpr size_t adj_buf_stride(unsigned ns)
 {
-	return sizeof(R) == 4 ? (size_t)C * BT * S : (size_t)C * ns;
+	return (sizeof(R) == 4 || (adj_lean<R, C>() && S == 1)) ? (size_t)C * BT * S : (size_t)C * ns;
 }
 
 template <typename R, int C, int BT, int S>
@@ -154,10 +161,15 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	const unsigned ns = a.ns;
+	// lean one-site form (fp64 only up to degree 4; beyond it, with two sites per thread, or in
+	// fp64 at 4 columns, the guarded form and the run-time degree hold fewer registers); fp32
+	// always reads the six neighbours together
+	constexpr bool LEAN = adj_lean<R, C>() && S == 1 && (sizeof(R) == 4 || DEG <= 4);
+	constexpr bool BATCH = sizeof(R) == 4 || LEAN;
 	// LDS: buf0, buf1, wb0, wb1 of bs values each, then hx [site][3] and the reduction scratch; in
-	// fp32 bs is the constant C * BT * S, so the four buffers sit at fixed offsets from one
-	// neighbour address and a read needs no address add (the offsets fit the 16-bit immediate
-	// when C * BT * S * 4 B <= 16 KiB)
+	// fp32 and in the lean fp64 form bs is the constant C * BT * S, so the four buffers sit at
+	// fixed offsets from one neighbour address and a read needs no address add (the offsets fit
+	// the 16-bit immediate when C * BT * S * sizeof(R) <= 16 KiB)
 	const size_t bs = adj_buf_stride<R, C, BT, S>(ns);
 	R *buf0 = reinterpret_cast<R *>(smem_raw);
 	R *buf1 = buf0 + bs;
@@ -219,7 +231,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 				__syncthreads();
 #pragma unroll
 				for(int j = 0; j < S; j++)
-					if(act[j]) gather6<R, C>(buf0, nbr[j], cf[j], a.mu, v[j], w[j]);
+					if(act[j]) gather6<R, C, BATCH>(buf0, nbr[j], cf[j], a.mu, v[j], w[j]);
 				if(a.cplx[p]){
 #pragma unroll
 					for(int j = 0; j < S; j++)
@@ -231,7 +243,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 					for(int j = 0; j < S; j++){
 						if(!act[j]) continue;
 						R z[C];
-						gather6<R, C>(buf1, nbr[j], cf[j], a.mu, w[j], z);
+						gather6<R, C, BATCH>(buf1, nbr[j], cf[j], a.mu, w[j], z);
 #pragma unroll
 						for(int c = 0; c < C; c++) v[j][c] = v[j][c] + fmac(a.a_quad[p], z[c], a.a_lin[p] * w[j][c]);
 					}
@@ -261,10 +273,9 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			wv[j][c] = val;
 		}
 
-	// fp32 with one site per thread: every lane loads and gathers, so no lane needs zero values
-	// of its own, and the degree is the constant DEG (with two sites per thread, or in fp64, the
-	// guarded form and the run-time degree hold fewer registers)
-	constexpr bool ALL_LANES = sizeof(R) == 4 && S == 1;
+	// lean form: every lane loads and gathers, so no lane needs zero values of its own, and the
+	// degree is the constant DEG
+	constexpr bool ALL_LANES = LEAN;
 	const int deg = ALL_LANES ? DEG : a.deg;
 	// coefficients and T of slice tau-1 are loaded while slice tau is processed (each thread
 	// reads back only the scratch entries it wrote itself)
@@ -354,7 +365,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 				// with ALL_LANES a thread past the last site gathers too (neighbour 0, in bounds)
 				// and stores nothing
 				R tn[C];
-				if(ALL_LANES || act[j]) gather6<R, C>(bb, nbr[j], cf[j], a.mu, t[j], tn);
+				if(ALL_LANES || act[j]) gather6<R, C, BATCH>(bb, nbr[j], cf[j], a.mu, t[j], tn);
 				else
 #pragma unroll
 					for(int c = 0; c < C; c++) tn[c] = 0;
@@ -401,9 +412,9 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			for(int j = 0; j < S; j++){
 				if(!act[j]) continue;
 				R wn[C];
-				// fp32: all six neighbour reads are issued before any of them is used (fp64 reads
-				// each just before its sums, which holds fewer registers; same arithmetic)
-				constexpr int NV = sizeof(R) == 4 ? 6 : 1;
+				// BATCH: all six neighbour reads are issued before any of them is used (otherwise
+				// each is read just before its sums; same arithmetic)
+				constexpr int NV = BATCH ? 6 : 1;
 				R nv[NV][C];
 				if constexpr(NV == 6){
 #pragma unroll
