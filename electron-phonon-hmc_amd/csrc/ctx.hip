@@ -882,24 +882,33 @@ int ephmc_autotune(ephmc_ctx *c, ephmc_tune_result *res)
 		if((rc = set_zero(c, c->d_mhat, c->mat)) != EPHMC_OK || (rc = add_identity(c, c->d_mhat)) != EPHMC_OK) break;
 		// force sweep widths: (4 C + 3) N values of LDS plus the reduction scratch; both orders of
 		// the first W phase (its own barrier, or W_0 written ahead)
-		r.adjoint_ms = 1e30;
-		int best_early = 1;
+		struct Sweep { unsigned C; int early; double ms; };
+		std::vector<Sweep> sweeps;
+		unsigned max_nwg = 0;
 		for(unsigned C : {1u, 2u, 4u}){
 			const unsigned bt = ns <= 256 ? 256 : (ns <= 512 ? 512 : 1024);
 			if((4 * (size_t)C * ns + 3 * (size_t)ns) * rsz + bt * 8 > 160 * 1024 || C > ns) continue;
-			const unsigned nwg = (ns + C - 1) / C;
-			if((rc = reserve_partials(c, nwg)) != EPHMC_OK) break;
-			c->cols = C;
-			c->nwg = nwg;
-			for(int early = 0; early < 2 && rc == EPHMC_OK; early++){
-				c->adj_w0_early = early;
+			max_nwg = std::max(max_nwg, (ns + C - 1) / C);
+			for(int early = 0; early < 2; early++) sweeps.push_back({C, early, 1e30});
+		}
+		if(sweeps.empty()){ set_error("autotune: no force-sweep width fits %u sites", ns); rc = EPHMC_ERR_ARG; break; }
+		// partials for the widest grid first, then rounds over all shapes as for the chain
+		if((rc = reserve_partials(c, max_nwg)) != EPHMC_OK) break;
+		for(int round = 0; round < 3 && rc == EPHMC_OK; round++){
+			for(Sweep &sw : sweeps){
+				c->cols = sw.C;
+				c->nwg = (ns + sw.C - 1) / sw.C;
+				c->adj_w0_early = sw.early;
 				double ms;
-				if((rc = time_ms(c, e0, e1, 3, &ms, [&]{ return force_dispatch(c, prec, c->d_mhat, c->d_pdot, nullptr); })) != EPHMC_OK) break;
-				if(ms < r.adjoint_ms){ r.adjoint_ms = ms; r.adjoint_cols = C; best_early = early; }
+				if((rc = time_ms(c, e0, e1, 2, &ms, [&]{ return force_dispatch(c, prec, c->d_mhat, c->d_pdot, nullptr); })) != EPHMC_OK) break;
+				if(ms < sw.ms) sw.ms = ms;
 			}
-			if(rc != EPHMC_OK) break;
 		}
 		if(rc != EPHMC_OK) break;
+		r.adjoint_ms = 1e30;
+		int best_early = 1;
+		for(const Sweep &sw : sweeps)
+			if(sw.ms < r.adjoint_ms){ r.adjoint_ms = sw.ms; r.adjoint_cols = sw.C; best_early = sw.early; }
 		c->cols = r.adjoint_cols;
 		c->nwg = (ns + c->cols - 1) / c->cols;
 		c->adj_w0_early = best_early;
