@@ -500,6 +500,34 @@ __global__ void broadcast_slice_kernel(double *__restrict__ pdot, unsigned loc, 
 		pdot[e] = pdot[e % loc];
 }
 
+// LDS bytes of the force sweep at C columns per workgroup, for the launch shape that
+// launch_adjoint_c picks for ns sites (the same rule, so a width chosen by this size launches)
+template <typename R, int C>
+static size_t adj_lds_for(unsigned ns)
+{
+	if(ns <= 256) return adj_lds_bytes<R, C, 256, 1>(ns);
+	if(ns <= 512) return adj_lds_bytes<R, C, 512, 1>(ns);
+	if(ns <= 1024 && sites_per_thread("EPHMC_ADJOINT_SITES") == 2) return adj_lds_bytes<R, C, 512, 2>(ns);
+	if(ns <= 1024) return adj_lds_bytes<R, C, 1024, 1>(ns);
+	return adj_lds_bytes<R, C, 1024, 2>(ns);
+}
+
+size_t adjoint_lds(unsigned ns, int prec, unsigned C)
+{
+	if(prec == EPHMC_FP64){
+		switch(C){
+			case 1: return adj_lds_for<double, 1>(ns);
+			case 2: return adj_lds_for<double, 2>(ns);
+			default: return adj_lds_for<double, 4>(ns);
+		}
+	}
+	switch(C){
+		case 1: return adj_lds_for<float, 1>(ns);
+		case 2: return adj_lds_for<float, 2>(ns);
+		default: return adj_lds_for<float, 4>(ns);
+	}
+}
+
 unsigned adjoint_cols(const ephmc_ctx *c, int prec)
 {
 	return c->cols ? c->cols : (prec == EPHMC_FP64 ? 2u : 4u);

@@ -13,6 +13,7 @@
 // the rows j and jb (b = 0..2) of both propagators in LDS, from where its threads gather the
 // entries at i and ia; block sums per (tau, j-chunk), then a fixed-order pass per tau.
 #include "internal.h"
+#include "alloc_group.h"
 
 namespace ephmc {
 
@@ -23,6 +24,25 @@ constexpr unsigned CC_J = 8;   // values of j per workgroup
 static size_t cc_partials(unsigned ns, unsigned nt)
 {
 	return (size_t)nt * ((ns + CC_J - 1) / CC_J) * 9;
+}
+
+// R_t0 and R_0t (store values each) and the result block (res values), all or none
+static int cc_alloc(ephmc_ctx *c, size_t store, size_t res)
+{
+	if(c->d_Rt0) (void)hipFree(c->d_Rt0);
+	if(c->d_R0t) (void)hipFree(c->d_R0t);
+	if(c->d_ccg) (void)hipFree(c->d_ccg);
+	void **ptrs[3] = {(void **)&c->d_Rt0, (void **)&c->d_R0t, (void **)&c->d_ccg};
+	const size_t bytes[3] = {store * sizeof(double), store * sizeof(double), res * sizeof(double)};
+	const bool ok = alloc_all_or_none(ptrs, bytes, 3,
+		[](void **p, size_t b){ return hipMalloc(p, b) == hipSuccess; },
+		[](void *p){ (void)hipFree(p); });
+	if(!ok){
+		(void)hipGetLastError();
+		set_error("correlator buffers (%zu MB) could not be allocated", 2 * store * 8 >> 20);
+		return EPHMC_ERR_ALLOC;
+	}
+	return EPHMC_OK;
 }
 
 // grid (j-chunks of CC_J, tau).  rows (dynamic LDS, 8 N doubles): R_t0[j], R_0t[j], R_t0[jb]
@@ -130,14 +150,11 @@ extern "C" int ephmc_cc_correlator(ephmc_ctx *c, const double *x, const double *
 	}
 	const unsigned ns = c->ns, nt = c->nt;
 	const size_t mat = c->mat;
-	if(!c->d_Rt0){
-		if(hipMalloc((void **)&c->d_Rt0, nt * mat * sizeof(double)) != hipSuccess
-			|| hipMalloc((void **)&c->d_R0t, nt * mat * sizeof(double)) != hipSuccess
-			|| hipMalloc((void **)&c->d_ccg, (nt * 9 + cc_partials(ns, nt)) * sizeof(double)) != hipSuccess){
-			(void)hipGetLastError();
-			set_error("correlator buffers (%zu MB) could not be allocated", 2 * nt * mat * 8 >> 20);
-			return EPHMC_ERR_ALLOC;
-		}
+	if(!c->d_Rt0 || !c->d_R0t || !c->d_ccg){
+		// all three or none: a failed allocation releases the others, so the next call retries
+		// instead of running on a partial set
+		const int rc = cc_alloc(c, nt * mat, nt * 9 + cc_partials(ns, nt));
+		if(rc != EPHMC_OK) return rc;
 	}
 	EPHMC_TRY(timer_begin(c, "correlator"));
 	EPHMC_TRY(coefficients(c, x, false));

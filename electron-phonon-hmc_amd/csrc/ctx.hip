@@ -58,7 +58,7 @@ static int make_poly(int order, Poly &P)
 	for(int i = 0; i < np; i++){
 		const double re = .5 * tab[ord[i]][0] / np, im = .5 * tab[ord[i]][1] / np;
 		P.a_lin[i] = 2 * re;
-		P.a_quad[i] = re * re + im * im;
+		P.a_quad[i] = std::pow(std::hypot(re, im), 2);   // |a|^2 as pow(cabs(a), 2), organic_dense_matrix.c:169
 		P.cplx[i] = im != 0;
 		const double f[3] = {1, P.a_lin[i], P.cplx[i] ? P.a_quad[i] : 0};
 		const int fd = P.cplx[i] ? 2 : 1;
@@ -148,16 +148,14 @@ int timer_end(ephmc_ctx *c, const char *group)
 // Columns per workgroup of the adjoint sweep: the widest that fits LDS and keeps at least one
 // workgroup per CU (at 32x32: 4 columns, 0.13 ms per 32-slice launch in fp32 and 0.32 ms in
 // fp64).  EPHMC_ADJOINT_COLS overrides it for tuning sweeps.
-static unsigned adjoint_cols_auto(unsigned ns, size_t rsize)
+static unsigned adjoint_cols_auto(unsigned ns, int prec)
 {
 	if(const char *e = getenv("EPHMC_ADJOINT_COLS")){
 		const unsigned v = (unsigned)atoi(e);
 		if(v == 1 || v == 2 || v == 4) return v;
 	}
-	for(unsigned C = 4; C > 1; C >>= 1){
-		const size_t lds = (4 * (size_t)C * ns + 3 * (size_t)ns) * rsize;
-		if(lds <= 160 * 1024 && (ns + C - 1) / C >= 256) return C;
-	}
+	for(unsigned C = 4; C > 1; C >>= 1)
+		if(adjoint_lds(ns, prec, C) <= 160 * 1024 && (ns + C - 1) / C >= 256) return C;
 	return 1;
 }
 
@@ -348,7 +346,7 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 	else c->ckpt = (size_t)c->nt * c->mat * rsz <= ((size_t)4 << 30) ? 1u : std::min(16u, c->nt);
 	c->nseg = (c->nt + c->ckpt - 1) / c->ckpt;
 	c->span = c->ckpt == 1 ? std::min(32u, c->nt) : c->ckpt;
-	c->cols = adjoint_cols_auto(ns, rsz);
+	c->cols = adjoint_cols_auto(ns, cfg->precision);
 	c->adj_w0_early = 1;
 	if(const char *e = getenv("EPHMC_ADJOINT_W0EARLY")) c->adj_w0_early = atoi(e) != 0;
 	c->nwg = (ns + c->cols - 1) / c->cols;
@@ -633,6 +631,19 @@ static int trajectory_body(ephmc_ctx *c)
 	return EPHMC_OK;
 }
 
+// Releases a captured trajectory graph (it is captured again on the next trajectory).
+static void drop_graph(ephmc_ctx *c)
+{
+	if(c->graph_exec){
+		(void)hipGraphExecDestroy(c->graph_exec);
+		c->graph_exec = nullptr;
+	}
+	if(c->graph){
+		(void)hipGraphDestroy(c->graph);
+		c->graph = nullptr;
+	}
+}
+
 // Captures trajectory_body into a hipGraph.  A capture that fails leaves the context on the
 // eager path (graph_failed), with the reason in ephmc_last_error.
 static int capture_body(ephmc_ctx *c)
@@ -812,9 +823,16 @@ int ephmc_autotune(ephmc_ctx *c, ephmc_tune_result *res)
 {
 	if(!c){ set_error("null context"); return EPHMC_ERR_ARG; }
 	const int prec = c->cfg.precision;
-	const size_t rsz = prec == EPHMC_FP64 ? 8 : 4;
 	const unsigned ns = c->ns;
 	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	// a captured trajectory graph holds the current buffers and launch shapes: drop it before
+	// either can change (it is captured again on the next trajectory)
+	drop_graph(c);
+	// the shapes in force on entry, restored if tuning fails part way
+	const unsigned cols0 = c->cols, nwg0 = c->nwg;
+	const int early0 = c->adj_w0_early;
+	const unsigned ccols0[2] = {c->chain_cols_set[0], c->chain_cols_set[1]};
+	const int cstrip0[2] = {c->chain_strip_set[0], c->chain_strip_set[1]};
 	const bool timing = c->timing;
 	c->timing = false;
 	hipEvent_t e0, e1;
@@ -886,8 +904,7 @@ int ephmc_autotune(ephmc_ctx *c, ephmc_tune_result *res)
 		std::vector<Sweep> sweeps;
 		unsigned max_nwg = 0;
 		for(unsigned C : {1u, 2u, 4u}){
-			const unsigned bt = ns <= 256 ? 256 : (ns <= 512 ? 512 : 1024);
-			if((4 * (size_t)C * ns + 3 * (size_t)ns) * rsz + bt * 8 > 160 * 1024 || C > ns) continue;
+			if(adjoint_lds(ns, prec, C) > 160 * 1024 || C > ns) continue;
 			max_nwg = std::max(max_nwg, (ns + C - 1) / C);
 			for(int early = 0; early < 2; early++) sweeps.push_back({C, early, 1e30});
 		}
@@ -917,14 +934,15 @@ int ephmc_autotune(ephmc_ctx *c, ephmc_tune_result *res)
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
 	c->timing = timing;
-	if(rc != EPHMC_OK) return rc;
-	if(c->graph_exec){
-		(void)hipGraphExecDestroy(c->graph_exec);
-		c->graph_exec = nullptr;
-	}
-	if(c->graph){
-		(void)hipGraphDestroy(c->graph);
-		c->graph = nullptr;
+	if(rc != EPHMC_OK){
+		c->cols = cols0;
+		c->nwg = nwg0;
+		c->adj_w0_early = early0;
+		for(int q = 0; q < 2; q++){
+			c->chain_cols_set[q] = ccols0[q];
+			c->chain_strip_set[q] = cstrip0[q];
+		}
+		return rc;
 	}
 	if(res) *res = r;
 	return EPHMC_OK;

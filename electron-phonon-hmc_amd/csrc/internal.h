@@ -204,6 +204,8 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M);
 // adjoint.hip
 int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p, const double *fscale);
 int force_constant(ephmc_ctx *c, int prec, const double *minv, double *pdot, const double *fscale);
+// LDS bytes of one force-sweep workgroup at C columns (the launch shape used for ns sites)
+size_t adjoint_lds(unsigned ns, int prec, unsigned C);
 
 // bosons.hip
 int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick);

@@ -258,12 +258,19 @@ class Context:
         return dx.download(), dp_.download()
 
     # -- chain ------------------------------------------------------------------------
+    def _host_field(self, a, what):
+        """Contiguous float64 copy of a host field of dim values (the C side reads dim of them)."""
+        a = np.ascontiguousarray(a, np.float64).ravel()
+        if a.size != self.dim:
+            raise ValueError("%s: %d values, the context needs dim = %d" % (what, a.size, self.dim))
+        return a
+
     def chain_init(self, normals):
-        normals = np.ascontiguousarray(normals, np.float64)
+        normals = self._host_field(normals, "normals")
         check(LIB.ephmc_chain_init(self.h, _dp(normals)))
 
     def trajectory(self, normals, uniform, wait=True):
-        normals = np.ascontiguousarray(normals, np.float64)
+        normals = self._host_field(normals, "normals")
         rec = Record()
         check(LIB.ephmc_trajectory(self.h, _dp(normals), uniform, C.byref(rec) if wait else None))
         return rec if wait else None
@@ -279,7 +286,7 @@ class Context:
         return out
 
     def chain_set_x(self, x):
-        x = np.ascontiguousarray(x, np.float64)
+        x = self._host_field(x, "x")
         check(LIB.ephmc_chain_set_x(self.h, _dp(x)))
 
     def chain_minv(self):

@@ -12,20 +12,32 @@
  * expansion, binary-splitting force), so the oracle is also a faithful CPU timing
  * baseline of the reference algorithm.
  *
- * How each part is pinned (see tests/test_oracle_*.py):
- *   lattice tables, MT19937-64, Box-Muller normals, aux writers, flags: bit-exact against
- *     the reference's own sources compiled into oracle/_ref/libref.so;
- *   Taylor pairs: polynomial identity prod(pairs) == sum_{k<=4} z^k/k!;
- *   stencil, slice exponential, M_hat: dense numpy brute force on small lattices and the
- *     closed form for spatially uniform fields (Fourier diagonalisation);
- *   LU log-det / inverse: numpy slogdet / inv;
- *   force (construct_all_QM1P + refine + projection): central finite differences of the
- *     oracle's own log-det, the closed form for uniform fields, and the constant-force
- *     single-slice variant;
- *   FFT conventions: numpy rfft / irfft (unnormalised FFTW half-spectrum convention);
- *   boson flow / energies: exact invariants (kappa=0 conserves H, reversibility, Parseval);
- *   transient localisation: numpy eigh + brute-force binning.
- * Parity unpinned: SCHUR_FORM_INV (needs LAPACK dgees; not restated, returns an error).
+ * How each function is pinned (tests/test_oracle_ref.py, test_oracle_math.py, test_oracle_pins.py):
+ *   orc_mt_*, orc_random_vector, orc_construct_triangular, orc_construct_contraction_mat,
+ *     orc_contract_greens, orc_fprint_results, orc_fprint_corr: bit-exact against the reference's
+ *     own sources compiled into oracle/_ref/libref.so, plus the C++ standard's MT19937-64 answer;
+ *   orc_set_taylor: every table's pair product equals sum_{k<=order} z^k/k! (orders 2, 4, 6, 8),
+ *     and partial products follow the hand-derived application order;
+ *   orc_scale_params: closed-form numbers of organic_main.c:99-116 (both branches);
+ *   orc_apply_mu_K_to_v/M, orc_apply_exp_mu_K_to_v/M, orc_construct_M_hat, orc_transpose: dense
+ *     numpy brute force, and the closed form for spatially uniform fields;
+ *   orc_logdet_inv: numpy slogdet / inv;
+ *   orc_construct_all_QM1P: dense numpy products; orc_refine_QM1P, orc_force,
+ *     orc_construct_QM1P: central finite differences of the log det, the uniform-field closed
+ *     form, and the constant-force slice-0 identity;
+ *   orc_r2c_many / orc_c2r_many: numpy rfft / irfft (FFTW's unnormalised half spectrum);
+ *   orc_sample_*, orc_energy_momenta, orc_evolve_bosons: numpy colouring, the full-spectrum
+ *     energy and the explicit mode rotation (energy-conserving, h -> -h inverse);
+ *   orc_evolve_bosons_approx_step: the gradient of orc_energy_momenta with the CFA mass and a
+ *     numpy rfft restatement; orc_boson_force: central differences of the boson action;
+ *   orc_update_x, orc_leap_frog: second-order convergence of dH and exact p -> -p reversibility
+ *     in the Fourier-accelerated, NO_FOURIER_ACC and CLASSICAL_FOURIER_ACC schemes;
+ *   orc_hamilton, orc_higher_matsubaras_norm: numpy sums; orc_trajectory: kappa = 0 exactness,
+ *     <exp(-dH)> = 1 and its results columns recomputed from the state;
+ *   orc_cc_correlator: dense numpy evaluation from explicit transfer matrices and the
+ *     plane-wave closed form for fields uniform in space;
+ *   orc_construct_hamilton, orc_apply_current_to_v, orc_eigh, orc_hist_trans_loc, orc_tl_width:
+ *     numpy eigh, brute-force binning and the closed-form bin width.
  */
 #ifndef EPHMC_ORACLE_H
 #define EPHMC_ORACLE_H

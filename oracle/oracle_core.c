@@ -210,7 +210,7 @@ void orc_apply_mu_K_to_M(const orc_model *m, const double *x, double *M, double 
 static void orc_apply_pair(const orc_model *m, const double *x, double *v, double *w, int i){
 	const unsigned ns = m->ns;
 	const double a_lin = 2 * m->a_re[i];
-	const double a_quad = m->a_re[i]*m->a_re[i] + m->a_im[i]*m->a_im[i];
+	const double a_quad = pow(hypot(m->a_re[i], m->a_im[i]), 2);   /* pow(cabs(a), 2), organic_dense_matrix.c:169 */
 	double *z = w + ns;
 	orc_apply_mu_K_to_v(m, x, v, w, m->mu);
 	if(m->a_im[i] != 0){

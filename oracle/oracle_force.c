@@ -36,7 +36,7 @@ void orc_refine_QM1P(const orc_model *m, const double *x, double *QM1P, double *
 	for(int j = 0; j < m->np; j++){
 		const int cplx = m->a_im[j] != 0;
 		const double a_lin = 2*m->a_re[j];
-		const double a_quad = m->a_re[j]*m->a_re[j] + m->a_im[j]*m->a_im[j];
+		const double a_quad = pow(hypot(m->a_re[j], m->a_im[j]), 2);   /* pow(cabs(a), 2), organic_dense_matrix.c:169 */
 		memcpy(loc, cp, mat*sizeof(double));
 		orc_apply_exp_mu_K_to_M(m, xt, loc, w, 0, j);
 		orc_transpose(loc, ns);

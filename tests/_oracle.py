@@ -80,6 +80,7 @@ def oracle():
             "orc_sample_fourier_momenta": (None, [C.POINTER(Model), C.POINTER(MT), dp]),
             "orc_energy_momenta": (C.c_double, [C.POINTER(Model), dp]),
             "orc_evolve_bosons": (None, [C.POINTER(Model), dp, dp, C.c_double]),
+            "orc_evolve_bosons_approx_step": (None, [C.POINTER(Model), dp, dp, C.c_double]),
             "orc_boson_force": (None, [C.POINTER(Model), dp, dp, C.c_int]),
             "orc_higher_matsubaras_norm": (C.c_double, [C.POINTER(Model), dp]),
             "orc_hamilton": (C.c_double, [C.POINTER(Model), dp, C.c_double, C.c_double]),
