@@ -480,14 +480,21 @@ int chain_range(ephmc_ctx *c, int prec, const double *src, void *dst, int tau0, 
 	return launch_chain<float>(c, a, chain_cols(c, prec));
 }
 
-// apply_exp_mu_K_to_M with partial pair products (revert/stop, organic_dense_matrix.c:186-194)
-// for a single slice whose 3N couplings are x_tau.  Uses d_coef64 slot 0 as scratch.
-int apply_exp_partial(ephmc_ctx *c, const double *x_tau, double *M, int revert, int stop)
+// fp64 coefficients of the single slice whose 3N couplings are x_tau, into d_coef64 slot 0
+int slice_coefficients(ephmc_ctx *c, const double *x_tau)
 {
 	const unsigned ns = c->ns;
 	coef_kernel<<<(6 * ns + 255) / 256, 256, 0, c->stream>>>(x_tau, c->d_link, c->cfg.t[0], c->cfg.t[1], c->cfg.t[2],
 		c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], ns, 1, c->d_coef64, nullptr);
 	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
+// apply_exp_mu_K_to_M with partial pair products (revert/stop, organic_dense_matrix.c:186-194)
+// for a single slice whose 3N couplings are x_tau.  Uses d_coef64 slot 0 as scratch.
+int apply_exp_partial(ephmc_ctx *c, const double *x_tau, double *M, int revert, int stop)
+{
+	EPHMC_TRY(slice_coefficients(c, x_tau));
 	ChainArgs<double> a = make_args<double>(c, c->d_coef64);
 	a.np = 0;
 	auto add = [&](int p){

@@ -518,8 +518,28 @@ int ephmc_logdet_inv(ephmc_ctx *c, double *M_hat, double *logdet, int *sign)
 	EPHMC_TRY(timer_begin(c, "gj"));
 	EPHMC_TRY(gj_logdet_inverse(c, M_hat));
 	EPHMC_TRY(timer_end(c, "gj"));
-	if(logdet) EPHMC_TRY(read_scal(c, SCAL_LOGDET, logdet));
-	if(sign){ double s; EPHMC_TRY(read_scal(c, SCAL_SIGN, &s)); *sign = s < 0 ? -1 : 1; }
+	if(!logdet && !sign) return EPHMC_OK;
+	double ld, sg, sing;
+	EPHMC_TRY(read_scal(c, SCAL_LOGDET, &ld));
+	EPHMC_TRY(read_scal(c, SCAL_SIGN, &sg));
+	EPHMC_TRY(read_scal(c, SCAL_SINGULAR, &sing));
+	if(logdet) *logdet = ld;
+	if(sign) *sign = sg < 0 ? -1 : 1;
+	// a zero pivot: log|det| = -inf as in the reference (organic_dense_matrix.c:582-584)
+	if(sing != 0 || ld == -HUGE_VAL){
+		if(sing != 0) set_error("singular matrix: column %d has no nonzero pivot; log|det| = -inf", (int)sing);
+		else set_error("singular matrix: zero pivot; log|det| = -inf");
+		return EPHMC_ERR_NUMERIC;
+	}
+	return EPHMC_OK;
+}
+
+int ephmc_last_pivots(ephmc_ctx *c, int *ipiv)
+{
+	if(!c || !ipiv){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	EPHMC_HIP(hipMemcpyAsync(ipiv, c->d_piv, c->ns * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+	EPHMC_HIP(hipStreamSynchronize(c->stream));
+	for(unsigned i = 0; i < c->ns; i++) ipiv[i] += 1;        // LAPACK numbering
 	return EPHMC_OK;
 }
 

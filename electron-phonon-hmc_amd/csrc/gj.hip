@@ -218,7 +218,10 @@ __global__ __launch_bounds__(256) void gj_first_kernel(const double *__restrict_
 	constexpr int NB = GJ_NB;
 	__shared__ GjInvLds sh;
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	if(threadIdx.x == 0) scal[SCAL_GJ_FLAG] = 0;     // no multiplier above 1 seen yet
+	if(threadIdx.x == 0){
+		scal[SCAL_GJ_FLAG] = 0;     // no multiplier above 1 seen yet
+		scal[SCAL_SINGULAR] = 0;
+	}
 	__syncthreads();
 	double a[8];
 #pragma unroll
@@ -531,8 +534,14 @@ __global__ void gj_copy_kernel(const double *__restrict__ W, double *__restrict_
 }
 
 // Fallback: in-place Gauss-Jordan with partial pivoting on M (one workgroup).  Runs only when
-// the fast path saw a multiplier above 1.
-__global__ __launch_bounds__(1024) void gj_fallback_kernel(double *__restrict__ M, unsigned n, double *__restrict__ scal)
+// the fast path saw a multiplier above 1 (or a non-finite one).  The row interchanges go to
+// piv_out (LAPACK's sequential form, 0-based: row g was exchanged with row piv_out[g] at step g).
+// A column without a nonzero candidate pivot makes the matrix singular: log|det| = -inf (the
+// reference's sum of log|U_ii| with a zero U_ii, organic_dense_matrix.c:582-584), the step is
+// recorded in SCAL_SINGULAR, and M is filled with NaN, so that a force or a Metropolis step built
+// on it rejects instead of using a partial inverse.
+__global__ __launch_bounds__(1024) void gj_fallback_kernel(double *__restrict__ M, unsigned n, double *__restrict__ scal,
+		int *__restrict__ piv_out)
 {
 	extern __shared__ double fsh[];
 	double *colg = fsh;                                   // n: multipliers of the step
@@ -560,8 +569,23 @@ __global__ __launch_bounds__(1024) void gj_fallback_kernel(double *__restrict__ 
 		__syncthreads();
 		best = rv[0]; bidx = ri[0];
 		for(int w = 1; w < 16; w++) if(rv[w] > best || (rv[w] == best && ri[w] < bidx)){ best = rv[w]; bidx = ri[w]; }
+		if(best == 0){
+			// every candidate in column g is zero (best is the same in every thread): singular
+			for(size_t e = tid; e < (size_t)n * n; e += 1024) M[e] = __longlong_as_double(0x7ff8000000000000LL);
+			if(tid == 0){
+				scal[SCAL_LOGDET] = -HUGE_VAL;
+				scal[SCAL_SIGN] = 1.0;
+				scal[SCAL_SINGULAR] = g + 1;
+				scal[SCAL_GJ_FALLBACKS] += 1;
+			}
+			for(unsigned r = g + tid; r < n; r += 1024) piv_out[r] = (int)r;
+			return;
+		}
 		const unsigned p = bidx < (int)n ? (unsigned)bidx : g;
-		if(tid == 0) piv[g] = (int)p;
+		if(tid == 0){
+			piv[g] = (int)p;
+			piv_out[g] = (int)p;
+		}
 		if(p != g)
 			for(unsigned cc = tid; cc < n; cc += 1024){
 				const double t = M[(size_t)cc * n + g];
@@ -641,7 +665,7 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M)
 	}
 	const size_t lds = (size_t)n * (sizeof(double) + sizeof(int));
 	(void)hipFuncSetAttribute((const void *)gj_fallback_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-	gj_fallback_kernel<<<1, 1024, lds, c->stream>>>(M, n, c->d_scal);
+	gj_fallback_kernel<<<1, 1024, lds, c->stream>>>(M, n, c->d_scal, c->d_piv);
 	EPHMC_LAUNCHED();
 	return EPHMC_OK;
 }

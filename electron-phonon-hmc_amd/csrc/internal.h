@@ -176,6 +176,7 @@ enum Scalar {            // layout of d_scal
 	SCAL_GJ_FLAG = 12,    // a multiplier above 1 was seen by the Gauss-Jordan fast path
 	SCAL_GJ_FALLBACKS = 13, // number of pivoted fallback inversions run
 	SCAL_FSCALE = 14,     // force factor exp(-logdet) of SINGLE_PARTICLE
+	SCAL_SINGULAR = 15,   // 1 + step of a zero pivot column in the last inversion, or 0
 	SCAL_COUNT = 16
 };
 
@@ -191,6 +192,7 @@ int chain_forward(ephmc_ctx *c, int prec, const void *src, void *dst, void *ck, 
 		int descending, bool from_identity);
 int chain_range(ephmc_ctx *c, int prec, const double *src, void *dst, int tau0, int dir, int ntau);
 int apply_exp_partial(ephmc_ctx *c, const double *x_tau, double *M, int revert, int stop);
+int slice_coefficients(ephmc_ctx *c, const double *x_tau);
 int transpose_plus_identity(ephmc_ctx *c, int prec, const void *T, double *M, int add_identity);
 int convert_f64_to_f32(ephmc_ctx *c, const double *a, float *b, size_t n);
 int add_identity(ephmc_ctx *c, double *M);

@@ -85,6 +85,15 @@ def _load():
         "ephmc_timing": [vp, C.c_int],
         "ephmc_timing_get": [vp, C.c_char_p, dp, C.POINTER(C.c_ulong)],
         "ephmc_autotune": [vp, C.POINTER(TuneResult)],
+        "ephmc_last_pivots": [vp, C.POINTER(C.c_int)],
+        "ephmc_apply_mu_K": [vp, vp, vp, vp, C.c_uint],
+        "ephmc_apply_pairs": [vp, vp, vp, C.c_uint, C.c_int, dp, dp],
+        "ephmc_apply_exp_cols": [vp, vp, vp, C.c_uint, C.c_int, C.c_int],
+        "ephmc_refine_QM1P": [vp, vp, vp],
+        "ephmc_construct_QM1P": [vp, vp, vp, vp, C.c_int],
+        "ephmc_construct_all_QM1P": [vp, vp, vp, vp],
+        "ephmc_boson_force": [vp, vp, vp, C.c_int],
+        "ephmc_evolve_bosons_approx_step": [vp, vp, vp, C.c_double],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -319,6 +328,28 @@ class Context:
         res = np.zeros(normals.shape[0])
         check(LIB.ephmc_tl_hist(self.h, _dp(normals), normals.shape[0], _dp(cm), width, nbins, _dp(greens), _dp(res)))
         return greens, res
+
+    def last_pivots(self):
+        """LAPACK-style 1-based row interchanges of the last log-det / inverse."""
+        out = np.zeros(self.ns, np.int32)
+        check(LIB.ephmc_last_pivots(self.h, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
+    def logdet_inv_status(self, M):
+        """(status code, log det, sign, inverse): a singular matrix gives EPHMC_ERR_NUMERIC (-5)
+        with log det = -inf instead of raising."""
+        dm = self.array(self.mat, M)
+        ld, sg = C.c_double(), C.c_int()
+        rc = LIB.ephmc_logdet_inv(self.h, dm.ptr, C.byref(ld), C.byref(sg))
+        if rc not in (0, -5):
+            check(rc)
+        return rc, ld.value, sg.value, dm.download()
+
+    def construct_all_QM1P(self, x, Minv):
+        dx, dq = self.array(self.dim, x), self.array(self.nt * self.mat)
+        dm = self.array(self.mat, Minv) if Minv is not None else None
+        check(LIB.ephmc_construct_all_QM1P(self.h, dx.ptr, dm.ptr if dm is not None else None, dq.ptr))
+        return dq.download()
 
     def gj_fallbacks(self):
         v = C.c_ulong()

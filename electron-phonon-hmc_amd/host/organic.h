@@ -80,25 +80,49 @@ organic_plan organic_plan_create(double *x, double complex *xc, double *t, doubl
 		unsigned flavors, unsigned l1, unsigned l2, unsigned nt, unsigned steps, double traj_length, organic_flags *mode);
 void organic_plan_destroy(organic_plan plan);
 
-/* ---- organic_dense_matrix.h (GPU) */
+/* ---- organic_dense_matrix.h.  GPU: the stencil and pair products, M_hat, the inverse, QM1P
+ * and refine, the correlator, construct_hamilton.  Host: the small dense helpers (mat_mul,
+ * mat_mul_tr, mat_mat_mul, construct_id, transpose, transpose_rect, trace), apply_current_to_v
+ * (one vector) and phonon_correlator (sums over x). */
+void mat_mul(double *m, double *x, double *y, unsigned n);
+void mat_mul_tr(double *m, double *x, double *y, unsigned n);
+void mat_mat_mul(double *m1, double *m2, double *y, unsigned n);
 void construct_id(double *m, unsigned n);
 void transpose(double *m, unsigned n);
+void transpose_rect(double *x, double *y, unsigned n1, unsigned n2);
 double trace(double *m, unsigned n);
+void apply_mu_K_to_v(double *x, double *v, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn);
+void apply_mu_K_to_M(double *x, double *M, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn);
+void set_taylor_pair_order(void);
+void apply_K_plus_1_v(double *x, double *v, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn);
+void apply_2nd_order_mu_K_to_v(double *x, double *v, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, double complex a);
+void apply_pair_expansion_mu_K_to_v(double *x, double *v, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, int revert, int stop);
+void apply_exp_mu_K_to_v(double *x, double *v, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, int revert, int stop);
 void apply_exp_mu_K_to_M(double *x, double *M, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, int revert, int stop);
-void construct_M_hat(double *x, double *M_hat, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, organic_flags *mode);
-double logDetM_and_inv_M_hat(double *M_hat, double *w, int *p, unsigned ns, unsigned flavors, organic_flags *mode);
-void cc_correlator(double *x, double *M_hat, double *R_t0, double *R_0t, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, double *greens, double logDetM, organic_flags *mode);
 void construct_hamilton(double *x, double *ham, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn);
 void apply_current_to_v(double *x, double *v, double *w, double *t, double *kappa, unsigned *nnt, unsigned ns, unsigned nn, unsigned k);
+void construct_M_hat(double *x, double *M_hat, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, organic_flags *mode);
+void construct_QM1P(double *x, double *M_hat, double *QM1P, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, int time, organic_flags *mode);
+void construct_all_QM1P(double *x, double *M_hat, double *QM1P, double *w, int *perm, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, organic_flags *mode);
+void refine_QM1P(double *x, double *QM1P, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, int time);
+void refine_QM1P_approx(double *x, double *QM1P, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, int time);
+void cc_correlator(double *x, double *M_hat, double *R_t0, double *R_0t, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, double *greens, double logDetM, organic_flags *mode);
+void phonon_correlator(double *x, unsigned ns, unsigned nn, unsigned nt, double *greens);
+double logDetM_and_inv_M_hat(double *M_hat, double *w, int *p, unsigned ns, unsigned flavors, organic_flags *mode);
 
-/* ---- organic_bosons.h (GPU) */
+/* ---- organic_bosons.h.  GPU: the colouring, energies, flows and the boson force.  Host:
+ * hot_start and higher_Matsubaras_norm (random stream, sums over x). */
+void hot_start(double *x, double omega, unsigned ns, unsigned nn, unsigned nt);
 double higher_Matsubaras_norm(double *x, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt);
 void sample_free_bosons(double *x, double complex *xc, double omega, unsigned ns, unsigned nn, unsigned nt, const organic_plan *fft);
 void sample_fourier_momenta(double *p, double complex *pc, double omega, unsigned ns, unsigned nn, unsigned nt, const organic_plan *fft, organic_flags *mode);
 double energy_fourier_momenta(double complex *pc, double omega, unsigned ns, unsigned nn, unsigned nt, const organic_plan *fft);
 void evolve_bosons(double complex *xc, double omega, unsigned ns, unsigned nn, unsigned nt, const organic_plan *fft, double h);
+void evolve_bosons_approx_step(double complex *xc, double omega, unsigned ns, unsigned nn, unsigned nt, const organic_plan *fft, double h, organic_flags *mode);
+void boson_force(double *x, double *p_dot, double omega, unsigned ns, unsigned nn, unsigned nt, int overwrite);
 
-/* ---- organic_hmc.h (GPU) */
+/* ---- organic_hmc.h (GPU; check_force is the host loop of finite differences over GPU log dets) */
+double check_force(double *x, double *p, double *p_dot, double *M_hat, double *w, int *perm, double *t, double *kappa, double mu, double omega, unsigned flavors, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, organic_flags *mode);
 double energy_momenta(double *p, double complex *pc, double omega, unsigned ns, unsigned nn, unsigned nt, const organic_plan *fft, organic_flags *mode);
 double hamilton(double *x, double *p, double logDetM, double energyP, double omega, unsigned flavors, unsigned ns, unsigned nn, unsigned nt);
 void force(double *x, double *p_dot, double *M_hat, double *QM1P, double *w, int *perm, double *t, double *kappa, double mu, double omega, unsigned flavors, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, double logDetM, organic_flags *mode);
@@ -108,6 +132,7 @@ short trajectory(double *x, double complex *xc, int *perm, double *t, double *ka
 unsigned run_hmc(double *greens, unsigned l1, unsigned l2, unsigned nt, double *t, double *kappa, double mu, double omega, unsigned flavors, double *contr_mat, unsigned steps, double traj_length, unsigned therm, unsigned meas, unsigned meas_freq, const char *res_name, const char *corr_name, organic_flags *mode);
 
 /* ---- organic_trans_loc.h (GPU) */
+void hist_trans_loc(double *x, double *t, double *kappa, double mu, double *contr_mat, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, double width, double *results, double *greens, FILE *res_out);
 unsigned run_trans_loc(double *greens, unsigned l1, unsigned l2, unsigned nt, double *t, double *kappa, double mu, double *contr_mat, unsigned meas, const char *res_name, const char *corr_name);
 
 #ifdef __cplusplus

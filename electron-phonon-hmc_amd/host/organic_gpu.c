@@ -37,6 +37,8 @@ struct organic_engine {
 	organic_flags mode;
 	size_t dim, mat;
 	double *d_x, *d_p, *d_m, *d_v;          /* staging buffers for host-array calls */
+	double *d_m2;                           /* second N x N staging buffer */
+	double *d_q;                            /* nt N x N matrices of construct_all_QM1P (on first use) */
 	double *x_host;                         /* buffers bound by organic_plan_create */
 	double complex *xc_host;
 	int chain_synced;                       /* device chain state equals the host field */
@@ -87,6 +89,14 @@ static void lattice_extents(const unsigned *nnt, unsigned ns, unsigned *l1, unsi
 
 static int same(double a, double b){ return a == b; }
 
+static void engine_release(struct organic_engine *e)
+{
+	ephmc_free(e->d_x); ephmc_free(e->d_p); ephmc_free(e->d_v); ephmc_free(e->d_m); ephmc_free(e->d_m2);
+	ephmc_free(e->d_q);
+	ephmc_destroy(e->ctx);
+	free(e);
+}
+
 static struct organic_engine *engine_get(unsigned l1, unsigned l2, unsigned nt, const double *t, const double *kappa,
 		double mu, double omega, unsigned flavors, unsigned steps, double traj_length, const organic_flags *mode)
 {
@@ -135,14 +145,12 @@ static struct organic_engine *engine_get(unsigned l1, unsigned l2, unsigned nt, 
 	GPU(ephmc_malloc(e->dim * sizeof(double), (void **)&e->d_p));
 	GPU(ephmc_malloc(e->dim * sizeof(double), (void **)&e->d_v));
 	GPU(ephmc_malloc(e->mat * sizeof(double), (void **)&e->d_m));
+	GPU(ephmc_malloc(e->mat * sizeof(double), (void **)&e->d_m2));
 	int slot = 0;
 	while(slot < MAX_ENGINES && engines[slot]) slot++;
 	if(slot == MAX_ENGINES){
 		/* evict the oldest */
-		struct organic_engine *old = engines[0];
-		ephmc_free(old->d_x); ephmc_free(old->d_p); ephmc_free(old->d_v); ephmc_free(old->d_m);
-		ephmc_destroy(old->ctx);
-		free(old);
+		engine_release(engines[0]);
 		memmove(engines, engines + 1, (MAX_ENGINES - 1) * sizeof(engines[0]));
 		slot = MAX_ENGINES - 1;
 	}
@@ -185,10 +193,9 @@ void organic_plan_destroy(organic_plan plan)
 	if(!plan) return;
 	for(int i = 0; i < MAX_ENGINES; i++)
 		if(engines[i] == plan){
-			ephmc_free(plan->d_x); ephmc_free(plan->d_p); ephmc_free(plan->d_v); ephmc_free(plan->d_m);
-			ephmc_destroy(plan->ctx);
-			free(plan);
+			engine_release(plan);
 			engines[i] = NULL;
+			return;
 		}
 }
 
@@ -256,9 +263,12 @@ double logDetM_and_inv_M_hat(double *M_hat, double *w, int *p, unsigned ns, unsi
 	up(e, e->d_m, M_hat, e->mat);
 	double logdet;
 	int sign;
-	GPU(ephmc_logdet_inv(e->ctx, e->d_m, &logdet, &sign));
+	const int rc = ephmc_logdet_inv(e->ctx, e->d_m, &logdet, &sign);
+	/* a singular M_hat is no error for the caller: log|det| = -inf, as dgetrf's zero U_ii gives
+	 * in the reference; the inverse is then NaN (the Metropolis step rejects) */
+	if(rc != EPHMC_OK && rc != EPHMC_ERR_NUMERIC) GPU(rc);
 	down(e, M_hat, e->d_m, e->mat);
-	if(p) for(unsigned i = 0; i < ns; i++) p[i] = (int)i + 1;
+	if(p) GPU(ephmc_last_pivots(e->ctx, p));
 	if(sign < 0 && flavors % 2) printf("Negative determinant!\n");
 	return logdet;
 }
@@ -577,27 +587,18 @@ unsigned run_hmc(double *greens, unsigned l1, unsigned l2, unsigned nt, double *
 }
 
 /* ---- transient localisation (organic_trans_loc.c, organic_dense_matrix.c:218-249) --------- */
-/* ham = (mu + K(x)) applied to the identity, column by column (one slice, no exponential) */
+/* organic_dense_matrix.c:218-224: ham = (mu + K(x)) applied to the identity (one slice, no
+ * exponential), on the GPU */
 void construct_hamilton(double *x, double *ham, double *w, double *t, double *kappa, double mu, unsigned *nnt,
 		unsigned ns, unsigned nn)
 {
-	const unsigned nn2 = nn / 2;
+	(void)w;
 	construct_id(ham, ns);
-	for(unsigned c = 0; c < ns; c++){
-		double *v = ham + (size_t)c * ns;
-		for(unsigned i = 0; i < ns; i++) w[i] = mu * v[i];
-		for(unsigned i = 0; i < ns; i++)
-			for(unsigned k = 0; k < nn2; k++){
-				const unsigned nb = nnt[i * nn + 2 * k];
-				const double f = t[k] - kappa[k] * x[i * nn2 + k];
-				w[i] += f * v[nb];
-				w[nb] += f * v[i];
-			}
-		memcpy(v, w, ns * sizeof(double));
-	}
+	apply_mu_K_to_M(x, ham, NULL, t, kappa, mu, nnt, ns, nn);
 }
 
-/* w = I_k v: antisymmetric hopping along direction k */
+/* organic_dense_matrix.c:226-249: w = I_k v, the antisymmetric hopping along direction k (one
+ * vector of N values: a host loop) */
 void apply_current_to_v(double *x, double *v, double *w, double *t, double *kappa, unsigned *nnt, unsigned ns,
 		unsigned nn, unsigned k)
 {
@@ -642,4 +643,272 @@ unsigned run_trans_loc(double *greens, unsigned l1, unsigned l2, unsigned nt, do
 	free(normals);
 	if(res_out) fclose(res_out);
 	return meas + nt;
+}
+
+/* ---- the remaining functions of organic_dense_matrix.h, organic_hmc.h, organic_bosons.h and
+ * organic_trans_loc.h ------------------------------------------------------------------------ */
+
+/* Dense helpers of organic_dense_matrix.c:27-55 (not on the hot path; host loops). */
+void mat_mul(double *m, double *x, double *y, unsigned n)
+{
+	for(unsigned i = 0; i < n; i++){
+		double s = 0;
+		for(unsigned k = 0; k < n; k++) s += m[(size_t)i * n + k] * x[k];
+		y[i] = s;
+	}
+}
+
+void mat_mul_tr(double *m, double *x, double *y, unsigned n)
+{
+	for(unsigned k = 0; k < n; k++) y[k] = 0;
+	for(unsigned i = 0; i < n; i++)
+		for(unsigned k = 0; k < n; k++) y[k] += m[(size_t)i * n + k] * x[i];
+}
+
+/* every chunk of m2 replaced by m1 times it; y: n scratch values */
+void mat_mat_mul(double *m1, double *m2, double *y, unsigned n)
+{
+	for(unsigned c = 0; c < n; c++){
+		double *col = m2 + (size_t)c * n;
+		mat_mul(m1, col, y, n);
+		memcpy(col, y, n * sizeof(double));
+	}
+}
+
+/* y (n2 x n1) = x (n1 x n2) transposed */
+void transpose_rect(double *x, double *y, unsigned n1, unsigned n2)
+{
+	for(unsigned i = 0; i < n1; i++)
+		for(unsigned j = 0; j < n2; j++) y[(size_t)j * n1 + i] = x[(size_t)i * n2 + j];
+}
+
+/* The pair order is fixed per engine context when it is created (ephmc_create builds the
+ * application order of organic_dense_matrix.c:133-159), so there is nothing left to set. */
+void set_taylor_pair_order(void)
+{
+}
+
+/* organic_dense_matrix.c:96-120: w = (mu + K(x)) v for the slice couplings x (GPU) */
+void apply_mu_K_to_v(double *x, double *v, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns,
+		unsigned nn)
+{
+	(void)nn;
+	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, 1, NULL);
+	up(e, e->d_x, x, 3 * (size_t)ns);
+	up(e, e->d_v, v, ns);
+	GPU(ephmc_apply_mu_K(e->ctx, e->d_x, e->d_v, e->d_p, 1));
+	down(e, w, e->d_p, ns);
+}
+
+/* organic_dense_matrix.c:123-131: every chunk of M <- A chunk (GPU) */
+void apply_mu_K_to_M(double *x, double *M, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns,
+		unsigned nn)
+{
+	(void)w; (void)nn;
+	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, 1, NULL);
+	up(e, e->d_x, x, 3 * (size_t)ns);
+	up(e, e->d_m, M, e->mat);
+	GPU(ephmc_apply_mu_K(e->ctx, e->d_x, e->d_m, e->d_m2, ns));
+	down(e, M, e->d_m2, e->mat);
+}
+
+static void pairs_on_vector(double *x, double *v, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns,
+		int np, const double *al, const double *aq)
+{
+	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, 1, NULL);
+	up(e, e->d_x, x, 3 * (size_t)ns);
+	up(e, e->d_v, v, ns);
+	GPU(ephmc_apply_pairs(e->ctx, e->d_x, e->d_v, 1, np, al, aq));
+	down(e, v, e->d_v, ns);
+}
+
+/* organic_dense_matrix.c:160-165: v <- (1 + A) v (GPU) */
+void apply_K_plus_1_v(double *x, double *v, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns,
+		unsigned nn)
+{
+	(void)w; (void)nn;
+	const double al = 1, aq = 0;
+	pairs_on_vector(x, v, t, kappa, mu, nnt, ns, 1, &al, &aq);
+}
+
+/* organic_dense_matrix.c:168-183: v <- (1 + 2 Re(a) A + |a|^2 A^2) v, or (1 + 2 Re(a) A) v for a
+ * real a (GPU) */
+void apply_2nd_order_mu_K_to_v(double *x, double *v, double *w, double *t, double *kappa, double mu, unsigned *nnt,
+		unsigned ns, unsigned nn, double complex a)
+{
+	(void)w; (void)nn;
+	const double al = 2 * creal(a), aq = cimag(a) != 0 ? pow(cabs(a), 2) : 0;
+	pairs_on_vector(x, v, t, kappa, mu, nnt, ns, 1, &al, &aq);
+}
+
+/* organic_dense_matrix.c:186-194: the Taylor pairs [0, stop) or np-1 down to stop+1 on v (GPU) */
+void apply_pair_expansion_mu_K_to_v(double *x, double *v, double *w, double *t, double *kappa, double mu, unsigned *nnt,
+		unsigned ns, unsigned nn, int revert, int stop)
+{
+	(void)w; (void)nn;
+	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, 1, NULL);
+	up(e, e->d_x, x, 3 * (size_t)ns);
+	up(e, e->d_v, v, ns);
+	GPU(ephmc_apply_exp_cols(e->ctx, e->d_x, e->d_v, 1, revert, stop));
+	down(e, v, e->d_v, ns);
+}
+
+/* organic_dense_matrix.c:197-207 (mu and K expanded together, as the reference's active branch) */
+void apply_exp_mu_K_to_v(double *x, double *v, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns,
+		unsigned nn, int revert, int stop)
+{
+	apply_pair_expansion_mu_K_to_v(x, v, w, t, kappa, mu, nnt, ns, nn, revert, stop);
+}
+
+/* organic_dense_matrix.c:383-442 on the slice time of the field x (GPU) */
+void refine_QM1P(double *x, double *QM1P, double *w, double *t, double *kappa, double mu, unsigned *nnt, unsigned ns,
+		unsigned nn, int time)
+{
+	(void)w; (void)nn;
+	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, 1, NULL);
+	up(e, e->d_x, x + (size_t)time * 3 * ns, 3 * (size_t)ns);
+	up(e, e->d_m, QM1P, e->mat);
+	GPU(ephmc_refine_QM1P(e->ctx, e->d_x, e->d_m));
+	down(e, QM1P, e->d_m, e->mat);
+}
+
+/* organic_dense_matrix.c:444-449: the stencil in place of the polynomial's derivative */
+void refine_QM1P_approx(double *x, double *QM1P, double *w, double *t, double *kappa, double mu, unsigned *nnt,
+		unsigned ns, unsigned nn, int time)
+{
+	apply_mu_K_to_M(x + (size_t)time * 3 * ns, QM1P, w, t, kappa, mu, nnt, ns, nn);
+}
+
+/* organic_dense_matrix.c:267-309 (non-Schur branch; M_hat holds the inverse) (GPU) */
+void construct_QM1P(double *x, double *M_hat, double *QM1P, double *w, double *t, double *kappa, double mu, unsigned *nnt,
+		unsigned ns, unsigned nn, unsigned nt, int time, organic_flags *mode)
+{
+	(void)w; (void)nn;
+	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, nt, mode);
+	up(e, e->d_x, x, e->dim);
+	const int sp = mode && mode->single_particle;
+	if(!sp) up(e, e->d_m, M_hat, e->mat);
+	GPU(ephmc_construct_QM1P(e->ctx, e->d_x, sp ? NULL : e->d_m, e->d_m2, time));
+	down(e, QM1P, e->d_m2, e->mat);
+}
+
+/* organic_dense_matrix.c:311-381: QM1P holds nt matrices; perm receives the counts of slices
+ * taken from the left (perm[tau] = nt-1-tau) and from the right (perm[nt+tau] = tau) (GPU) */
+void construct_all_QM1P(double *x, double *M_hat, double *QM1P, double *w, int *perm, double *t, double *kappa, double mu,
+		unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, organic_flags *mode)
+{
+	(void)w; (void)nn;
+	struct organic_engine *e = engine_model(t, kappa, mu, nnt, ns, nt, mode);
+	if(!e->d_q) GPU(ephmc_malloc((size_t)nt * e->mat * sizeof(double), (void **)&e->d_q));
+	up(e, e->d_x, x, e->dim);
+	const int sp = mode && mode->single_particle;
+	if(!sp) up(e, e->d_m, M_hat, e->mat);
+	GPU(ephmc_construct_all_QM1P(e->ctx, e->d_x, sp ? NULL : e->d_m, e->d_q));
+	down(e, QM1P, e->d_q, (size_t)nt * e->mat);
+	if(perm)
+		for(unsigned tau = 0; tau < nt; tau++){
+			perm[tau] = (int)(nt - 1 - tau);
+			perm[nt + tau] = (int)tau;
+		}
+}
+
+/* organic_dense_matrix.c:526-545: equal-site phonon correlator <x_a(0) x_b(tau)> (host) */
+void phonon_correlator(double *x, unsigned ns, unsigned nn, unsigned nt, double *greens)
+{
+	const unsigned nn2 = nn / 2;
+	for(unsigned tau = 0; tau < nt; tau++)
+		for(unsigned a = 0; a < nn2; a++)
+			for(unsigned b = 0; b < nn2; b++){
+				double g = 0;
+				for(unsigned i = 0; i < ns; i++) g += x[(size_t)i * nn2 + a] * x[(size_t)tau * ns * nn2 + (size_t)i * nn2 + b];
+				greens[(size_t)tau * nn2 * nn2 + a * nn2 + b] = g / ns;
+			}
+}
+
+/* organic_hmc.c:20-54: relative error of p_dot against central differences of -flavors log det
+ * (step DBL_EPSILON^(1/3)), over every component of x; each difference builds M_hat and its log
+ * det on the GPU.  M_hat and w are scratch as in the reference. */
+double check_force(double *x, double *p, double *p_dot, double *M_hat, double *w, int *perm, double *t, double *kappa,
+		double mu, double omega, unsigned flavors, unsigned *nnt, unsigned ns, unsigned nn, unsigned nt, organic_flags *mode)
+{
+	(void)p; (void)omega;
+	const double delta = cbrt(2.220446049250313e-16);
+	const size_t dim = (size_t)nt * ns * nn / 2;
+	double err = 0, norm = 0;
+	for(size_t i = 0; i < dim; i++){
+		const double keep = x[i];
+		x[i] = keep + delta;
+		construct_M_hat(x, M_hat, w, t, kappa, mu, nnt, ns, nn, nt, mode);
+		const double lp = logDetM_and_inv_M_hat(M_hat, w, perm, ns, flavors, mode);
+		x[i] = keep - delta;
+		construct_M_hat(x, M_hat, w, t, kappa, mu, nnt, ns, nn, nt, mode);
+		const double lm = logDetM_and_inv_M_hat(M_hat, w, perm, ns, flavors, mode);
+		x[i] = keep;
+		/* force = -dE/dx with E = -flavors log det */
+		const double fd = flavors * (lp - lm) / (2 * delta);
+		err += (p_dot[i] - fd) * (p_dot[i] - fd);
+		norm += (fabs(p_dot[i]) + fabs(fd)) * (fabs(p_dot[i]) + fabs(fd));
+	}
+	return sqrt(err / norm);
+}
+
+/* organic_bosons.c:16-34: a field constant in time, each pair of link components drawn by one
+ * Box-Muller step with the thermal spread sqrt(1/2 / tanh(nt omega / 2)) (host random stream) */
+void hot_start(double *x, double omega, unsigned ns, unsigned nn, unsigned nt)
+{
+	const unsigned loc = ns * nn / 2;
+	const double sd = sqrt(0.5 / tanh(0.5 * nt * omega));
+	for(unsigned k = 0; k < loc; k += 2){
+		const double r = sd * sqrt(-2 * log(genrand64_real2()));
+		const double phi = 6.283185307179586 * genrand64_real2();
+		for(unsigned tau = 0; tau < nt; tau++){
+			x[(size_t)tau * loc + k] = r * cos(phi);
+			x[(size_t)tau * loc + (k + 1) % loc] = r * sin(phi);
+		}
+	}
+}
+
+/* organic_bosons.c:152-177 on the plan-bound field x and momenta x + dim (GPU) */
+void evolve_bosons_approx_step(double complex *xc, double omega, unsigned ns, unsigned nn, unsigned nt,
+		const organic_plan *fft, double h, organic_flags *mode)
+{
+	(void)xc; (void)omega; (void)ns; (void)nn; (void)nt; (void)mode;
+	struct organic_engine *e = fft[0];
+	up(e, e->d_x, e->x_host, e->dim);
+	up(e, e->d_p, e->x_host + e->dim, e->dim);
+	GPU(ephmc_evolve_bosons_approx_step(e->ctx, e->d_x, e->d_p, h));
+	down(e, e->x_host, e->d_x, e->dim);
+	e->chain_synced = 0;
+}
+
+/* organic_bosons.c:179-194 (GPU) */
+void boson_force(double *x, double *p_dot, double omega, unsigned ns, unsigned nn, unsigned nt, int overwrite)
+{
+	(void)nn;
+	struct organic_engine *e = NULL;
+	for(int i = 0; i < MAX_ENGINES && !e; i++)
+		if(engines[i] && engines[i]->ns == ns && engines[i]->nt == nt && same(engines[i]->omega, omega)) e = engines[i];
+	if(!e){
+		const double z[3] = {0, 0, 0};
+		e = engine_get(ns, 1, nt, z, z, 0, omega, 1, 1, 0, NULL);
+	}
+	up(e, e->d_x, x, e->dim);
+	if(!overwrite) up(e, e->d_p, p_dot, e->dim);
+	GPU(ephmc_boson_force(e->ctx, e->d_x, e->d_p, overwrite));
+	down(e, p_dot, e->d_p, e->dim);
+}
+
+/* organic_trans_loc.c:25-83: one disorder sample drawn into x (3N normals, host random stream),
+ * diagonalised and binned on the GPU; greens accumulates, results[0] = sum exp(-E_i) / N.  Pairs
+ * whose bin index is nt or more are dropped (the reference writes them past the end of greens). */
+void hist_trans_loc(double *x, double *t, double *kappa, double mu, double *contr_mat, unsigned *nnt, unsigned ns,
+		unsigned nn, unsigned nt, double width, double *results, double *greens, FILE *res_out)
+{
+	unsigned l1, l2;
+	lattice_extents(nnt, ns, &l1, &l2);
+	const unsigned dim = ns * nn / 2;
+	random_vector(x, dim);
+	struct organic_engine *e = engine_get(l1, l2, 1, t, kappa, mu, 1, 1, 1, 0, NULL);
+	GPU(ephmc_tl_hist(e->ctx, x, 1, contr_mat, width, nt, greens, results));
+	fprint_results(res_out, results, 1, 1);
 }

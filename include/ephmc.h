@@ -98,8 +98,15 @@ int ephmc_construct_M_hat(ephmc_ctx *ctx, const double *x_dev, double *M_hat_dev
 /* ---- (c) log-det, inverse, force ------------------------------------------------------ */
 /* log|det M_hat| and M_hat <- M_hat^{-1} in place (FP64, partial pivoting); replaces
  * logDetM_and_inv_M_hat (organic_dense_matrix.c:547-607).  SINGLE_PARTICLE: log tr M_hat,
- * M_hat unchanged.  logdet_host / sign_host may be NULL; when given, the call waits. */
+ * M_hat unchanged.  logdet_host / sign_host may be NULL; when given, the call waits and
+ * reports a singular matrix (a zero pivot) as EPHMC_ERR_NUMERIC with *logdet_host = -inf
+ * (the reference's log|0|, :582-584) and M_hat filled with NaN. */
 int ephmc_logdet_inv(ephmc_ctx *ctx, double *M_hat_dev, double *logdet_host, int *sign_host);
+/* Row interchanges of the last ephmc_logdet_inv in LAPACK's 1-based ipiv form (row i was
+ * exchanged with row ipiv[i] at step i): the identity when no exchange was needed, else the
+ * pivoted factorisation's; the p argument of logDetM_and_inv_M_hat (dgetrf's ipiv,
+ * organic_dense_matrix.c:580).  ipiv_host: N ints.  Waits. */
+int ephmc_last_pivots(ephmc_ctx *ctx, int *ipiv_host);
 /* p_dot = fermion force for field x given M_inv = (1 + B..B)^{-1}; replaces force
  * (organic_hmc.c:86-134) with construct_all_QM1P + refine_QM1P
  * (organic_dense_matrix.c:311-442) computed as an adjoint sweep.  Uses the context's
@@ -107,6 +114,34 @@ int ephmc_logdet_inv(ephmc_ctx *ctx, double *M_hat_dev, double *logdet_host, int
  * :267-309) on every slice; SINGLE_PARTICLE ignores M_inv (may be NULL) and scales by
  * exp(-logdet); NO_FOURIER_ACC adds the boson force (organic_bosons.c:179-194). */
 int ephmc_force(ephmc_ctx *ctx, const double *x_dev, const double *M_inv_dev, double logdet, double *p_dot_dev);
+
+/* ---- retained dense-matrix entry points (organic_dense_matrix.h:14-38) --------------------
+ * FP64, on the context's lattice; x_tau_dev: the 3N couplings of one slice; a "column" is one
+ * contiguous chunk of N values (the reference's LAPACK column). */
+/* out = A_tau in for ncols columns, A_tau = mu + K(x_tau) (out of place); replaces
+ * apply_mu_K_to_v (organic_dense_matrix.c:96-120; ncols = 1) and apply_mu_K_to_M (:123-131;
+ * ncols = N, with the result copied back) */
+int ephmc_apply_mu_K(ephmc_ctx *ctx, const double *x_tau_dev, const double *in_dev, double *out_dev, unsigned ncols);
+/* M <- prod_j (1 + a_lin[j] A_tau + a_quad[j] A_tau^2) M for ncols <= N columns, j = 0..npairs-1
+ * in order (a_quad[j] = 0: the linear factor); a_lin / a_quad are host arrays.  Replaces
+ * apply_2nd_order_mu_K_to_v (:168-183; one pair, a_lin = 2 Re a, a_quad = |a|^2 or 0 for real a)
+ * and apply_K_plus_1_v (:160-165; a_lin = 1) */
+int ephmc_apply_pairs(ephmc_ctx *ctx, const double *x_tau_dev, double *M_dev, unsigned ncols, int npairs,
+		const double *a_lin_host, const double *a_quad_host);
+/* M <- partial pair product of the context's Taylor pairs for ncols <= N columns: positions
+ * [0, stop) (revert = 0) or np-1 down to stop+1 (revert != 0); replaces
+ * apply_pair_expansion_mu_K_to_v / apply_exp_mu_K_to_v (:186-207) */
+int ephmc_apply_exp_cols(ephmc_ctx *ctx, const double *x_tau_dev, double *M_dev, unsigned ncols, int revert, int stop);
+/* Q <- sum_j a_lin_j Z_j + a_quad_j (A Z_j + Z_j A), Z_j = P_<j Q P_>j: the derivative of the
+ * pair product of one slice (tr[Q dB] = tr[Y dA]); replaces refine_QM1P (:383-442) */
+int ephmc_refine_QM1P(ephmc_ctx *ctx, const double *x_tau_dev, double *Q_dev);
+/* Q = refine(B_{time+1} ... B_{nt-1} M_inv B_0 ... B_{time-1}) (identity for M_inv under
+ * SINGLE_PARTICLE, M_inv may then be NULL); replaces construct_QM1P (:267-309, the non-Schur
+ * branch) */
+int ephmc_construct_QM1P(ephmc_ctx *ctx, const double *x_dev, const double *M_inv_dev, double *Q_dev, int time);
+/* Q[tau] = B_{tau+1} ... B_{nt-1} M_inv B_0 ... B_{tau-1} for every tau (Q: nt N^2 doubles), by
+ * recursive halving in nt log2(nt) slice applications; replaces construct_all_QM1P (:311-381) */
+int ephmc_construct_all_QM1P(ephmc_ctx *ctx, const double *x_dev, const double *M_inv_dev, double *Q_dev);
 
 /* ---- (d) Fourier-accelerated bosons ---------------------------------------------------- */
 /* x = free-boson sample coloured from dim normals; replaces sample_free_bosons
@@ -125,6 +160,13 @@ int ephmc_evolve_bosons(ephmc_ctx *ctx, double *x_dev, double *p_dev, double h);
  * for time |half| h, x += |half| h p (NO_FOURIER_ACC) or the CLASSICAL_FOURIER_ACC sub-steps
  * with the boson force; replaces update_x (organic_hmc.c:136-165) */
 int ephmc_update_x(ephmc_ctx *ctx, double *x_dev, double *p_dev, double h, double half);
+/* p_dot (+)= -dS_B/dx = -omega x + (x_{tau+1} - 2 x_tau + x_{tau-1}) / omega, overwriting
+ * p_dot when overwrite != 0; replaces boson_force (organic_bosons.c:179-194) */
+int ephmc_boson_force(ephmc_ctx *ctx, const double *x_dev, double *p_dot_dev, int overwrite);
+/* x += h M^{-1} p with the Fourier-space mass of the context's kinetic term (omega ->
+ * sqrt(cfa_mass^2 + omega^2) under CLASSICAL_FOURIER_ACC); replaces evolve_bosons_approx_step
+ * (organic_bosons.c:152-177) */
+int ephmc_evolve_bosons_approx_step(ephmc_ctx *ctx, double *x_dev, const double *p_dev, double h);
 /* H = -flavors logdet + energyP + S_B(x); replaces hamilton (organic_hmc.c:67-84).  Waits. */
 int ephmc_hamilton(ephmc_ctx *ctx, const double *x_dev, double logdet, double energyP, double *H_host);
 /* position-Verlet leapfrog of Nmd steps on x, p (the context's precision); replaces

@@ -35,3 +35,29 @@ def test_python_binding_imports():
     t, kappa, mu, omega = ephmc.scale_params(40.0, (0.10, -0.03, 0.02), (0.3, 0.3, 0.3), 0.0, 0.006, 128)
     assert omega == pytest.approx(0.006 * 40 / 128)
     assert mu == pytest.approx(-2 * 0.15 * 40 / 128)
+
+
+# every function the reference's headers declare on the replaced path (organic_dense_matrix.h,
+# organic_hmc.h, organic_bosons.h, organic_trans_loc.h; names only)
+REFERENCE_HOST_API = [
+    "mat_mul", "mat_mul_tr", "mat_mat_mul", "construct_id", "transpose", "transpose_rect", "trace",
+    "apply_mu_K_to_v", "apply_mu_K_to_M", "set_taylor_pair_order", "apply_K_plus_1_v",
+    "apply_2nd_order_mu_K_to_v", "apply_pair_expansion_mu_K_to_v", "apply_exp_mu_K_to_v", "apply_exp_mu_K_to_M",
+    "construct_hamilton", "apply_current_to_v", "construct_M_hat", "construct_QM1P", "construct_all_QM1P",
+    "refine_QM1P", "refine_QM1P_approx", "cc_correlator", "phonon_correlator", "logDetM_and_inv_M_hat",
+    "check_force", "energy_momenta", "hamilton", "force", "update_x", "leap_frog", "trajectory", "run_hmc",
+    "hot_start", "higher_Matsubaras_norm", "sample_free_bosons", "sample_fourier_momenta",
+    "energy_fourier_momenta", "evolve_bosons", "evolve_bosons_approx_step", "boson_force",
+    "hist_trans_loc", "run_trans_loc",
+]
+
+
+def test_host_library_exports_the_reference_interface():
+    lib = C.CDLL(os.path.join(PKG, "libephmc_host.so"))
+    missing = [n for n in REFERENCE_HOST_API if not hasattr(lib, n)]
+    assert not missing, missing
+    hdr = open(os.path.join(PKG, "host", "organic.h")).read()
+    declared_host = re.findall(r"^\s*(?:unsigned\s*\*|double\s*\*|void|double|short|unsigned|int|const char \*|organic_plan)\s+(\w+)\s*\(", hdr, re.M)
+    missing = [n for n in declared_host if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(REFERENCE_HOST_API) <= set(declared_host)
