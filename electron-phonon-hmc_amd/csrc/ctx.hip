@@ -565,6 +565,12 @@ int ephmc_force(ephmc_ctx *c, const double *x, const double *M_inv, double logde
 	return EPHMC_OK;
 }
 
+int ephmc_md_force(ephmc_ctx *c, const double *x, double *p_dot)
+{
+	if(!c || !x || !p_dot){ set_error("null argument"); return EPHMC_ERR_ARG; }
+	return md_force(c, x, p_dot);
+}
+
 int ephmc_sample_free_bosons(ephmc_ctx *c, const double *normals, double *x)
 {
 	if(!c || !normals || !x){ set_error("null argument"); return EPHMC_ERR_ARG; }

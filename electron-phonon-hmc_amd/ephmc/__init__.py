@@ -86,6 +86,7 @@ def _load():
         "ephmc_timing_get": [vp, C.c_char_p, dp, C.POINTER(C.c_ulong)],
         "ephmc_autotune": [vp, C.POINTER(TuneResult)],
         "ephmc_last_pivots": [vp, C.POINTER(C.c_int)],
+        "ephmc_md_force": [vp, vp, vp],
         "ephmc_apply_mu_K": [vp, vp, vp, vp, C.c_uint],
         "ephmc_apply_pairs": [vp, vp, vp, C.c_uint, C.c_int, dp, dp],
         "ephmc_apply_exp_cols": [vp, vp, vp, C.c_uint, C.c_int, C.c_int],
@@ -222,6 +223,13 @@ class Context:
         dx, dp_ = self.array(self.dim, x), self.array(self.dim)
         dm = self.array(self.mat, Minv) if Minv is not None else None
         check(LIB.ephmc_force(self.h, dx.ptr, dm.ptr if dm is not None else None, logdet, dp_.ptr))
+        return dp_.download()
+
+    def md_force(self, x):
+        """The force of one molecular-dynamics step at x (the context's chain precision builds
+        M_hat; FP64 inverse; adjoint sweep), as ephmc_leap_frog uses it."""
+        dx, dp_ = self.array(self.dim, x), self.array(self.dim)
+        check(LIB.ephmc_md_force(self.h, dx.ptr, dp_.ptr))
         return dp_.download()
 
     def apply_exp_M(self, x_tau, M, revert, stop):

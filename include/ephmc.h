@@ -143,6 +143,12 @@ int ephmc_construct_QM1P(ephmc_ctx *ctx, const double *x_dev, const double *M_in
  * recursive halving in nt log2(nt) slice applications; replaces construct_all_QM1P (:311-381) */
 int ephmc_construct_all_QM1P(ephmc_ctx *ctx, const double *x_dev, const double *M_inv_dev, double *Q_dev);
 
+/* One molecular-dynamics force at field x exactly as the leapfrog computes it: the forward chain
+ * in the context's precision (FP32 under EPHMC_MIXED) gives M_hat, the FP64 Gauss-Jordan its
+ * inverse and log det, the adjoint sweep the force; replaces the construct_M_hat +
+ * logDetM_and_inv_M_hat + force sequence of leap_frog (organic_hmc.c:178-183). */
+int ephmc_md_force(ephmc_ctx *ctx, const double *x_dev, double *p_dot_dev);
+
 /* ---- (d) Fourier-accelerated bosons ---------------------------------------------------- */
 /* x = free-boson sample coloured from dim normals; replaces sample_free_bosons
  * (organic_bosons.c:55-74) after its random_vector draw. */
