@@ -52,13 +52,13 @@ def chain_flops(ns, nt):
     return 60 * ns * ns * nt
 
 
-def run_chain(E, precision, normals_list, uniforms, x0_normals, warmup, steps, graph=False):
+def run_chain(E, precision, normals_list, uniforms, x0_normals, warmup, steps, graph=False, checkpoint=K_CKPT):
     """Chain of `steps` timed trajectories after `warmup`.  Eager launches record the kernel
     groups with HIP events on the context stream; graph mode (the trajectory body replayed as one
     captured hipGraph) records no per-kernel events, only the wall time."""
     t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, NT)
     ctx = E.Context(L, L, NT, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=precision,
-                    checkpoint=K_CKPT, graph=int(graph))
+                    checkpoint=checkpoint, graph=int(graph))
     if AUTOTUNE:
         log("autotune", precision, ctx.autotune())
     ctx.chain_init(x0_normals)
@@ -83,16 +83,22 @@ def run_chain(E, precision, normals_list, uniforms, x0_normals, warmup, steps, g
     return ctx, el, groups, acc
 
 
-def accuracy(E, x_host, normals, uniform, nobs=40):
-    """Mixed vs FP64 on the same field and the same trajectory inputs:
-    force_rel_err       |F32 - F64| / |F64| at the chain's field, same M_hat^{-1};
-    dH_abs_err          Metropolis dH of the mixed path against the FP64 path evaluated at the
-                        same leapfrog endpoint (the mixed chain evaluates H in FP64);
-    dH_md_drift         dH of a trajectory integrated with FP32 forces minus the same trajectory
-                        with FP64 forces (the force error carried along the molecular dynamics);
-    obs_*               means and standard errors of log det / N and the boson action per link over
-                        nobs trajectories of each chain, started from the same field with the same
-                        random inputs."""
+def accuracy(E, x_host, ndraw=20, nobs=100, nsame=40):
+    """Mixed precision against FP64 at 32x32, Nt=128 (every number compares two different
+    computations; nothing is evaluated twice the same way):
+    force_rel_err        |F_mixed - F_64| / |F_64| of the molecular-dynamics force at the chain's
+                         field x, each built as the leapfrog builds it (mixed: FP32 chain -> M_hat ->
+                         FP64 Gauss-Jordan -> FP32 adjoint sweep; FP64: all FP64);
+    force_rel_err_sweep  the FP32 sweep alone: FP32 chain + sweep fed the FP64 M_hat^{-1};
+    dH_drift_*           |dH_mixed - dH_64| of whole trajectories integrated from the same (x, p)
+                         with FP32 and with FP64 forces (both dH evaluated with the FP64 Metropolis
+                         H), over ndraw momentum draws: mean, max, and the mean |dH| for scale;
+    obs_*                two independent chains (different random inputs, nobs trajectories each,
+                         from the same start) per precision: means, binned standard errors (bins of
+                         10) of log det / N, the boson action per link and the acceptance, and the
+                         z-score of each difference;
+    same_inputs_*        the two precisions with identical random inputs over nsame trajectories:
+                         fraction of identical accept decisions and the largest log-det difference."""
     t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, NT)
     out = {}
     ctxs = {}
@@ -103,43 +109,61 @@ def accuracy(E, x_host, normals, uniform, nobs=40):
     c64, c32 = ctxs["fp64"], ctxs["mixed"]
     minv = c64.chain_minv()
     ld = c64.chain_logdet()
-    F64 = c64.force(x_host, minv, ld)
-    F32 = c32.force(x_host, minv, ld)
+    F64 = c64.md_force(x_host)
+    F32 = c32.md_force(x_host)
     out["force_rel_err"] = float(np.linalg.norm(F32 - F64) / np.linalg.norm(F64))
-    out["logdet_rel_err"] = abs(c32.chain_logdet() - ld) / abs(ld)
+    out["force_rel_err_sweep"] = float(np.linalg.norm(c32.force(x_host, minv, ld) - F64) / np.linalg.norm(F64))
+    out["_F64"], out["_F32"], out["_logdet"] = F64, F32, ld
 
-    def H(c, x, p):
-        lx = c.logdet_inv(c.M_hat(x))[0]
-        return c.hamilton(x, lx, c.energy_momenta(p))
+    def H(x, p):
+        return c64.hamilton(x, c64.logdet_inv(c64.M_hat(x))[0], c64.energy_momenta(p))
 
-    p0 = c64.sample_fourier_momenta(normals)
-    x1, p1 = c32.leap_frog(x_host, p0)             # mixed molecular dynamics
-    dH_mixed = H(c32, x1, p1) - H(c32, x_host, p0)
-    dH_fp64_same = H(c64, x1, p1) - H(c64, x_host, p0)
-    x1d, p1d = c64.leap_frog(x_host, p0)           # FP64 molecular dynamics
-    dH_fp64 = H(c64, x1d, p1d) - H(c64, x_host, p0)
-    out["dH_mixed"] = dH_mixed
-    out["dH_fp64"] = dH_fp64
-    out["dH_abs_err"] = abs(dH_mixed - dH_fp64_same)
-    out["dH_md_drift"] = abs(dH_mixed - dH_fp64)
-    r64 = c64.trajectory(normals, uniform)
-    r32 = c32.trajectory(normals, uniform)
-    out["same_decision"] = bool(r64.accepted == r32.accepted)
-    # short chains from the same state with the same random inputs
-    rng = np.random.default_rng(SEED + 1)
-    draws = [(rng.standard_normal(len(x_host)), float(rng.random())) for _ in range(nobs)]
-    for name, c in ctxs.items():
+    rng = np.random.default_rng(SEED + 2)
+    drift, dhs = [], []
+    for _ in range(ndraw):
+        p0 = c64.sample_fourier_momenta(rng.standard_normal(len(x_host)))
+        H0 = H(x_host, p0)
+        x1, p1 = c32.leap_frog(x_host, p0)
+        x1d, p1d = c64.leap_frog(x_host, p0)
+        d32, d64 = H(x1, p1) - H0, H(x1d, p1d) - H0
+        drift.append(abs(d32 - d64))
+        dhs.append(abs(d64))
+    out["dH_drift_mean"] = float(np.mean(drift))
+    out["dH_drift_max"] = float(np.max(drift))
+    out["dH_abs_mean"] = float(np.mean(dhs))
+    out["dH_drift_trajectories"] = ndraw
+
+    def chain(c, seed, n):
         c.chain_set_x(x_host)
-        lds, sbs, acc = [], [], 0
-        for nrm, u in draws:
-            rec = c.trajectory(nrm, u)
-            acc += rec.accepted
+        r = np.random.default_rng(seed)
+        lds, sbs, acc = [], [], []
+        for _ in range(n):
+            rec = c.trajectory(r.standard_normal(len(x_host)), float(r.random()))
+            acc.append(rec.accepted)
             lds.append(rec.logdet / (L * L))
             sbs.append(c.hamilton(c.chain_x(), 0.0, 0.0) / len(x_host))
-        for key, v in (("logdet_per_site", lds), ("boson_action_per_link", sbs)):
-            v = np.asarray(v)
-            out["obs_%s_%s" % (key, name)] = [float(v.mean()), float(v.std(ddof=1) / np.sqrt(len(v)))]
-        out["obs_acceptance_%s" % name] = acc / nobs
+        return np.array(lds), np.array(sbs), np.array(acc, float)
+
+    def binned(v, b=10):
+        m = len(v) // b
+        blocks = v[:m * b].reshape(m, b).mean(1)
+        return float(v.mean()), float(blocks.std(ddof=1) / np.sqrt(m))
+
+    series = {"fp64": chain(c64, SEED + 10, nobs), "mixed": chain(c32, SEED + 20, nobs)}
+    for k, key in enumerate(("logdet_per_site", "boson_action_per_link", "acceptance")):
+        a, b = binned(series["fp64"][k]), binned(series["mixed"][k])
+        out["obs_" + key] = {"fp64": a, "mixed": b,
+                             "z": abs(a[0] - b[0]) / max(math.hypot(a[1], b[1]), 1e-300)}
+    out["obs_trajectories"] = nobs
+    r64, r32 = [], []
+    for c, recs in ((c64, r64), (c32, r32)):
+        c.chain_set_x(x_host)
+        r = np.random.default_rng(SEED + 30)
+        for _ in range(nsame):
+            rec = c.trajectory(r.standard_normal(len(x_host)), float(r.random()))
+            recs.append((rec.accepted, rec.logdet))
+    out["same_inputs_decisions_equal"] = sum(a[0] == b[0] for a, b in zip(r64, r32)) / nsame
+    out["same_inputs_max_logdet_diff"] = float(max(abs(a[1] - b[1]) for a, b in zip(r64, r32)))
     for c in ctxs.values():
         c.close()
     return out
@@ -159,28 +183,99 @@ def force_only(n):
     ctx.close()
 
 
-def cpu_baseline(sample_threads):
-    """The oracle (CPU restatement of the reference algorithm, C/OpenMP) timed on this host:
-    one M_hat + LU log-det/inverse and one force at the full size, extrapolated to one
-    trajectory as (Nmd+1) (M_hat + LU) + Nmd force (boson FFT work is below 1%)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def _omp_threads(n):
+    """Threads of the oracle's OpenMP regions (the runtime the oracle library was linked with)."""
+    import ctypes as C
+    try:
+        C.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+    except OSError:
+        os.environ["OMP_NUM_THREADS"] = str(n)
+
+
+def cpu_baseline(threads, x_state, gpu):
+    """The oracle (the C/OpenMP restatement of the reference algorithm) timed on this host's cores,
+    and used as the checker of the GPU at the full size.
+
+    Timed sample (32x32, Nt=128, `threads` OpenMP threads): (i) one complete trajectory with Nmd=1
+    through the oracle's trajectory driver (momentum draw and colouring, leapfrog with one force, two
+    M_hat + LU, Metropolis), and (ii) one M_hat + LU and one force at the chain's field x; the time of
+    one trajectory at Nmd=10 is t(i) + 9 (t_M+LU + t_force).  The oracle's own LU and FFT run (no
+    LAPACK or FFTW).  Also at 1 thread: the 6x6, Nt=32 chain (Nmd=10), 100 trajectories in full.
+
+    Checks at x (the GPU values computed in the accuracy leg): the GPU's FP64 log det and its FP64
+    and mixed molecular-dynamics forces against the oracle's."""
     import ctypes as C
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    os.environ["OMP_NUM_THREADS"] = str(sample_threads)
-    from _oracle import OracleModel
+    from _oracle import HMC, OracleModel, oracle
+    lib = oracle()
+    _omp_threads(threads)
     om = OracleModel(L, L, NT)
-    x = np.random.default_rng(SEED).standard_normal(om.dim) * 0.5
+    h = HMC()
+    assert lib.orc_hmc_init(C.byref(h), C.byref(om.m), SEED, 1, 0.0, None) == 0
+    dH = C.c_double()
     t0 = time.perf_counter()
-    M = om.M_hat(x)
-    ld, _, minv = om.logdet_inv(M)
+    lib.orc_trajectory(C.byref(h), None, None, C.byref(dH), None, None)
+    t_traj1 = time.perf_counter() - t0
+    lib.orc_hmc_free(C.byref(h))
+    t0 = time.perf_counter()
+    ld, _, minv = om.logdet_inv(om.M_hat(x_state))
     t_mlu = time.perf_counter() - t0
     t0 = time.perf_counter()
-    om.force(x, minv, ld)
+    F = om.force(x_state, minv, ld)
     t_force = time.perf_counter() - t0
-    t_traj = (NMD + 1) * t_mlu + NMD * t_force
-    return {"value": 1.0 / t_traj, "unit": "traj/s", "cores": sample_threads, "kind": "port",
-            "sample": "one M_hat+LU (%.2f s) and one force (%.2f s) at 32x32, Nt=128 on %d OpenMP threads, "
-                      "extrapolated to (Nmd+1)(M_hat+LU)+Nmd force = %.1f s per trajectory; CPU %s"
-                      % (t_mlu, t_force, sample_threads, t_traj, platform.processor() or platform.machine())}
+    t_traj = t_traj1 + (NMD - 1) * (t_mlu + t_force)
+    checks = {"logdet_rel_err_fp64_vs_oracle": abs(gpu["logdet"] - ld) / abs(ld),
+              "force_rel_err_fp64_vs_oracle": float(np.linalg.norm(gpu["F64"] - F) / np.linalg.norm(F)),
+              "force_rel_err_mixed_vs_oracle": float(np.linalg.norm(gpu["F32"] - F) / np.linalg.norm(F))}
+    # 6x6, Nt=32 on one thread, in full
+    _omp_threads(1)
+    om1 = OracleModel(6, 6, 32)
+    h = HMC()
+    assert lib.orc_hmc_init(C.byref(h), C.byref(om1.m), 1, NMD, 0.0, None) == 0
+    n1 = 100
+    t0 = time.perf_counter()
+    for _ in range(n1):
+        lib.orc_trajectory(C.byref(h), None, None, C.byref(dH), None, None)
+    t_c1 = (time.perf_counter() - t0) / n1
+    lib.orc_hmc_free(C.byref(h))
+    _omp_threads(threads)
+    base = {"value": 1.0 / t_traj, "unit": "traj/s", "cores": threads, "kind": "port",
+            "sample": "32x32, Nt=128 on %d OpenMP threads: one complete Nmd=1 trajectory (%.2f s) plus one "
+                      "M_hat+LU (%.2f s) and one force (%.2f s); Nmd=10 trajectory = %.1f s.  Oracle's own LU "
+                      "and FFT.  CPU %s (%s logical CPUs visible)"
+                      % (threads, t_traj1, t_mlu, t_force, t_traj, cpu_model(), os.cpu_count())}
+    c1 = {"traj_per_s": 1.0 / t_c1, "threads": 1, "trajectories": n1,
+          "config": "6x6, Nt=32, Nmd=10, seed 1, FP64 (oracle, complete trajectories)"}
+    return base, c1, checks
+
+
+def c1_gpu(E, ntraj=100):
+    """The 6x6, Nt=32 FP64 chain (Nmd=10) on the GPU, seed-1 normals from numpy, graph replay."""
+    t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, 32)
+    ctx = E.Context(6, 6, 32, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=E.FP64, graph=1)
+    rng = np.random.default_rng(1)
+    draws = [(rng.standard_normal(ctx.dim), float(rng.random())) for _ in range(ntraj + 1)]
+    ctx.chain_init(rng.standard_normal(ctx.dim))
+    ctx.trajectory(*draws[0], wait=False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for nrm, u in draws[1:]:
+        ctx.trajectory(nrm, u, wait=False)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    ctx.close()
+    return {"traj_per_s": ntraj / el, "trajectories": ntraj, "config": "6x6, Nt=32, Nmd=10, FP64, graph replay"}
 
 
 def copy_calibration():
@@ -294,6 +389,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tl", action="store_true", help="skip the transient-localisation leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-chain leg")
+    ap.add_argument("--no-k16", action="store_true", help="skip the chains with partial products every 16 slices")
     ap.add_argument("--batch-traj", type=int, default=3)
     ap.add_argument("--tl-samples", type=int, default=1024)
     ap.add_argument("--no-graph", action="store_true",
@@ -346,10 +442,21 @@ def main():
             ctx, el, groups, rec = run_chain(E, prec, normals_list, uniforms, x0, args.warmup, args.steps, graph=True)
             res[name] = {"el": el, "groups": groups, "ctx": ctx}
             log(name, "%.2f ms/traj" % (1e3 * el / args.steps))
+    # the stated checkpoint interval of the force check (k = 16: partial products every 16 slices,
+    # the others recomputed inside the sweep), same chains
+    k16 = {}
+    if not args.no_k16:
+        for name, prec in (("fp64", E.FP64), ("mixed", E.MIXED)):
+            ctx, el, groups, rec = run_chain(E, prec, normals_list, uniforms, x0, args.warmup, args.steps,
+                                             graph=not args.no_graph, checkpoint=16)
+            k16[name] = args.steps / el
+            ctx.close()
+            log("k16", name, "%.2f ms/traj" % (1e3 * el / args.steps))
     x_state = res["fp64"]["ctx"].chain_x()
     for r in res.values():
         r["ctx"].close()
-    acc = accuracy(E, x_state, normals_list[0], float(uniforms[0]))
+    acc = accuracy(E, x_state)
+    gpu_vals = {"F64": acc.pop("_F64"), "F32": acc.pop("_F32"), "logdet": acc.pop("_logdet")}
     log("accuracy", acc)
     corr = correlator_leg(E, x_state)
     log("correlator", corr)
@@ -361,9 +468,13 @@ def main():
     per_launch_ms, adj_calls = res["mixed"]["groups"]["adjoint_kernel"]
     flops = stencil_flops_adjoint_per_launch(L * L, SPAN)
     achieved = flops / (per_launch_ms * 1e-3)
-    roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+    roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32, "traffic": None,
-            "kernel": "adjoint_kernel<float,4,1024,1,4> (FP32 VALU stencils; one launch = 32 slices)"}
+            "peak_unpacked": PEAK_FP64 / 1e12, "frac_unpacked": achieved / PEAK_FP64,
+            "kernel": "adjoint_kernel<float,4,1024,1,4> (FP32 vector-ALU stencils, no matrix cores; one launch = "
+                      "32 slices; the published 157.3 TFLOP/s FP32 peak assumes packed FP32, compiled out here, "
+                      "so 78.6 TFLOP/s is its unpacked ceiling)",
+            "flop_per_launch": flops, "avg_launch_ms": per_launch_ms, "launches": adj_calls}
     # HBM bytes per launch of the same kernel from the two counter passes (FETCH_SIZE, WRITE_SIZE of
     # `bench.py --force-only 3`, each scaled by the counter's reading for a 512 MiB copy)
     if os.path.exists(TRAFFIC_FILE):
@@ -381,14 +492,20 @@ def main():
                    "parallelism": "none (1 GPU)"},
         "fp64_traj_per_s": tps64, "fp64_ms_per_step": 1e3 * el64 / args.steps,
         "mixed_over_fp64": tpsmx / tps64,
-        "force_rel_err": acc["force_rel_err"], "dH_abs_err": acc["dH_abs_err"],
-        "dH_md_drift": acc["dH_md_drift"],
-        "logdet_rel_err_metropolis": acc["logdet_rel_err"],
+        "force_rel_err": acc["force_rel_err"], "force_rel_err_sweep": acc["force_rel_err_sweep"],
+        "dH_drift": {"mean": acc["dH_drift_mean"], "max": acc["dH_drift_max"], "mean_abs_dH": acc["dH_abs_mean"],
+                     "trajectories": acc["dH_drift_trajectories"], "gate": 1e-8,
+                     "met": acc["dH_drift_max"] <= 1e-8},
         "observables": {k[4:]: v for k, v in acc.items() if k.startswith("obs_")},
+        "same_inputs": {"decisions_equal": acc["same_inputs_decisions_equal"],
+                        "max_logdet_diff": acc["same_inputs_max_logdet_diff"]},
         "kernel_ms": {p: {g: round(v[0], 4) for g, v in res[p]["groups"].items()} for p in res if res[p]["groups"]},
         "cc_correlator": corr,
         "roofline": roof,
     }
+    if k16:
+        out["checkpoint16_traj_per_s"] = k16
+        out["checkpoint16_mixed_over_fp64"] = k16["mixed"] / k16["fp64"]
     if not args.no_graph:
         # headline: the trajectory body replayed as a captured hipGraph (the same kernels; the eager
         # chains above give the per-kernel HIP-event timings)
@@ -407,8 +524,13 @@ def main():
         out["tl"] = tl_leg(E, args.tl_samples)
         log("tl", out["tl"])
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(threads)
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or min(16, os.cpu_count() or 1))
+        out["c1_gpu"] = c1_gpu(E)
+        base, c1, checks = cpu_baseline(threads, x_state, gpu_vals)
+        out["cpu_baseline"] = base
+        out["cpu_c1_1thread"] = c1
+        out["full_size_checks_vs_oracle"] = checks
+        log("cpu", base, c1, checks)
     print(json.dumps(out), flush=True)
 
 

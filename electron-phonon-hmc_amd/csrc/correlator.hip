@@ -138,6 +138,35 @@ __global__ __launch_bounds__(576) void cc_sum_kernel(const double *__restrict__ 
 	if(lane == 0) out[tau * 9 + q] = s * scale;
 }
 
+// The correlator's kernels with the transfer-matrix stores Rt0, R0t (nt N^2 values each) and the
+// block sums `partial` (cc_partials values) given: g_ab(tau) (nt x 9) into out, asynchronous.
+static int correlate(ephmc_ctx *c, const double *x, const double *minv, double logdet, double *Rt0, double *R0t,
+		double *partial, double *out)
+{
+	const unsigned ns = c->ns, nt = c->nt;
+	EPHMC_TRY(coefficients(c, x, false));
+	// ZERO_FILLING / SINGLE_PARTICLE start both propagators from the identity
+	// (organic_dense_matrix.c:458-470)
+	const bool ident = c->cfg.zero_filling || c->cfg.single_particle;
+	// R_t0: slices nt-1 .. 0 applied to M_hat^{-1}, the state stored before every slice: step i
+	// (slice nt-1-i) stores slot nt-i, so slot s >= 1 is stored entry nt-s; slot 0 is the result
+	EPHMC_TRY(chain_forward(c, EPHMC_FP64, minv, c->d_Tfin, Rt0, 1, 1, ident));
+	// R_0t: transposed inverse, then slices 0 .. nt-1 ascending; stored entry tau = state at tau
+	if(!ident) EPHMC_TRY(transpose_plus_identity(c, EPHMC_FP64, minv, c->d_gj[0], 0));
+	EPHMC_TRY(chain_forward(c, EPHMC_FP64, c->d_gj[0], c->d_gj[1], R0t, 1, 0, ident));
+	const unsigned by = (ns + CC_J - 1) / CC_J;
+	const size_t lds = 8 * (size_t)ns * sizeof(double);
+	if(lds + 9 * (CC_BT / 64) * sizeof(double) > 160 * 1024){ set_error("correlator: %u sites exceed the LDS", ns); return EPHMC_ERR_UNSUPPORTED; }
+	(void)hipFuncSetAttribute((const void *)cc_contract_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	cc_contract_kernel<<<dim3(by, nt), CC_BT, lds, c->stream>>>(c->d_Tfin, Rt0, R0t, x, c->d_nb,
+		c->cfg.t[0], c->cfg.t[1], c->cfg.t[2], c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], ns, nt, partial);
+	EPHMC_LAUNCHED();
+	cc_sum_kernel<<<nt, 576, 0, c->stream>>>(partial, by, c->cfg.single_particle ? exp(-logdet) : 1.0 / ns,
+		out);
+	EPHMC_LAUNCHED();
+	return EPHMC_OK;
+}
+
 } // namespace ephmc
 
 using namespace ephmc;
@@ -157,27 +186,7 @@ extern "C" int ephmc_cc_correlator(ephmc_ctx *c, const double *x, const double *
 		if(rc != EPHMC_OK) return rc;
 	}
 	EPHMC_TRY(timer_begin(c, "correlator"));
-	EPHMC_TRY(coefficients(c, x, false));
-	// ZERO_FILLING / SINGLE_PARTICLE start both propagators from the identity
-	// (organic_dense_matrix.c:458-470)
-	const bool ident = c->cfg.zero_filling || c->cfg.single_particle;
-	// R_t0: slices nt-1 .. 0 applied to M_hat^{-1}, the state stored before every slice: step i
-	// (slice nt-1-i) stores slot nt-i, so slot s >= 1 is stored entry nt-s; slot 0 is the result
-	EPHMC_TRY(chain_forward(c, EPHMC_FP64, minv, c->d_Tfin, c->d_Rt0, 1, 1, ident));
-	// R_0t: transposed inverse, then slices 0 .. nt-1 ascending; stored entry tau = state at tau
-	if(!ident) EPHMC_TRY(transpose_plus_identity(c, EPHMC_FP64, minv, c->d_gj[0], 0));
-	EPHMC_TRY(chain_forward(c, EPHMC_FP64, c->d_gj[0], c->d_gj[1], c->d_R0t, 1, 0, ident));
-	const unsigned by = (ns + CC_J - 1) / CC_J;
-	const size_t lds = 8 * (size_t)ns * sizeof(double);
-	if(lds + 9 * (CC_BT / 64) * sizeof(double) > 160 * 1024){ set_error("correlator: %u sites exceed the LDS", ns); return EPHMC_ERR_UNSUPPORTED; }
-	double *partial = c->d_ccg + (size_t)nt * 9;     // cc_partials(ns, nt) block sums
-	(void)hipFuncSetAttribute((const void *)cc_contract_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-	cc_contract_kernel<<<dim3(by, nt), CC_BT, lds, c->stream>>>(c->d_Tfin, c->d_Rt0, c->d_R0t, x, c->d_nb,
-		c->cfg.t[0], c->cfg.t[1], c->cfg.t[2], c->cfg.kappa[0], c->cfg.kappa[1], c->cfg.kappa[2], ns, nt, partial);
-	EPHMC_LAUNCHED();
-	cc_sum_kernel<<<nt, 576, 0, c->stream>>>(partial, by, c->cfg.single_particle ? exp(-logdet) : 1.0 / ns,
-		c->d_ccg);
-	EPHMC_LAUNCHED();
+	EPHMC_TRY(correlate(c, x, minv, logdet, c->d_Rt0, c->d_R0t, c->d_ccg + (size_t)nt * 9, c->d_ccg));
 	EPHMC_TRY(timer_end(c, "correlator"));
 	EPHMC_HIP(hipMemcpyAsync(greens_host, c->d_ccg, nt * 9 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
 	EPHMC_HIP(hipStreamSynchronize(c->stream));
