@@ -181,10 +181,16 @@ static int md_force(ephmc_ctx *c, const double *x, double *pdot)
 	EPHMC_TRY(coefficients(c, x, prec != EPHMC_FP64));
 	void *tfin = prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32;
 	EPHMC_TRY(chain_forward(c, prec, nullptr, tfin, c->d_ck, c->ckpt, 0, true));
-	if(!sp) EPHMC_TRY(transpose_plus_identity(c, prec, tfin, c->d_mhat, 1));
+	if(!sp && !c->cfg.stabilize) EPHMC_TRY(transpose_plus_identity(c, prec, tfin, c->d_mhat, 1));
 	EPHMC_TRY(timer_end(c, "chain"));
 	const double *minv = c->d_mhat, *fscale = nullptr;
-	if(sp){
+	if(c->cfg.stabilize){
+		// M_hat^{-1} from the product stabilised every k slices (the slice precision of the molecular
+		// dynamics); the sweep below reads the checkpoints of the chain above
+		EPHMC_TRY(timer_begin(c, "gj"));
+		EPHMC_TRY(stable_logdet_inverse(c, prec, c->cfg.stabilize, c->d_mhat));
+		EPHMC_TRY(timer_end(c, "gj"));
+	}else if(sp){
 		EPHMC_TRY(log_trace(c, prec, tfin, SCAL_LOGDET, SCAL_FSCALE));
 		fscale = c->d_scal + SCAL_FSCALE;
 		minv = nullptr;
@@ -205,6 +211,10 @@ static int md_force(ephmc_ctx *c, const double *x, double *pdot)
 // scal[SCAL_LOGDET]; SINGLE_PARTICLE: d_mhat = B..B and log tr; ZERO_FILLING: d_mhat = B..B.
 static int exact_logdet(ephmc_ctx *c, const double *x)
 {
+	if(c->cfg.stabilize){
+		EPHMC_TRY(coefficients(c, x, false));
+		return stable_logdet_inverse(c, EPHMC_FP64, c->cfg.stabilize, c->d_mhat);
+	}
 	EPHMC_TRY(coefficients(c, x, false));
 	EPHMC_TRY(chain_forward(c, EPHMC_FP64, nullptr, c->d_mhat, nullptr, 0, 1, true));
 	if(c->cfg.zero_filling) return set_scalar(c, SCAL_LOGDET, 0.0);
@@ -318,6 +328,10 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 	if(cfg->precision != EPHMC_FP64 && cfg->precision != EPHMC_MIXED){ set_error("precision"); return EPHMC_ERR_ARG; }
 	if(cfg->classical_fourier_acc && cfg->cfa_steps == 0){ set_error("CLASSICAL_FOURIER_ACC needs cfa_steps >= 1"); return EPHMC_ERR_ARG; }
 	if(cfg->no_fourier_acc && cfg->classical_fourier_acc){ set_error("NO_FOURIER_ACC and CLASSICAL_FOURIER_ACC exclude each other"); return EPHMC_ERR_ARG; }
+	if(cfg->stabilize && (cfg->zero_filling || cfg->single_particle)){
+		set_error("stabilize: the stabilised product forms 1 + B..B, not provided for ZERO_FILLING / SINGLE_PARTICLE");
+		return EPHMC_ERR_UNSUPPORTED;
+	}
 	const unsigned ns = cfg->l1 * cfg->l2;
 	if(ns > 2048){ set_error("lattice of %u sites exceeds 2048", ns); return EPHMC_ERR_UNSUPPORTED; }
 	ephmc_ctx *c = new ephmc_ctx();
@@ -411,6 +425,8 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 		set_error("initial uploads failed");
 		return fail(EPHMC_ERR_HIP);
 	}
+	// the stabilised product's buffers now, so that a captured trajectory allocates nothing
+	if(cfg->stabilize && (rc = stab_reserve(c)) != EPHMC_OK) return fail(rc);
 	*out = c;
 	return EPHMC_OK;
 }
@@ -424,6 +440,7 @@ int ephmc_destroy(ephmc_ctx *c)
 		c->d_x, c->d_p, c->d_pdot, c->d_xnew, c->d_normals, c->d_minv, c->d_mhat, c->d_Rt0, c->d_R0t, c->d_ccg};
 	for(void *b : bufs) if(b) (void)hipFree(b);
 	tl_release(c);
+	stab_release(c);
 	if(c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
 	if(c->graph) (void)hipGraphDestroy(c->graph);
 	for(int s = 0; s < 2; s++){

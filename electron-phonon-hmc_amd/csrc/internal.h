@@ -83,6 +83,13 @@ struct TlWork {
 	double *res = nullptr, *part = nullptr, *acc = nullptr, *red = nullptr;
 };
 
+// Buffers of the stabilised slice product (allocated on first use): U, V, the QR work matrix and
+// X (N x N each), the column scales d, reflector factors tau, the column order, the reflector count.
+struct StabWork {
+	double *U = nullptr, *V = nullptr, *ZR = nullptr, *X = nullptr, *d = nullptr, *tau = nullptr;
+	int *perm = nullptr, *nref = nullptr;
+};
+
 } // namespace ephmc
 
 struct ephmc_ctx {
@@ -140,6 +147,9 @@ struct ephmc_ctx {
 
 	// transient localisation
 	ephmc::TlWork tl;
+
+	// stabilised slice product (ephmc_config.stabilize, ephmc_stable_logdet_inv)
+	ephmc::StabWork stab;
 
 	// chain state
 	double *d_x, *d_p, *d_pdot, *d_xnew, *d_minv, *d_mhat, *d_normals;
@@ -224,5 +234,10 @@ int metropolis(ephmc_ctx *c, double uniform);
 
 // tl.hip
 void tl_release(ephmc_ctx *c);
+
+// stab.hip
+int stab_reserve(ephmc_ctx *c);
+void stab_release(ephmc_ctx *c);
+int stable_logdet_inverse(ephmc_ctx *c, int prec, unsigned k, double *minv);
 
 } // namespace ephmc

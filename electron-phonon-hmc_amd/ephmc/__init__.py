@@ -27,7 +27,7 @@ class Config(C.Structure):
                 ("zero_filling", C.c_int), ("single_particle", C.c_int), ("constant_force", C.c_int),
                 ("no_fourier_acc", C.c_int), ("classical_fourier_acc", C.c_int),
                 ("cfa_steps", C.c_uint), ("cfa_mass", C.c_double),
-                ("device", C.c_int), ("stream", C.c_void_p), ("graph", C.c_int)]
+                ("device", C.c_int), ("stream", C.c_void_p), ("graph", C.c_int), ("stabilize", C.c_uint)]
 
 
 class TuneResult(C.Structure):
@@ -87,6 +87,7 @@ def _load():
         "ephmc_autotune": [vp, C.POINTER(TuneResult)],
         "ephmc_last_pivots": [vp, C.POINTER(C.c_int)],
         "ephmc_md_force": [vp, vp, vp],
+        "ephmc_stable_logdet_inv": [vp, vp, C.c_int, C.c_uint, vp, dp, C.POINTER(C.c_int)],
         "ephmc_apply_mu_K": [vp, vp, vp, vp, C.c_uint],
         "ephmc_apply_pairs": [vp, vp, vp, C.c_uint, C.c_int, dp, dp],
         "ephmc_apply_exp_cols": [vp, vp, vp, C.c_uint, C.c_int, C.c_int],
@@ -224,6 +225,14 @@ class Context:
         dm = self.array(self.mat, Minv) if Minv is not None else None
         check(LIB.ephmc_force(self.h, dx.ptr, dm.ptr if dm is not None else None, logdet, dp_.ptr))
         return dp_.download()
+
+    def stable_logdet_inv(self, x, precision, k):
+        """(log det, sign, inverse) of 1 + B..B from the product stabilised by an FP64 QR every k
+        slices, slices in `precision`."""
+        dx, dm = self.array(self.dim, x), self.array(self.mat)
+        ld, sg = C.c_double(), C.c_int()
+        check(LIB.ephmc_stable_logdet_inv(self.h, dx.ptr, precision, k, dm.ptr, C.byref(ld), C.byref(sg)))
+        return ld.value, sg.value, dm.download()
 
     def md_force(self, x):
         """The force of one molecular-dynamics step at x (the context's chain precision builds

@@ -58,6 +58,9 @@ typedef struct ephmc_config {
 	void *stream;            /* hipStream_t to run on; NULL creates a default-priority one */
 	int graph;               /* 1: ephmc_trajectory replays its molecular dynamics as a
 	                            captured hipGraph (captured on first use; not while timing) */
+	unsigned stabilize;      /* k > 0: M_hat^{-1} and its log det (Metropolis and molecular
+	                            dynamics) from the slice product kept as U diag(d) V with an FP64
+	                            QR every k slices (ephmc_stable_logdet_inv); 0: the plain product */
 } ephmc_config;
 
 /* Per-trajectory record (organic_hmc.c:233-245, results columns 3,4,7,8). */
@@ -96,6 +99,15 @@ int ephmc_apply_exp_M(ephmc_ctx *ctx, const double *x_tau_dev, double *M_dev, in
 int ephmc_construct_M_hat(ephmc_ctx *ctx, const double *x_dev, double *M_hat_dev);
 
 /* ---- (c) log-det, inverse, force ------------------------------------------------------ */
+/* Stabilised log|det(1 + B_0 ... B_{nt-1})| and its inverse for field x: the product grown k slices
+ * at a time in the slice precision (EPHMC_FP64, or EPHMC_MIXED for FP32 slices in a mixed context),
+ * kept as U diag(d) V through a pre-pivoted FP64 Householder QR after every block, and inverted as
+ * X^{-1} D_b^{-1} U^T with X = D_b^{-1} U^T + D_s V (D_b = max(d, 1), D_s = min(d, 1)).  The
+ * periodic stabilisation that construct_M_hat + logDetM_and_inv_M_hat (organic_dense_matrix.c:251-265,
+ * :547-607) do without: equal to them where the plain product is well conditioned, and accurate
+ * where it is not.  M_inv_dev: N x N.  logdet_host / sign_host may be NULL (then asynchronous). */
+int ephmc_stable_logdet_inv(ephmc_ctx *ctx, const double *x_dev, int precision, unsigned k, double *M_inv_dev,
+		double *logdet_host, int *sign_host);
 /* log|det M_hat| and M_hat <- M_hat^{-1} in place (FP64, partial pivoting); replaces
  * logDetM_and_inv_M_hat (organic_dense_matrix.c:547-607).  SINGLE_PARTICLE: log tr M_hat,
  * M_hat unchanged.  logdet_host / sign_host may be NULL; when given, the call waits and
