@@ -14,6 +14,7 @@
 // entries at i and ia; block sums per (tau, j-chunk), then a fixed-order pass per tau.
 #include "internal.h"
 #include "alloc_group.h"
+#include <mutex>
 
 namespace ephmc {
 
@@ -167,9 +168,62 @@ static int correlate(ephmc_ctx *c, const double *x, const double *minv, double l
 	return EPHMC_OK;
 }
 
+// Allocation-free variant (ephmc_cc_correlator_static): its transfer-matrix stores and block sums
+// live in this file-scope device array, 64 MiB on every GPU the library runs on, held until the
+// process ends.  In host code `scratch` is only a symbol: its device address on each GPU comes from
+// hipGetSymbolAddress, taken once per device on the entry point's first call there, and only that
+// address is passed to kernels.  Calls that overlap on one GPU share (and race on) the array.
+constexpr size_t CC_STATIC_DOUBLES = (size_t)64 << 17;     // 64 MiB of doubles
+static __device__ __align__(256) double scratch[CC_STATIC_DOUBLES];
+
+constexpr int CC_MAX_DEVICES = 64;
+static double *g_scratch_addr[CC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// device address of `scratch` on the context's GPU (looked up once per device)
+static int static_scratch(ephmc_ctx *c, double **out)
+{
+	if(c->device < 0 || c->device >= CC_MAX_DEVICES){ set_error("device %d out of range", c->device); return EPHMC_ERR_ARG; }
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if(!g_scratch_addr[c->device]){
+		int prev = 0;
+		EPHMC_HIP(hipGetDevice(&prev));
+		EPHMC_HIP(hipSetDevice(c->device));
+		void *p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		(void)hipSetDevice(prev);
+		if(e != hipSuccess || !p){
+			(void)hipGetLastError();
+			set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", c->device, hipGetErrorString(e));
+			return EPHMC_ERR_HIP;
+		}
+		g_scratch_addr[c->device] = (double *)p;
+	}
+	*out = g_scratch_addr[c->device];
+	return EPHMC_OK;
+}
+
 } // namespace ephmc
 
 using namespace ephmc;
+
+extern "C" int ephmc_cc_correlator_static(ephmc_ctx *c, const double *x, const double *minv, double logdet, double *greens_dev)
+{
+	if(!c || !x || !greens_dev || (!minv && !c->cfg.zero_filling && !c->cfg.single_particle)){
+		set_error("null argument");
+		return EPHMC_ERR_ARG;
+	}
+	const size_t store = (size_t)c->nt * c->mat, need = 2 * store + cc_partials(c->ns, c->nt);
+	if(need > CC_STATIC_DOUBLES){
+		set_error("ephmc_cc_correlator_static: needs %zu doubles (2 nt N^2 + block sums), the static scratch holds %zu "
+			"(64 MiB); use ephmc_cc_correlator", need, CC_STATIC_DOUBLES);
+		return EPHMC_ERR_ARG;
+	}
+	double *base = nullptr;
+	EPHMC_TRY(static_scratch(c, &base));
+	// every value of the stores and block sums is written before it is read: nothing to clear
+	return correlate(c, x, minv, logdet, base, base + store, base + 2 * store, greens_dev);
+}
 
 extern "C" int ephmc_cc_correlator(ephmc_ctx *c, const double *x, const double *minv, double logdet, double *greens_host)
 {

@@ -88,6 +88,7 @@ def _load():
         "ephmc_last_pivots": [vp, C.POINTER(C.c_int)],
         "ephmc_md_force": [vp, vp, vp],
         "ephmc_stable_logdet_inv": [vp, vp, C.c_int, C.c_uint, vp, dp, C.POINTER(C.c_int)],
+        "ephmc_cc_correlator_static": [vp, vp, vp, C.c_double, vp],
         "ephmc_apply_mu_K": [vp, vp, vp, vp, C.c_uint],
         "ephmc_apply_pairs": [vp, vp, vp, C.c_uint, C.c_int, dp, dp],
         "ephmc_apply_exp_cols": [vp, vp, vp, C.c_uint, C.c_int, C.c_int],

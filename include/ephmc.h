@@ -215,6 +215,16 @@ int ephmc_chain_buffers(ephmc_ctx *ctx, double **x_dev, double **p_dev, double *
  * Allocates 2 nt N^2 doubles of device memory on first use.  Waits. */
 int ephmc_cc_correlator(ephmc_ctx *ctx, const double *x_dev, const double *M_inv_dev, double logdet, double *greens_host);
 
+/* Allocation-free variant of ephmc_cc_correlator.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops).  The
+ * transfer matrices and block sums live in a static 64 MiB device array (one per GPU, held until
+ * the process ends), so 2 nt N^2 + 9 nt ceil(N/8) <= 8,388,608 doubles (e.g. N = 256, Nt <= 63;
+ * larger sizes return EPHMC_ERR_ARG).  greens_dev: nt x 9 doubles in device memory, written
+ * asynchronously on the context's stream (no host copy, no wait).  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (two streams or threads) race. */
+int ephmc_cc_correlator_static(ephmc_ctx *ctx, const double *x_dev, const double *M_inv_dev, double logdet,
+		double *greens_dev);
+
 /* ---- (e) transient localisation ---------------------------------------------------------- */
 /* Batched disorder samples; replaces hist_trans_loc x meas (organic_trans_loc.c:25-83,
  * :101-105) with the normals of each sample given (3N per sample, sample-major).
