@@ -49,6 +49,7 @@ struct AdjArgs {
 	const double *fscale;   // device scalar multiplying the force (SINGLE_PARTICLE: exp(-logdet)), or null
 	double *pdot, *p;       // outputs of the reduction
 	double kick;            // p += kick F when p != null
+	int clip;               // SCHUR_FORM_INV: bond projections clipped to [-2, 2] (organic_hmc.c:113-114)
 };
 
 // Deterministic sum over workgroups of the partial bond sums of one segment:
@@ -62,7 +63,7 @@ constexpr unsigned RED_CHUNKS = 4;
 template <typename P>
 __device__ void reduce_links(const P *__restrict__ part, unsigned nwg, unsigned loc, unsigned l0, unsigned nl,
 		unsigned nlinks_total, int tau_b, const double (&f)[3], const double *__restrict__ fscale, double *__restrict__ pdot,
-		double *__restrict__ p, double kick, double *scratch, unsigned nthreads)
+		double *__restrict__ p, double kick, double *scratch, unsigned nthreads, int clip)
 {
 	const double fs = fscale ? *fscale : 1.0;
 	const unsigned t = threadIdx.x;
@@ -97,6 +98,7 @@ __device__ void reduce_links(const P *__restrict__ part, unsigned nwg, unsigned 
 				double tot = 0;
 				for(unsigned q = 0; q < CH; q++) tot += scratch[q * G + ll];
 				const unsigned tl = lk2 / loc, link = lk2 - tl * loc;
+				if(clip && fabs(tot) > 2.0) tot = tot > 0 ? 2.0 : -2.0;     // NaN stays NaN
 				const double F = f[link % 3] * tot * fs;
 				const size_t pos = (size_t)(tau_b + tl) * loc + link;
 				if(pdot) pdot[pos] = F;
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 		const unsigned total = (unsigned)a.red_ntau * 3 * ns;
 		const unsigned per = (total + a.nwg - 1) / a.nwg;
 		reduce_links(a.red_src, a.nwg, 3 * ns, blockIdx.x * per, per, total, a.red_tau_b, a.f, a.fscale, a.pdot, a.p, a.kick,
-			rscratch, BT);
+			rscratch, BT, a.clip);
 	}
 	R *wbs[2] = {wb0, wb1};
 
@@ -484,12 +486,12 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 template <typename P>
 __global__ __launch_bounds__(256) void force_reduce_kernel(const P *__restrict__ partial, unsigned nwg, unsigned loc,
 		int tau_b, int ntau, double f0, double f1, double f2, const double *__restrict__ fscale, double *__restrict__ pdot,
-		double *__restrict__ p, double kick)
+		double *__restrict__ p, double kick, int clip)
 {
 	__shared__ double scratch[256];
 	const double f[3] = {f0, f1, f2};
 	const unsigned total = (unsigned)ntau * loc;
-	reduce_links(partial, nwg, loc, blockIdx.x * 64, 64, total, tau_b, f, fscale, pdot, p, kick, scratch, 256);
+	reduce_links(partial, nwg, loc, blockIdx.x * 64, 64, total, tau_b, f, fscale, pdot, p, kick, scratch, 256, clip);
 }
 
 // CONSTANT_FORCE: every slice gets the force of slice 0 (organic_hmc.c:92-98,128)
@@ -602,6 +604,7 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 	a.p = p;
 	a.kick = kick;
 	a.fscale = fscale;
+	a.clip = c->cfg.schur_form_inv != 0;
 	const unsigned nseg = (nt + span - 1) / span;
 	int prev_b = 0, prev_n = 0;
 	const R *prev = nullptr;
@@ -633,7 +636,7 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 	const size_t n = (size_t)prev_n * c->loc;
 	EPHMC_TRY(timer_begin(c, "force_reduce"));
 	force_reduce_kernel<R><<<(unsigned)((n + 63) / 64), 256, 0, c->stream>>>(prev, a.nwg, c->loc, prev_b, prev_n,
-		f[0], f[1], f[2], fscale, pdot, p, kick);
+		f[0], f[1], f[2], fscale, pdot, p, kick, a.clip);
 	EPHMC_LAUNCHED();
 	EPHMC_TRY(timer_end(c, "force_reduce"));
 	return EPHMC_OK;
@@ -657,6 +660,12 @@ int force_constant(ephmc_ctx *c, int prec, const double *minv, double *pdot, con
 	EPHMC_TRY(chain_range(c, prec, minv, w0, (int)c->nt - 1, -1, (int)c->nt - 1));
 	if(prec == EPHMC_FP64) EPHMC_TRY(run_adjoint<double>(c, prec, nullptr, pdot, 0, nullptr, fscale, 1));
 	else EPHMC_TRY(run_adjoint<float>(c, prec, nullptr, pdot, 0, nullptr, fscale, 1));
+	return broadcast_slice0(c, pdot);
+}
+
+// every slice gets the force of slice 0
+int broadcast_slice0(ephmc_ctx *c, double *pdot)
+{
 	if(c->nt > 1){
 		broadcast_slice_kernel<<<1024, 256, 0, c->stream>>>(pdot, c->loc, c->nt);
 		EPHMC_LAUNCHED();

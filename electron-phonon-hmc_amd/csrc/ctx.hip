@@ -159,10 +159,19 @@ static unsigned adjoint_cols_auto(unsigned ns, int prec)
 	return 1;
 }
 
+// log|det M| and M <- M^{-1}: the Gauss-Jordan kernels, or through the real Schur form of M - 1
+// under SCHUR_FORM_INV (organic_dense_matrix.c:556-603)
+static int logdet_inverse(ephmc_ctx *c, double *M)
+{
+	return c->cfg.schur_form_inv ? schur_logdet_inverse(c, M) : gj_logdet_inverse(c, M);
+}
+
 // Adjoint force for the current coefficients and checkpoints.  minv: M_hat^{-1} (fp64), or
 // null for SINGLE_PARTICLE (the sweep starts from the identity).
 static int force_dispatch(ephmc_ctx *c, int prec, const double *minv, double *pdot, const double *fscale)
 {
+	// CONSTANT_FORCE under SCHUR_FORM_INV: QM1P = 1 - M_hat^{-1}, no chain rule (organic_dense_matrix.c:270-278)
+	if(c->cfg.constant_force && c->cfg.schur_form_inv) return schur_constant_force(c, minv, pdot);
 	if(c->cfg.constant_force) return force_constant(c, prec, minv, pdot, fscale);
 	if(!minv) EPHMC_TRY(set_identity(c, prec, prec == EPHMC_FP64 ? (void *)c->d_Tfin : (void *)c->d_Tfin32));
 	return force_adjoint(c, prec, minv, pdot, 0, nullptr, fscale);
@@ -196,7 +205,7 @@ static int md_force(ephmc_ctx *c, const double *x, double *pdot)
 		minv = nullptr;
 	}else{
 		EPHMC_TRY(timer_begin(c, "gj"));
-		EPHMC_TRY(gj_logdet_inverse(c, c->d_mhat));
+		EPHMC_TRY(logdet_inverse(c, c->d_mhat));
 		EPHMC_TRY(timer_end(c, "gj"));
 	}
 	EPHMC_TRY(timer_begin(c, "adjoint"));
@@ -220,7 +229,7 @@ static int exact_logdet(ephmc_ctx *c, const double *x)
 	if(c->cfg.zero_filling) return set_scalar(c, SCAL_LOGDET, 0.0);
 	if(c->cfg.single_particle) return log_trace(c, EPHMC_FP64, c->d_mhat, SCAL_LOGDET, -1);
 	EPHMC_TRY(add_identity(c, c->d_mhat));
-	EPHMC_TRY(gj_logdet_inverse(c, c->d_mhat));
+	EPHMC_TRY(logdet_inverse(c, c->d_mhat));
 	return EPHMC_OK;
 }
 
@@ -328,6 +337,14 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 	if(cfg->precision != EPHMC_FP64 && cfg->precision != EPHMC_MIXED){ set_error("precision"); return EPHMC_ERR_ARG; }
 	if(cfg->classical_fourier_acc && cfg->cfa_steps == 0){ set_error("CLASSICAL_FOURIER_ACC needs cfa_steps >= 1"); return EPHMC_ERR_ARG; }
 	if(cfg->no_fourier_acc && cfg->classical_fourier_acc){ set_error("NO_FOURIER_ACC and CLASSICAL_FOURIER_ACC exclude each other"); return EPHMC_ERR_ARG; }
+	if(cfg->schur_form_inv && cfg->stabilize){
+		set_error("SCHUR_FORM_INV and stabilize exclude each other");
+		return EPHMC_ERR_UNSUPPORTED;
+	}
+	if(cfg->schur_form_inv && cfg->single_particle && cfg->constant_force){
+		set_error("SCHUR_FORM_INV with SINGLE_PARTICLE and CONSTANT_FORCE is not provided");
+		return EPHMC_ERR_UNSUPPORTED;
+	}
 	if(cfg->stabilize && (cfg->zero_filling || cfg->single_particle)){
 		set_error("stabilize: the stabilised product forms 1 + B..B, not provided for ZERO_FILLING / SINGLE_PARTICLE");
 		return EPHMC_ERR_UNSUPPORTED;
@@ -427,6 +444,7 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 	}
 	// the stabilised product's buffers now, so that a captured trajectory allocates nothing
 	if(cfg->stabilize && (rc = stab_reserve(c)) != EPHMC_OK) return fail(rc);
+	if(cfg->schur_form_inv && (rc = schur_reserve(c)) != EPHMC_OK) return fail(rc);
 	*out = c;
 	return EPHMC_OK;
 }
@@ -441,6 +459,7 @@ int ephmc_destroy(ephmc_ctx *c)
 	for(void *b : bufs) if(b) (void)hipFree(b);
 	tl_release(c);
 	stab_release(c);
+	schur_release(c);
 	if(c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
 	if(c->graph) (void)hipGraphDestroy(c->graph);
 	for(int s = 0; s < 2; s++){
@@ -533,14 +552,19 @@ int ephmc_logdet_inv(ephmc_ctx *c, double *M_hat, double *logdet, int *sign)
 		return EPHMC_OK;
 	}
 	EPHMC_TRY(timer_begin(c, "gj"));
-	EPHMC_TRY(gj_logdet_inverse(c, M_hat));
+	EPHMC_TRY(logdet_inverse(c, M_hat));
 	EPHMC_TRY(timer_end(c, "gj"));
 	if(!logdet && !sign) return EPHMC_OK;
-	double ld, sg, sing;
+	double ld, sg, sing, sfail;
 	EPHMC_TRY(read_scal(c, SCAL_LOGDET, &ld));
 	EPHMC_TRY(read_scal(c, SCAL_SIGN, &sg));
 	EPHMC_TRY(read_scal(c, SCAL_SINGULAR, &sing));
+	EPHMC_TRY(read_scal(c, SCAL_SCHUR_FAIL, &sfail));
 	if(logdet) *logdet = ld;
+	if(sfail != 0){
+		set_error("Schur form: the QR iteration did not converge at index %d", (int)sfail - 1);
+		return EPHMC_ERR_NUMERIC;
+	}
 	if(sign) *sign = sg < 0 ? -1 : 1;
 	// a zero pivot: log|det| = -inf as in the reference (organic_dense_matrix.c:582-584)
 	if(sing != 0 || ld == -HUGE_VAL){

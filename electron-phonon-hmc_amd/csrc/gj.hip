@@ -221,6 +221,7 @@ __global__ __launch_bounds__(256) void gj_first_kernel(const double *__restrict_
 	if(threadIdx.x == 0){
 		scal[SCAL_GJ_FLAG] = 0;     // no multiplier above 1 seen yet
 		scal[SCAL_SINGULAR] = 0;
+		scal[SCAL_SCHUR_FAIL] = 0;
 	}
 	__syncthreads();
 	double a[8];

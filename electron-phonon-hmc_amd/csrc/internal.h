@@ -83,6 +83,14 @@ struct TlWork {
 	double *res = nullptr, *part = nullptr, *acc = nullptr, *red = nullptr;
 };
 
+// Buffers of the Schur-form inverse (SCHUR_FORM_INV; allocated on first use): the quasi-triangular
+// factor, the Schur vectors, a product, reflector factors, eigenvalues, the threshold record and
+// the iteration status.
+struct SchurWork {
+	double *H = nullptr, *Z = nullptr, *P = nullptr, *tau = nullptr, *wr = nullptr, *wi = nullptr, *info = nullptr;
+	int *status = nullptr;
+};
+
 // Buffers of the stabilised slice product (allocated on first use): U, V, the QR work matrix and
 // X (N x N each), the column scales d, reflector factors tau, the column order, the reflector count.
 struct StabWork {
@@ -151,6 +159,9 @@ struct ephmc_ctx {
 	// stabilised slice product (ephmc_config.stabilize, ephmc_stable_logdet_inv)
 	ephmc::StabWork stab;
 
+	// Schur-form inverse (ephmc_config.schur_form_inv, ephmc_schur)
+	ephmc::SchurWork schur;
+
 	// chain state
 	double *d_x, *d_p, *d_pdot, *d_xnew, *d_minv, *d_mhat, *d_normals;
 	double *h_stage[2];               // pinned staging for normals
@@ -187,7 +198,8 @@ enum Scalar {            // layout of d_scal
 	SCAL_GJ_FALLBACKS = 13, // number of pivoted fallback inversions run
 	SCAL_FSCALE = 14,     // force factor exp(-logdet) of SINGLE_PARTICLE
 	SCAL_SINGULAR = 15,   // 1 + step of a zero pivot column in the last inversion, or 0
-	SCAL_COUNT = 16
+	SCAL_SCHUR_FAIL = 16, // 1 + index where the Schur iteration of the last inversion stopped, or 0
+	SCAL_COUNT = 20
 };
 
 // timing helpers
@@ -216,6 +228,7 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M);
 // adjoint.hip
 int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p, const double *fscale);
 int force_constant(ephmc_ctx *c, int prec, const double *minv, double *pdot, const double *fscale);
+int broadcast_slice0(ephmc_ctx *c, double *pdot);
 // LDS bytes of one force-sweep workgroup at C columns (the launch shape used for ns sites)
 size_t adjoint_lds(unsigned ns, int prec, unsigned C);
 
@@ -239,5 +252,11 @@ void tl_release(ephmc_ctx *c);
 int stab_reserve(ephmc_ctx *c);
 void stab_release(ephmc_ctx *c);
 int stable_logdet_inverse(ephmc_ctx *c, int prec, unsigned k, double *minv);
+
+// schur.hip
+int schur_reserve(ephmc_ctx *c);
+void schur_release(ephmc_ctx *c);
+int schur_logdet_inverse(ephmc_ctx *c, double *M);
+int schur_constant_force(ephmc_ctx *c, const double *minv, double *pdot);
 
 } // namespace ephmc

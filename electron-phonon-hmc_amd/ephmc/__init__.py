@@ -27,7 +27,7 @@ class Config(C.Structure):
                 ("zero_filling", C.c_int), ("single_particle", C.c_int), ("constant_force", C.c_int),
                 ("no_fourier_acc", C.c_int), ("classical_fourier_acc", C.c_int),
                 ("cfa_steps", C.c_uint), ("cfa_mass", C.c_double),
-                ("device", C.c_int), ("stream", C.c_void_p), ("graph", C.c_int), ("stabilize", C.c_uint)]
+                ("device", C.c_int), ("stream", C.c_void_p), ("graph", C.c_int), ("schur_form_inv", C.c_int), ("stabilize", C.c_uint)]
 
 
 class TuneResult(C.Structure):
@@ -87,6 +87,7 @@ def _load():
         "ephmc_autotune": [vp, C.POINTER(TuneResult)],
         "ephmc_last_pivots": [vp, C.POINTER(C.c_int)],
         "ephmc_md_force": [vp, vp, vp],
+        "ephmc_schur": [vp, vp, vp, vp, dp, dp],
         "ephmc_stable_logdet_inv": [vp, vp, C.c_int, C.c_uint, vp, dp, C.POINTER(C.c_int)],
         "ephmc_cc_correlator_static": [vp, vp, vp, C.c_double, vp],
         "ephmc_apply_mu_K": [vp, vp, vp, vp, C.c_uint],
@@ -234,6 +235,13 @@ class Context:
         ld, sg = C.c_double(), C.c_int()
         check(LIB.ephmc_stable_logdet_inv(self.h, dx.ptr, precision, k, dm.ptr, C.byref(ld), C.byref(sg)))
         return ld.value, sg.value, dm.download()
+
+    def schur(self, B):
+        """Real Schur form of the N x N matrix B (column-major flat): (Z, T, wr, wi)."""
+        db, dz, dt = self.array(self.mat, B), self.array(self.mat), self.array(self.mat)
+        wr, wi = np.zeros(self.ns), np.zeros(self.ns)
+        check(LIB.ephmc_schur(self.h, db.ptr, dz.ptr, dt.ptr, _dp(wr), _dp(wi)))
+        return dz.download(), dt.download(), wr, wi
 
     def md_force(self, x):
         """The force of one molecular-dynamics step at x (the context's chain precision builds

@@ -66,8 +66,8 @@ static const organic_flags no_flags;
 
 static void check_flags(const organic_flags *mode, const char *who)
 {
-	if(mode && mode->schur_form_inv){
-		fprintf(stderr, "organic: %s: SCHUR_FORM_INV is not provided by this build\n", who);
+	if(mode && mode->schur_form_inv && mode->single_particle && mode->constant_force){
+		fprintf(stderr, "organic: %s: SCHUR_FORM_INV with SINGLE_PARTICLE and CONSTANT_FORCE is not provided\n", who);
 		exit(EXIT_FAILURE);
 	}
 }
@@ -128,6 +128,7 @@ static struct organic_engine *engine_get(unsigned l1, unsigned l2, unsigned nt, 
 	cfg.classical_fourier_acc = mode->classical_fourier_acc;
 	cfg.cfa_steps = mode->cfa_steps;
 	cfg.cfa_mass = mode->cfa_mass;
+	cfg.schur_form_inv = mode->schur_form_inv;
 	GPU(ephmc_create(&cfg, &e->ctx));
 	/* EPHMC_AUTOTUNE=1: measured launch shapes (faster, but the choice follows timing noise and
 	 * moves the force's rounding, so runs are no longer bit-reproducible) */
@@ -255,10 +256,13 @@ double logDetM_and_inv_M_hat(double *M_hat, double *w, int *p, unsigned ns, unsi
 	if(mode && mode->single_particle) return log(trace(M_hat, ns));   /* M_hat unchanged */
 	struct organic_engine *e = NULL;
 	for(int i = 0; i < MAX_ENGINES && !e; i++)
-		if(engines[i] && engines[i]->ns == ns && !engines[i]->mode.single_particle) e = engines[i];
+		if(engines[i] && engines[i]->ns == ns && !engines[i]->mode.single_particle
+			&& engines[i]->mode.schur_form_inv == (mode ? mode->schur_form_inv : 0)) e = engines[i];
 	if(!e){
 		const double z[3] = {0, 0, 0};
-		e = engine_get(ns, 1, 1, z, z, 0, 1, flavors, 1, 0, NULL);
+		organic_flags m = {0};
+		if(mode) m.schur_form_inv = mode->schur_form_inv;
+		e = engine_get(ns, 1, 1, z, z, 0, 1, flavors, 1, 0, &m);
 	}
 	up(e, e->d_m, M_hat, e->mat);
 	double logdet;

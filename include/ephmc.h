@@ -58,6 +58,11 @@ typedef struct ephmc_config {
 	void *stream;            /* hipStream_t to run on; NULL creates a default-priority one */
 	int graph;               /* 1: ephmc_trajectory replays its molecular dynamics as a
 	                            captured hipGraph (captured on first use; not while timing) */
+	int schur_form_inv;      /* SCHUR_FORM_INV: every M_hat^{-1} and log det through the real Schur
+	                            form of M_hat - 1 with small entries zeroed
+	                            (organic_dense_matrix.c:556-603), bond projections clipped to
+	                            [-2, 2] (organic_hmc.c:113-114), CONSTANT_FORCE with
+	                            QM1P = 1 - M_hat^{-1} (organic_dense_matrix.c:270-278) */
 	unsigned stabilize;      /* k > 0: M_hat^{-1} and its log det (Metropolis and molecular
 	                            dynamics) from the slice product kept as U diag(d) V with an FP64
 	                            QR every k slices (ephmc_stable_logdet_inv); 0: the plain product */
@@ -114,6 +119,13 @@ int ephmc_stable_logdet_inv(ephmc_ctx *ctx, const double *x_dev, int precision, 
  * reports a singular matrix (a zero pivot) as EPHMC_ERR_NUMERIC with *logdet_host = -inf
  * (the reference's log|0|, :582-584) and M_hat filled with NaN. */
 int ephmc_logdet_inv(ephmc_ctx *ctx, double *M_hat_dev, double *logdet_host, int *sign_host);
+/* Real Schur form B = Z T Z^T of an N x N matrix (column-major, device): T quasi-upper-triangular
+ * with 2 x 2 blocks in standard form (real pair: triangular; complex pair: equal diagonal, off-
+ * diagonal entries of opposite sign), Z orthogonal, eigenvalues wr + i wi (host arrays, may be
+ * NULL); the factorisation of LAPACK dgees('V', 'N') that the SCHUR_FORM_INV branch uses
+ * (organic_dense_matrix.c:560).  Hessenberg reduction and Francis double-shift QR, FP64.  Waits;
+ * EPHMC_ERR_NUMERIC if the iteration does not converge. */
+int ephmc_schur(ephmc_ctx *ctx, const double *B_dev, double *Z_dev, double *T_dev, double *wr_host, double *wi_host);
 /* Row interchanges of the last ephmc_logdet_inv in LAPACK's 1-based ipiv form (row i was
  * exchanged with row ipiv[i] at step i): the identity when no exchange was needed, else the
  * pivoted factorisation's; the p argument of logDetM_and_inv_M_hat (dgetrf's ipiv,
