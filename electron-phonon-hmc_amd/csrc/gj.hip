@@ -39,69 +39,53 @@ __device__ inline double rcp_nr(double x)
 
 // Diagonal-block inversion.  D^{-1} by Gauss-Jordan on [D | 1] (32 x 64) without exchanges,
 // four pivots g..g+3 per step eliminated together through their 4 x 4 block P (rows and
-// columns g..g+3): pivot rows <- P^{-1} pivot rows, every other row r -= a_r,[g..g+3] (new pivot
-// rows).  Wave w holds rows 8w..8w+7 in a[], lane c column c (lanes 32..63 end with D^{-1}); one
-// barrier per step publishes the four pivot rows and the four pivot columns.  The pivot columns
-// of every step are kept, and the LU multipliers of D are formed from them afterwards: with
-// P = L_P U_P (U_P and the pivots U_gg come out of the block's own elimination), row r's
-// multipliers of the step are l[r][g+i] = (a_r,g+i - sum_{q<i} l[r][g+q] U_P[q][i]) / U_P[i][i]
-// (the division by a refined reciprocal; the multipliers only feed the |l| <= 1 check).
-// Writes dinv_out[r][c] = D^{-1}[r][c] and lm_out[m][j] = L[m][j] (unit lower triangular, zero
-// elsewhere, NB x NB row-major), the pivots of rows kb.. to diag/piv, and raises the flag when a
-// multiplier exceeds 1.  Called by all 256 threads of a workgroup.
+// columns g..g+3): with P = L_P U_P (no exchanges), every column's new pivot rows are
+// nr = P^{-1} u (u: the column's entries in the pivot rows) by forward and back substitution,
+// and every other row r -= a_r,[g..g+3] nr.  Wave w holds rows 8w..8w+7 in a[], lane c column c
+// (lanes 32..63 end with D^{-1}); one barrier per step publishes the four pivot rows and the four
+// pivot columns.  P's factorisation is formed by every thread (the same values everywhere, so no
+// second barrier is needed to share it); thread 0 keeps U_P per step in LDS (reciprocal pivots on
+// the diagonal) and the pivots U_gg, which are written out after the loop.  After the loop the LU multipliers of D are formed
+// from the kept pivot columns: row r's multipliers of step s are
+// l[r][g+i] = (a_r,g+i - sum_{q<i} l[r][g+q] U_P[q][i]) / U_P[i][i]  (they only feed the
+// |l| <= 1 check).  Writes dinv_out[r][c] = D^{-1}[r][c] and lm_out[m][j] = L[m][j] (unit lower
+// triangular, zero elsewhere, NB x NB row-major), the pivots of rows kb.. to diag/piv, and raises
+// the flag when a multiplier exceeds 1.  Called by all 256 threads of a workgroup.
 struct GjInvLds {
 	__align__(16) double prow[2][4][2 * GJ_NB];     // pivot rows of the step, by step parity
-	__align__(16) double pcs[GJ_NB / 4][4][GJ_NB];  // pivot columns of every step
+	__align__(16) double pcs[GJ_NB / 4][4][GJ_NB + 2];  // pivot columns of every step (padded: the
+	                                                // multiplier pass reads 8 steps at once)
+	double ublk[GJ_NB / 4][4][4];                   // U_P of every step, reciprocal pivots on the diagonal
+	double pvs[GJ_NB];                              // the pivots U_gg
 };
 
-// P[m][i] = row g+m, column g+i of the pivot block of step s
-__device__ __forceinline__ void gj_pivot_block(const GjInvLds &sh, int s, double (&P)[4][4])
+// L and U of the 4 x 4 pivot block of step s (P[m][i] = row g+m, column g+i) without exchanges:
+// lo[m][k] (m > k) the multipliers, up[k][i] (i > k) U, rp[k] = 1 / U[k][k], pv[k] = U[k][k]
+__device__ __forceinline__ void gj_lu4(const GjInvLds &sh, int s, double (&lo)[4][4], double (&up)[4][4], double (&rp)[4],
+		double (&pv)[4])
 {
 	const int g = 4 * s;
+	double M[4][4];
 #pragma unroll
 	for(int i = 0; i < 4; i++){
 		const double2 x0 = *(const double2 *)&sh.pcs[s][i][g], x1 = *(const double2 *)&sh.pcs[s][i][g + 2];
-		P[0][i] = x0.x;
-		P[1][i] = x0.y;
-		P[2][i] = x1.x;
-		P[3][i] = x1.y;
+		M[0][i] = x0.x;
+		M[1][i] = x0.y;
+		M[2][i] = x1.x;
+		M[3][i] = x1.y;
 	}
-}
-
-// Q = P^{-1} by Gauss-Jordan without exchanges on the 4 x 4 block.  Row k just before it is
-// scaled is row k of U in P = L_P U_P (only the rows below have been subtracted from it):
-// U[k][i] (i > k) and the reciprocal of the pivot U[k][k] are returned in U (its diagonal holds
-// the reciprocals) and the pivots in piv.
-__device__ __forceinline__ void gj_inv4(const double (&P)[4][4], double (&Q)[4][4], double (&U)[4][4], double (&pv)[4])
-{
-	double M[4][4];
-#pragma unroll
-	for(int m = 0; m < 4; m++)
-#pragma unroll
-		for(int i = 0; i < 4; i++){
-			M[m][i] = P[m][i];
-			Q[m][i] = m == i ? 1.0 : 0.0;
-		}
 #pragma unroll
 	for(int k = 0; k < 4; k++){
-		const double r = rcp_nr(M[k][k]);
 		pv[k] = M[k][k];
+		rp[k] = rcp_nr(M[k][k]);
 #pragma unroll
-		for(int i = 0; i < 4; i++) U[k][i] = i == k ? r : (i > k ? M[k][i] : 0.0);
+		for(int i = k + 1; i < 4; i++) up[k][i] = M[k][i];
 #pragma unroll
-		for(int i = 0; i < 4; i++){
-			M[k][i] *= r;
-			Q[k][i] *= r;
-		}
+		for(int m = k + 1; m < 4; m++){
+			const double l = M[m][k] * rp[k];
+			lo[m][k] = l;
 #pragma unroll
-		for(int m = 0; m < 4; m++){
-			if(m == k) continue;
-			const double f = M[m][k];
-#pragma unroll
-			for(int i = 0; i < 4; i++){
-				M[m][i] = fmac(-f, M[k][i], M[m][i]);
-				Q[m][i] = fmac(-f, Q[k][i], Q[m][i]);
-			}
+			for(int i = k + 1; i < 4; i++) M[m][i] = fmac(-l, M[k][i], M[m][i]);
 		}
 	}
 }
@@ -111,7 +95,6 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 {
 	constexpr int NB = GJ_NB;
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-	double my_u[4][4] = {}, my_pv[4] = {};
 	for(int s = 0; s < NB / 4; s++){
 		const int g = 4 * s, b = s & 1, gq = g & 7;
 		if(g >= (int)nbe) break;
@@ -130,28 +113,35 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 			for(int q = 0; q < 8; q += 2) *(double2 *)(dst + q) = make_double2(a[q], a[q + 1]);
 		}
 		__syncthreads();
-		double P[4][4], Q[4][4], U[4][4], pv[4];
-		gj_pivot_block(sh, s, P);
-		gj_inv4(P, Q, U, pv);
-		// every thread keeps U of the step its multipliers belong to (column tid % NB), and thread
-		// s the pivots of step s (register copies only: no memory traffic inside the loop)
-		if(((tid % NB) >> 2) == s)
+		double lo[4][4], up[4][4], rp[4], pv[4];
+		gj_lu4(sh, s, lo, up, rp, pv);
+		// (LDS only inside the loop: a global store here would make the next barrier wait for it)
+		if(tid == 0){
 #pragma unroll
-			for(int q = 0; q < 4; q++)
+			for(int k = 0; k < 4; k++){
 #pragma unroll
-				for(int i = 0; i < 4; i++) my_u[q][i] = U[q][i];
-		if(tid == s)
-#pragma unroll
-			for(int q = 0; q < 4; q++) my_pv[q] = pv[q];
-		double u[4], nr[4];
+				for(int i = 0; i < 4; i++) sh.ublk[s][k][i] = i == k ? rp[k] : (i > k ? up[k][i] : 0.0);
+				sh.pvs[g + k] = pv[k];
+			}
+		}
+		// new pivot rows of this column: y = L^{-1} u, nr = U^{-1} y
+		double u[4];
 #pragma unroll
 		for(int i = 0; i < 4; i++) u[i] = sh.prow[b][i][lane];
+		double y[4], nr[4];
 #pragma unroll
 		for(int m = 0; m < 4; m++){
-			double x = Q[m][0] * u[0];
+			double x = u[m];
 #pragma unroll
-			for(int i = 1; i < 4; i++) x = fmac(Q[m][i], u[i], x);
-			nr[m] = x;
+			for(int k = 0; k < m; k++) x = fmac(-lo[m][k], y[k], x);
+			y[m] = x;
+		}
+#pragma unroll
+		for(int m = 3; m >= 0; m--){
+			double x = y[m];
+#pragma unroll
+			for(int i = m + 1; i < 4; i++) x = fmac(-up[m][i], nr[i], x);
+			nr[m] = x * rp[m];
 		}
 		double c[8][4];
 #pragma unroll
@@ -171,21 +161,25 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 			a[q] = r == g ? nr[0] : (r == g + 1 ? nr[1] : (r == g + 2 ? nr[2] : (r == g + 3 ? nr[3] : x)));
 		}
 	}
+	__syncthreads();        // U_P and the pivots of the last step (before any global store: the
+	                        // barrier would wait for them)
 	if(lane >= NB){
 #pragma unroll
 		for(int q = 0; q < 8; q++) dinv_out[(8 * wave + q) * NB + lane - NB] = a[q];
 	}
-	// the pivots, and the multipliers from the pivot columns and the blocks' U
-	if(tid < NB / 4 && 4 * tid < (int)nbe)
-#pragma unroll
-		for(int q = 0; q < 4; q++)
-			if(4 * tid + q < (int)nbe){
-				diag[kb + 4 * tid + q] = my_pv[q];
-				piv[kb + 4 * tid + q] = (int)(kb + 4 * tid + q);
-			}
+	if(tid < (int)nbe){
+		diag[kb + tid] = sh.pvs[tid];
+		piv[kb + tid] = (int)(kb + tid);
+	}
+	// the multipliers from the pivot columns and the steps' U_P
 	bool bad = false;
 	{
 		const int j = tid % NB, s = j >> 2, ii = j & 3;
+		double my_u[4][4];
+#pragma unroll
+		for(int q = 0; q < 4; q++)
+#pragma unroll
+			for(int i = 0; i < 4; i++) my_u[q][i] = sh.ublk[s][q][i];
 #pragma unroll
 		for(int i = 0; i < NB * NB / 256; i++){
 			const int e = tid + 256 * i, r = e / NB;
