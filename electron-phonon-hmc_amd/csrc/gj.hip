@@ -11,7 +11,7 @@
 //
 //   fast path: blocked Gauss-Jordan without row exchanges, 32 columns per block K = [k0, k0+nb),
 //     one launch per block over 64 x 64 output tiles.  Every workgroup inverts the diagonal
-//     block D = A[K,K] (four waves, two pivots per barrier), forms the transform rows
+//     block D = A[K,K] (four waves, four pivots per barrier), forms the transform rows
 //     E[i,:] = -A[i,K] D^{-1} (i not in K) or D^{-1} rows (i in K) of its rows, and writes its
 //     tile of A' = [i not in K] A + E A[K,:] (columns inside K: E itself).  The LU multipliers
 //     of D and of the rows below K are checked |l| <= 1
@@ -44,9 +44,8 @@ __device__ inline double rcp_nr(double x)
 // and every other row r -= a_r,[g..g+3] nr.  Wave w holds rows 8w..8w+7 in a[], lane c column c
 // (lanes 32..63 end with D^{-1}); one barrier per step publishes the four pivot rows and the four
 // pivot columns.  P's factorisation is formed by every thread (the same values everywhere, so no
-// second barrier is needed to share it); thread 0 keeps U_P per step in LDS (reciprocal pivots on
-// the diagonal) and the pivots U_gg, which are written out after the loop.  After the loop the LU multipliers of D are formed
-// from the kept pivot columns: row r's multipliers of step s are
+// second barrier is needed to share it).  After the loop the pivots U_gg and the LU multipliers of
+// D are formed from the kept pivot columns: row r's multipliers of step s are
 // l[r][g+i] = (a_r,g+i - sum_{q<i} l[r][g+q] U_P[q][i]) / U_P[i][i]  (they only feed the
 // |l| <= 1 check).  Writes dinv_out[r][c] = D^{-1}[r][c] and lm_out[m][j] = L[m][j] (unit lower
 // triangular, zero elsewhere, NB x NB row-major), the pivots of rows kb.. to diag/piv, and raises
@@ -55,8 +54,6 @@ struct GjInvLds {
 	__align__(16) double prow[2][4][2 * GJ_NB];     // pivot rows of the step, by step parity
 	__align__(16) double pcs[GJ_NB / 4][4][GJ_NB + 2];  // pivot columns of every step (padded: the
 	                                                // multiplier pass reads 8 steps at once)
-	double ublk[GJ_NB / 4][4][4];                   // U_P of every step, reciprocal pivots on the diagonal
-	double pvs[GJ_NB];                              // the pivots U_gg
 };
 
 // L and U of the 4 x 4 pivot block of step s (P[m][i] = row g+m, column g+i) without exchanges:
@@ -115,15 +112,8 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 		__syncthreads();
 		double lo[4][4], up[4][4], rp[4], pv[4];
 		gj_lu4(sh, s, lo, up, rp, pv);
-		// (LDS only inside the loop: a global store here would make the next barrier wait for it)
-		if(tid == 0){
-#pragma unroll
-			for(int k = 0; k < 4; k++){
-#pragma unroll
-				for(int i = 0; i < 4; i++) sh.ublk[s][k][i] = i == k ? rp[k] : (i > k ? up[k][i] : 0.0);
-				sh.pvs[g + k] = pv[k];
-			}
-		}
+		// (no store of U_P or the pivots here: a store inside the loop delays the next barrier; they
+		// are formed again from the kept pivot columns after the loop)
 		// new pivot rows of this column: y = L^{-1} u, nr = U^{-1} y
 		double u[4];
 #pragma unroll
@@ -161,25 +151,28 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 			a[q] = r == g ? nr[0] : (r == g + 1 ? nr[1] : (r == g + 2 ? nr[2] : (r == g + 3 ? nr[3] : x)));
 		}
 	}
-	__syncthreads();        // U_P and the pivots of the last step (before any global store: the
-	                        // barrier would wait for them)
 	if(lane >= NB){
 #pragma unroll
 		for(int q = 0; q < 8; q++) dinv_out[(8 * wave + q) * NB + lane - NB] = a[q];
 	}
-	if(tid < (int)nbe){
-		diag[kb + tid] = sh.pvs[tid];
-		piv[kb + tid] = (int)(kb + tid);
-	}
-	// the multipliers from the pivot columns and the steps' U_P
+	// the pivots and the multipliers: thread column j refactors the pivot block of its step s
+	// (the same arithmetic as inside the loop, so the same values) and takes row j & 3 of U_P
 	bool bad = false;
 	{
 		const int j = tid % NB, s = j >> 2, ii = j & 3;
-		double my_u[4][4];
+		double my_u[4][4], rp[4], pv[4];
+		{
+			double lo[4][4], up[4][4];
+			gj_lu4(sh, s, lo, up, rp, pv);
 #pragma unroll
-		for(int q = 0; q < 4; q++)
+			for(int q = 0; q < 4; q++)
 #pragma unroll
-			for(int i = 0; i < 4; i++) my_u[q][i] = sh.ublk[s][q][i];
+				for(int i = 0; i < 4; i++) my_u[q][i] = i == q ? rp[q] : (i > q ? up[q][i] : 0.0);
+		}
+		if(tid < (int)nbe){
+			diag[kb + tid] = ii == 0 ? pv[0] : (ii == 1 ? pv[1] : (ii == 2 ? pv[2] : pv[3]));
+			piv[kb + tid] = (int)(kb + tid);
+		}
 #pragma unroll
 		for(int i = 0; i < NB * NB / 256; i++){
 			const int e = tid + 256 * i, r = e / NB;
