@@ -260,6 +260,54 @@ def cpu_baseline(threads, x_state, gpu):
     return base, c1, checks
 
 
+def c2_stab_leg(E, ndraw=10, ntherm=5):
+    """16x16, Nt=64, Nmd=10 (the stabilised configuration): FP64, mixed, and mixed with the slice
+    product stabilised by an FP64 QR every 8 slices (ephmc_config.stabilize = 8).  From a field
+    after ntherm FP64 trajectories (seed 2): the MD force of each mixed path against FP64, and the
+    dH of ndraw trajectories integrated from the same (x, p), each evaluated with the FP64 H:
+    |dH_path - dH_64|, mean and max; the time of one MD force of each path."""
+    l, nt = 16, 64
+    t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, nt)
+    mk = lambda prec, **kw: E.Context(l, l, nt, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=prec,
+                                      checkpoint=K_CKPT, **kw)
+    c64, c32, cst = mk(E.FP64), mk(E.MIXED), mk(E.MIXED, stabilize=8)
+    rng = np.random.default_rng(2)
+    dim = nt * 3 * l * l
+    c64.chain_init(rng.standard_normal(dim))
+    for _ in range(ntherm):
+        c64.trajectory(rng.standard_normal(dim), float(rng.random()))
+    x = c64.chain_x()
+    out = {"config": "16x16, Nt=64, Nmd=10, seed 2, %d FP64 trajectories of thermalisation" % ntherm}
+    F = {}
+    for name, c in (("fp64", c64), ("mixed", c32), ("stabilised", cst)):
+        c.md_force(x)
+        c.sync()
+        t0 = time.perf_counter()
+        F[name] = c.md_force(x)
+        out["md_force_ms_" + name] = 1e3 * (time.perf_counter() - t0)
+    for name in ("mixed", "stabilised"):
+        out["force_rel_err_" + name] = float(np.linalg.norm(F[name] - F["fp64"]) / np.linalg.norm(F["fp64"]))
+
+    def H(xx, p):
+        return c64.hamilton(xx, c64.logdet_inv(c64.M_hat(xx))[0], c64.energy_momenta(p))
+
+    drift = {"mixed": [], "stabilised": []}
+    for _ in range(ndraw):
+        p0 = c64.sample_fourier_momenta(rng.standard_normal(dim))
+        H0 = H(x, p0)
+        x64, p64 = c64.leap_frog(x, p0)
+        d64 = H(x64, p64) - H0
+        for name, c in (("mixed", c32), ("stabilised", cst)):
+            x1, p1 = c.leap_frog(x, p0)
+            drift[name].append(abs(H(x1, p1) - H0 - d64))
+    for name, v in drift.items():
+        out["dH_drift_" + name] = {"mean": float(np.mean(v)), "max": float(np.max(v))}
+    out["trajectories"] = ndraw
+    for c in (c64, c32, cst):
+        c.close()
+    return out
+
+
 def c1_gpu(E, ntraj=100):
     """The 6x6, Nt=32 FP64 chain (Nmd=10) on the GPU, seed-1 normals from numpy, graph replay."""
     t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, 32)
@@ -389,6 +437,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tl", action="store_true", help="skip the transient-localisation leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-chain leg")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 16x16 stabilised-product leg")
     ap.add_argument("--no-k16", action="store_true", help="skip the chains with partial products every 16 slices")
     ap.add_argument("--batch-traj", type=int, default=3)
     ap.add_argument("--tl-samples", type=int, default=1024)
@@ -499,6 +548,7 @@ def main():
         "observables": {k[4:]: v for k, v in acc.items() if k.startswith("obs_")},
         "same_inputs": {"decisions_equal": acc["same_inputs_decisions_equal"],
                         "max_logdet_diff": acc["same_inputs_max_logdet_diff"]},
+
         "kernel_ms": {p: {g: round(v[0], 4) for g, v in res[p]["groups"].items()} for p in res if res[p]["groups"]},
         "cc_correlator": corr,
         "roofline": roof,
@@ -520,6 +570,9 @@ def main():
     if not args.no_batch:
         out["chains64"] = c4_leg(E, args.batch_traj)
         log("chains64", out["chains64"])
+    if not args.no_c2:
+        out["c2_stabilised"] = c2_stab_leg(E)
+        log("c2 stabilised", out["c2_stabilised"])
     if not args.no_tl:
         out["tl"] = tl_leg(E, args.tl_samples)
         log("tl", out["tl"])
