@@ -23,8 +23,13 @@
 //                   pass with a register window of G+1 columns (a wavefront: sweep g runs one
 //                   position behind sweep g-1), so Z is read and written once per G sweeps;
 //                   columns are loaded several steps ahead
-//   tl_back         V = H_0 H_1 ... H_{N-3} Z: a wave holds CW columns of Z in registers and
-//                   applies every reflector (last first)
+//   tl_back         V = H_0 H_1 ... H_{N-3} Z in compact-WY blocks of 32 reflectors (N <= 1024):
+//                   tl_yb (each block's reflectors as a row-major N x 32 panel) and tl_wy_t (the
+//                   blocks' T factors), then
+//                   tl_wy_apply, a workgroup per 16 columns of Z held in matrix-core accumulators
+//                   for the whole sweep (Z <- Z - Y (T (Y^T Z)) per block, last block first);
+//                   larger N: a wave holds CW columns of Z in registers and applies every
+//                   reflector (last first)
 //   tl_sort         eigenvalue ranks, e^{-E}, the partition sum
 //   tl_current      V sorted by eigenvalue and W_c = G_c V (gather form of the bond currents)
 //   tl_pairs        64 x 64 tiles of J_c = V^T W_c (FP64 matrix cores, v_mfma_f64_16x16x4)
@@ -509,6 +514,188 @@ __global__ __launch_bounds__(256) void tl_back_kernel(const double *__restrict__
 		}
 }
 
+// ---- back-transformation in compact-WY blocks on the FP64 matrix cores ---------------------
+// The reflectors j = 32 b .. 32 b + 31 (of the N - 2) form H_{32b} ... H_{32b+31} = 1 - Y T Y^T,
+// Y = [v_j] (N x 32, v_j zero above row j + 1), T upper triangular (T_ii = tau_i,
+// T[0:i, i] = -tau_i T[0:i, 0:i] Y[:, 0:i]^T v_i).  V = Q Z is then Z <- Z - Y (T (Y^T Z)) for
+// b = last .. 0.
+
+constexpr int TL_WB = 32;        // reflectors per block
+constexpr int TL_WC = 16;        // columns of Z per workgroup
+constexpr int TL_WW = 8;         // waves per workgroup
+constexpr int TL_WRB = 8;        // 16-row blocks per wave (8 waves: N <= 1024)
+
+// Yb[b][r][k] = v_{32 b + k}[r]: the reflectors of each block as a row-major panel of n16 =
+// 16 ceil(N / 16) rows x 32 (zero outside their support and in the padding rows), so that a row's
+// 32 entries are contiguous; grid (row tiles of 32, blocks, samples).
+__global__ __launch_bounds__(256) void tl_yb_kernel(const double *__restrict__ Aall, unsigned n, unsigned nblk,
+		double *__restrict__ Yball)
+{
+	__shared__ double tile[32][33];
+	const unsigned s = blockIdx.z, r0 = blockIdx.x * 32, b = blockIdx.y, j0 = b * TL_WB;
+	const double *A = Aall + (size_t)s * n * n;
+	const unsigned n16 = (n + 15) & ~15u;
+	double *Yb = Yball + ((size_t)s * nblk + b) * n16 * TL_WB;
+	const unsigned tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+	for(unsigned k = ty; k < 32; k += 8){
+		const unsigned j = j0 + k, r = r0 + tx;          // column j of A, row r
+		tile[k][tx] = (j + 3 <= n && r >= j + 1 && r < n) ? A[(size_t)j * n + r] : 0.0;
+	}
+	__syncthreads();
+	for(unsigned k = ty; k < 32; k += 8){
+		const unsigned r = r0 + k;
+		if(r < n16) Yb[(size_t)r * TL_WB + tx] = tile[tx][k];
+	}
+}
+
+// T of every block of every sample: grid (blocks, samples).  The Gram entries v_k^T v_i (k < i)
+// are summed over 64-row chunks staged in LDS, then the triangular recurrence runs column by column.
+__global__ __launch_bounds__(256) void tl_wy_t_kernel(const double *__restrict__ Aall, const double *__restrict__ tall,
+		unsigned n, double *__restrict__ Tall)
+{
+	__shared__ double ych[64][TL_WB + 1];
+	__shared__ double G[TL_WB][TL_WB + 1];
+	__shared__ double T[TL_WB][TL_WB + 1];
+	const unsigned s = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+	const unsigned jb = b * TL_WB, nref = n - 2;
+	const double *A = Aall + (size_t)s * n * n, *tau = tall + (size_t)s * n;
+	// pairs (k, i), k < i: p = tid and tid + 256 of the 496
+	int pk[2], pi[2];
+	for(int h = 0; h < 2; h++){
+		int p = (int)tid + 256 * h, i = 1;
+		while(p >= i && i < TL_WB){ p -= i; i++; }
+		pk[h] = i < TL_WB ? p : -1;
+		pi[h] = i;
+	}
+	double g[2] = {0, 0};
+	for(unsigned r0 = jb + 1; r0 < n; r0 += 64){
+		for(unsigned e = tid; e < 64 * TL_WB; e += 256){
+			const unsigned k = e / 64, rr = e % 64, j = jb + k, r = r0 + rr;
+			ych[rr][k] = (j < nref && r >= j + 1 && r < n) ? A[(size_t)j * n + r] : 0.0;
+		}
+		__syncthreads();
+		for(int h = 0; h < 2; h++)
+			if(pk[h] >= 0)
+				for(int rr = 0; rr < 64; rr++) g[h] = fma(ych[rr][pk[h]], ych[rr][pi[h]], g[h]);
+		__syncthreads();
+	}
+	for(int h = 0; h < 2; h++) if(pk[h] >= 0) G[pk[h]][pi[h]] = g[h];
+	for(unsigned e = tid; e < TL_WB * TL_WB; e += 256) T[e / TL_WB][e % TL_WB] = 0.0;
+	__syncthreads();
+	if(tid == 0) T[0][0] = jb < nref ? tau[jb] : 0.0;
+	__syncthreads();
+	for(int i = 1; i < TL_WB; i++){
+		const unsigned j = jb + i;
+		const double ti = j < nref ? tau[j] : 0.0;
+		double x = 0;
+		if((int)tid < i){
+			for(int m = (int)tid; m < i; m++) x = fma(T[tid][m], G[m][i], x);
+		}
+		__syncthreads();
+		if((int)tid < i) T[tid][i] = -ti * x;
+		if(tid == 0) T[i][i] = ti;
+		__syncthreads();
+	}
+	double *To = Tall + ((size_t)s * gridDim.x + b) * TL_WB * TL_WB;
+	for(unsigned e = tid; e < TL_WB * TL_WB; e += 256) To[e] = T[e / TL_WB][e % TL_WB];
+}
+
+// Z <- Q Z for 16 columns per workgroup (grid (N / 16, samples)), N <= 1024.  Wave w of the 8 holds
+// rows 128 w .. 128 w + 127 of its columns in the matrix-core accumulator layout (16 x 16 blocks: lane
+// l, element e = row 4 e + l / 16, column l % 16) for the whole sweep over the blocks, so Z is read
+// and written once.  Per block: W^T = Z^T Y (each wave over its rows; the accumulator elements
+// serve directly as the A operand, rows 16 q + 4 e + l / 16 of step (q, e)), the eight waves'
+// parts summed in LDS, W2^T = W^T T^T, then Z -= Y W2 (v_mfma_f64_16x16x4 both ways).
+__global__ __launch_bounds__(64 * TL_WW) void tl_wy_apply_kernel(const double *__restrict__ Yball, const double *__restrict__ Tall,
+		unsigned n, unsigned nblk, double *__restrict__ Zall)
+{
+	typedef double v4d __attribute__((ext_vector_type(4)));
+	__shared__ double Ts[TL_WB][TL_WB + 1];
+	__shared__ double wpart[TL_WW][2][64][4];
+	__shared__ double Wt[TL_WC][TL_WB + 1];
+	__shared__ double W2t[TL_WC][TL_WB + 1];
+	const unsigned s = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+	const unsigned g = lane >> 4, m = lane & 15, c0 = blockIdx.x * TL_WC, n16 = (n + 15) & ~15u;
+	const double *Tb = Tall + (size_t)s * nblk * TL_WB * TL_WB;
+	double *Z = Zall + (size_t)s * n * n;
+	const unsigned rw0 = wave * 16 * TL_WRB;                      // first row of this wave
+	const unsigned col = c0 + m;
+	v4d acc[TL_WRB];
+#pragma unroll
+	for(int q = 0; q < TL_WRB; q++)
+#pragma unroll
+		for(int e = 0; e < 4; e++){
+			const unsigned r = rw0 + 16 * q + 4 * e + g;
+			acc[q][e] = (r < n && col < n) ? Z[(size_t)col * n + r] : 0.0;
+		}
+	for(int b = (int)nblk - 1; b >= 0; b--){
+		const unsigned jb = (unsigned)b * TL_WB;
+		for(unsigned e = tid; e < TL_WB * TL_WB; e += 64 * TL_WW) Ts[e / TL_WB][e % TL_WB] = Tb[(size_t)b * TL_WB * TL_WB + e];
+		// this block's reflector panel: row r at Yb + r * 32 (rows of one lane at fixed offsets)
+		const double *Yb = Yball + ((size_t)s * nblk + b) * n16 * TL_WB;
+		const double *y1 = Yb + (size_t)(rw0 + g) * TL_WB + m;          // phase 1: row rw0 + 16 q + 4 e + g, column 16 h + m
+		const double *y3 = Yb + (size_t)(rw0 + m) * TL_WB + g;          // phase 3: row rw0 + 16 q + m, column 4 t + g
+		// W^T partial over this wave's rows r >= jb + 1 (Y is zero above)
+		v4d wt[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+		for(int q = 0; q < TL_WRB; q++){
+			const unsigned rb = rw0 + 16 * q;
+			if(rb + 16 <= jb + 1 || rb >= n) continue;           // uniform per wave
+#pragma unroll
+			for(int e = 0; e < 4; e++){
+				const double a = acc[q][e];
+#pragma unroll
+				for(int h = 0; h < 2; h++){
+					const double y = y1[(16 * q + 4 * e) * TL_WB + 16 * h]; // zero past row n
+					wt[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, y, wt[h], 0, 0, 0);
+				}
+			}
+		}
+#pragma unroll
+		for(int h = 0; h < 2; h++)
+#pragma unroll
+			for(int e = 0; e < 4; e++) wpart[wave][h][lane][e] = wt[h][e];
+		__syncthreads();
+		// W^T[c][k]: output element e of lane l of block h is row (column of Z) 4 e + l / 16, column
+		// (reflector) 16 h + l % 16
+		for(unsigned e2 = tid; e2 < TL_WC * TL_WB; e2 += 64 * TL_WW){
+			const unsigned c = e2 / TL_WB, k = e2 % TL_WB, h = k >> 4, l = ((c & 3) << 4) | (k & 15), e = c >> 2;
+			double x = wpart[0][h][l][e];
+#pragma unroll
+			for(int ww = 1; ww < TL_WW; ww++) x += wpart[ww][h][l][e];
+			Wt[c][k] = x;
+		}
+		__syncthreads();
+		// W2^T[c][k] = sum_{mm >= k} W^T[c][mm] T[k][mm]; stored negated for the update
+		for(unsigned e2 = tid; e2 < TL_WC * TL_WB; e2 += 64 * TL_WW){
+			const unsigned c = e2 / TL_WB, k = e2 % TL_WB;
+			double x = 0;
+			for(unsigned mm = k; mm < TL_WB; mm++) x = fma(Wt[c][mm], Ts[k][mm], x);
+			W2t[c][k] = -x;
+		}
+		__syncthreads();
+		// Z[rows][cols] -= Y[rows][k] W2[k][cols]: A = Y (row 16 q + l % 16, k 4 t + l / 16), B = -W2
+		// (k 4 t + l / 16, column l % 16)
+#pragma unroll
+		for(int q = 0; q < TL_WRB; q++){
+			const unsigned rb = rw0 + 16 * q;
+			if(rb + 16 <= jb + 1 || rb >= n) continue;
+#pragma unroll
+			for(int t = 0; t < TL_WB / 4; t++){
+				const double y = y3[16 * q * TL_WB + 4 * t];
+				acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(y, W2t[m][4 * t + g], acc[q], 0, 0, 0);
+			}
+		}
+	}
+#pragma unroll
+	for(int q = 0; q < TL_WRB; q++)
+#pragma unroll
+		for(int e = 0; e < 4; e++){
+			const unsigned r = rw0 + 16 * q + 4 * e + g;
+			if(r < n && col < n) Z[(size_t)col * n + r] = acc[q][e];
+		}
+}
+
 // ---- order, Boltzmann factors, partition sum -------------------------------------------------
 template <int BT>
 __global__ __launch_bounds__(BT) void tl_sort_kernel(const double *__restrict__ evall, unsigned n, double *__restrict__ evs_all,
@@ -857,9 +1044,23 @@ extern "C" int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamp
 		EPHMC_LAUNCHED();
 		EPHMC_TRY(timer_end(c, "tl_ql"));
 		EPHMC_TRY(timer_begin(c, "tl_back"));
-		if(n <= 1024) tl_back_kernel<16, 4><<<dim3((n + 15) / 16, nb_), 256, 0, c->stream>>>(w.A, w.tau, w.Z, n);
-		else tl_back_kernel<32, 2><<<dim3((n + 7) / 8, nb_), 256, 0, c->stream>>>(w.A, w.tau, w.Z, n);
-		EPHMC_LAUNCHED();
+		{
+			// compact-WY blocks on the matrix cores (N <= 1024), the reflector copy and the T factors in
+			// the rotation buffer, which is free once the rotations are applied; else one reflector at a time
+			const unsigned nblkw = n > 2 ? (n - 2 + TL_WB - 1) / TL_WB : 0;
+			const size_t ybsz = (size_t)nblkw * ((n + 15) & ~15u) * TL_WB;
+			double *yball = (double *)(void *)w.rot, *tall = yball + (size_t)nb_ * ybsz;
+			const bool wy = n >= 3 && n <= TL_WW * 16 * TL_WRB && ybsz + (size_t)nblkw * TL_WB * TL_WB <= 2 * w.rot_cap;
+			if(wy){
+				tl_yb_kernel<<<dim3((n + 31) / 32, nblkw, nb_), 256, 0, c->stream>>>(w.A, n, nblkw, yball);
+				EPHMC_LAUNCHED();
+				tl_wy_t_kernel<<<dim3(nblkw, nb_), 256, 0, c->stream>>>(w.A, w.tau, n, tall);
+				EPHMC_LAUNCHED();
+				tl_wy_apply_kernel<<<dim3((n + TL_WC - 1) / TL_WC, nb_), 64 * TL_WW, 0, c->stream>>>(yball, tall, n, nblkw, w.Z);
+			}else if(n <= 1024) tl_back_kernel<16, 4><<<dim3((n + 15) / 16, nb_), 256, 0, c->stream>>>(w.A, w.tau, w.Z, n);
+			else tl_back_kernel<32, 2><<<dim3((n + 7) / 8, nb_), 256, 0, c->stream>>>(w.A, w.tau, w.Z, n);
+			EPHMC_LAUNCHED();
+		}
 		EPHMC_TRY(timer_end(c, "tl_back"));
 		tl_sort_kernel<TL_BT><<<nb_, TL_BT, ((size_t)n + TL_BT / 64) * 8, c->stream>>>(w.ev, n, w.evs, w.ems, w.perm, w.res);
 		EPHMC_LAUNCHED();

@@ -56,8 +56,12 @@ def draws(om, nsamples, seed):
     return np.random.default_rng(seed).standard_normal((nsamples, 3 * om.ns))
 
 
-@pytest.mark.parametrize("l1,l2,ns_", [(1, 1, 3), (1, 2, 3), (2, 1, 3), (2, 2, 4), (3, 4, 5), (4, 4, 5), (6, 6, 5), (5, 7, 4)])
+@pytest.mark.parametrize("l1,l2,ns_", [(1, 1, 3), (1, 2, 3), (2, 1, 3), (2, 2, 4), (3, 4, 5), (4, 4, 5), (6, 6, 5), (5, 7, 4),
+                                       (8, 8, 3), (9, 7, 3), (10, 11, 2)])
 def test_tl_small_lattices_vs_oracle(l1, l2, ns_):
+    """N = 1 .. 110 against the oracle.  From N = 63 on (here 63, 64 and 110: a ragged last row
+    block, a full one, four reflector blocks) the back-transformation runs in compact-WY blocks on
+    the matrix cores; below, one reflector at a time."""
     nbins = 32
     say("tl", l1, l2, ns_)
     om, width = tl_model(l1, l2, nbins)
@@ -85,7 +89,8 @@ def test_tl_many_batches_vs_oracle():
 
 @pytest.mark.parametrize("l1,l2,nsamples", [(16, 16, 3), (32, 32, 2), (32, 40, 1)])
 def test_tl_large_vs_numpy(l1, l2, nsamples):
-    """N = 256, 1024 (the 32x32 size) and 1280 (two rows per thread) against LAPACK eigh."""
+    """N = 256, 1024 (the 32x32 size; both with the compact-WY back-transformation) and 1280 (two
+    rows per thread, reflectors one at a time) against LAPACK eigh."""
     nbins = 512
     om, width = tl_model(l1, l2, nbins)
     xs = draws(om, nsamples, 7)
