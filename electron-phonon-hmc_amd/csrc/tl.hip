@@ -12,10 +12,13 @@
 //
 // Kernels, a batch of B samples at a time (one workgroup per sample unless stated):
 //   tl_build        dense H (column c contiguous; H is symmetric)
-//   tl_tridiag      Householder reduction to tridiagonal form (lower, one column at a time) on
-//                   the full symmetric matrix; the rank-2 update of step j is fused with the
-//                   matrix-vector product of step j+1, so the trailing block is read and written
-//                   once per step.  Reflectors v_j overwrite column j below the diagonal.
+//   tl_tridiag      Householder reduction to tridiagonal form (lower) on the full symmetric
+//                   matrix, in panels of 8 reflectors (N <= 1024): each step reads the trailing
+//                   block once (its matrix-vector product, corrected by the panel's V and W) and
+//                   one rank-16 update per panel writes it.  Larger N: one column at a time, the
+//                   rank-2 update of step j fused with the matrix-vector product of step j+1
+//                   (read and written once per step).  Reflectors v_j overwrite column j below
+//                   the diagonal.
 //   tl_ql_scalar    implicit QL with Wilkinson shifts on (d, e) (the scalar recurrence of the
 //                   EISPACK tql2 method, one wave per sample computing it in every lane),
 //                   recording every rotation (c, s) and the position range of every sweep
@@ -251,6 +254,197 @@ __global__ __launch_bounds__(BT) void tl_tridiag_kernel(double *__restrict__ Aal
 		d[a] = A[(size_t)a * n + a] - 2 * v[a] * w[a];
 		e[a] = A[(size_t)a * n + b] - (v[b] * w[a] + w[b] * v[a]);
 		d[b] = A[(size_t)b * n + b] - 2 * v[b] * w[b];
+		e[b] = 0;
+		tau[a] = 0;
+		tau[b] = 0;
+	}
+}
+
+// Panel form of the same reduction (the scheme of LAPACK's dlatrd): for the reflectors
+// j = j0 .. j0 + NB - 1 the trailing matrix is left as it was at the start of the panel, and the
+// updates of the panel's earlier steps enter through their (v_i, w_i):
+//     column j:   a = A[:, j] - sum_i (v_i w_i[j] + w_i v_i[j]),
+//     product:    A' v = A v - V (W^T v) - W (V^T v),     w = tau A' v - (tau^2 / 2)(v^T A' v) v,
+// so each step reads the trailing block once and writes nothing; one update
+// A -= V W^T + W V^T per panel writes it.  Thread r keeps row r of the panel's V and W in registers
+// (v_i[r], w_i[r]); row j of them goes through LDS when column j needs it, and 64 rows at a time
+// when the panel's update needs them.  Reflectors, d, e and tau come out as from tl_tridiag_kernel.
+constexpr int TL_PNB = 8;        // reflectors per panel
+
+template <int BT, int RPT, int NB>
+__global__ __launch_bounds__(BT) void tl_tridiag_panel_kernel(double *__restrict__ Aall, unsigned n, double *__restrict__ dall,
+		double *__restrict__ eall, double *__restrict__ tall)
+{
+	extern __shared__ double sh[];
+	constexpr int NW = BT / 64;
+	double *vb = sh;                         // n: column j, then v_j
+	double *red = vb + n;                    // NW
+	double *rowvw = red + NW;                // 2 NB: v_i[j], w_i[j]
+	double *dots = rowvw + 2 * NB;           // NW x 2 NB wave partials
+	double *dsum = dots + NW * 2 * NB;       // 2 NB: W^T v, V^T v
+	double *chunk = dsum + 2 * NB;           // 64 x 2 NB: rows of V and W for the panel update
+	const unsigned s = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	double *A = Aall + (size_t)s * n * n, *d = dall + (size_t)s * n, *e = eall + (size_t)s * n, *tau = tall + (size_t)s * n;
+	if(n <= 2){
+		if(tid == 0){
+			d[0] = A[0];
+			e[0] = 0;
+			tau[0] = 0;
+			if(n == 2){
+				d[1] = A[3];
+				e[0] = A[1];
+				e[1] = 0;
+				tau[1] = 0;
+			}
+		}
+		return;
+	}
+	double vrow[RPT][NB], wrow[RPT][NB];
+	for(unsigned j0 = 0; j0 + 2 < n; j0 += NB){
+		const unsigned kmax = min((unsigned)NB, n - 2 - j0), jn = j0 + kmax;
+#pragma unroll
+		for(int q = 0; q < RPT; q++)
+#pragma unroll
+			for(int i = 0; i < NB; i++){ vrow[q][i] = 0; wrow[q][i] = 0; }
+		for(unsigned k = 0; k < kmax; k++){
+			const unsigned j = j0 + k;
+			// row j of V and W
+#pragma unroll
+			for(int q = 0; q < RPT; q++)
+				if(tid + q * BT == j){
+#pragma unroll
+					for(int i = 0; i < NB; i++){ rowvw[i] = vrow[q][i]; rowvw[NB + i] = wrow[q][i]; }
+				}
+			__syncthreads();
+			// column j of the updated matrix, rows j .. n-1
+#pragma unroll
+			for(int q = 0; q < RPT; q++){
+				const unsigned r = tid + q * BT;
+				if(r >= j && r < n){
+					double a = A[(size_t)j * n + r];
+#pragma unroll
+					for(int i = 0; i < NB; i++) a -= vrow[q][i] * rowvw[NB + i] + wrow[q][i] * rowvw[i];
+					if(r == j) d[j] = a;
+					else vb[r] = a;
+				}
+			}
+			__syncthreads();
+			const double tj = tl_reflector<BT, RPT>(A, n, j, vb, red, e, tau);
+			// A v over the panel-start trailing block, and the panel's W^T v, V^T v
+			double acc[RPT], pw[NB], pv[NB];
+#pragma unroll
+			for(int i = 0; i < NB; i++){ pw[i] = 0; pv[i] = 0; }
+#pragma unroll
+			for(int q = 0; q < RPT; q++){
+				const unsigned r = tid + q * BT;
+				acc[q] = 0;
+				if(r >= j + 1 && r < n){
+					const double vr = vb[r];
+					double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+					unsigned c = j + 1;
+					for(; c + 4 <= n; c += 4){
+						const size_t o = (size_t)c * n + r;
+						a0 += A[o] * vb[c];
+						a1 += A[o + n] * vb[c + 1];
+						a2 += A[o + 2 * (size_t)n] * vb[c + 2];
+						a3 += A[o + 3 * (size_t)n] * vb[c + 3];
+					}
+					for(; c < n; c++) a0 += A[(size_t)c * n + r] * vb[c];
+					acc[q] = (a0 + a1) + (a2 + a3);
+#pragma unroll
+					for(int i = 0; i < NB; i++){
+						pw[i] = fma(wrow[q][i], vr, pw[i]);
+						pv[i] = fma(vrow[q][i], vr, pv[i]);
+					}
+				}
+			}
+			if(k > 0){
+				// block sums of the 2k dot products (fixed order: waves, then their partials in order)
+#pragma unroll
+				for(int i = 0; i < NB; i++)
+					if(i < (int)k){
+						pw[i] = wave_sum(pw[i]);
+						pv[i] = wave_sum(pv[i]);
+					}
+				if(lane == 0)
+#pragma unroll
+					for(int i = 0; i < NB; i++)
+						if(i < (int)k){
+							dots[wave * 2 * NB + i] = pw[i];
+							dots[wave * 2 * NB + NB + i] = pv[i];
+						}
+				__syncthreads();
+				if(tid < 2 * NB){
+					double t = 0;
+					for(int w = 0; w < NW; w++) t += dots[w * 2 * NB + tid];
+					dsum[tid] = t;
+				}
+				__syncthreads();
+			}
+			// p = tau (A v - V (W^T v) - W (V^T v)), kk = tau / 2 p.v, w = p - kk v
+			double p[RPT], kk = 0;
+#pragma unroll
+			for(int q = 0; q < RPT; q++){
+				const unsigned r = tid + q * BT;
+				p[q] = 0;
+				if(r >= j + 1 && r < n){
+					double x = acc[q];
+#pragma unroll
+					for(int i = 0; i < NB; i++)
+						if(i < (int)k) x -= vrow[q][i] * dsum[i] + wrow[q][i] * dsum[NB + i];
+					p[q] = tj * x;
+					kk += p[q] * vb[r];
+				}
+			}
+			kk = 0.5 * tj * block_sum<BT>(kk, red);
+#pragma unroll
+			for(int q = 0; q < RPT; q++){
+				const unsigned r = tid + q * BT;
+				const bool in = r >= j + 1 && r < n;
+				const double vr = in ? vb[r] : 0.0, wr = in ? p[q] - kk * vr : 0.0;
+#pragma unroll
+				for(int i = 0; i < NB; i++)
+					if(i == (int)k){ vrow[q][i] = vr; wrow[q][i] = wr; }
+			}
+			__syncthreads();          // vb is rewritten by the next column
+		}
+		// the panel's update of the trailing block, rows and columns jn .. n-1
+		for(unsigned c0 = jn; c0 < n; c0 += 64){
+#pragma unroll
+			for(int q = 0; q < RPT; q++){
+				const unsigned r = tid + q * BT;
+				if(r >= c0 && r < c0 + 64 && r < n){
+#pragma unroll
+					for(int i = 0; i < NB; i++){
+						chunk[(r - c0) * 2 * NB + i] = vrow[q][i];
+						chunk[(r - c0) * 2 * NB + NB + i] = wrow[q][i];
+					}
+				}
+			}
+			__syncthreads();
+			const unsigned cn = min(64u, n - c0);
+#pragma unroll
+			for(int q = 0; q < RPT; q++){
+				const unsigned r = tid + q * BT;
+				if(r >= jn && r < n){
+					for(unsigned cc = 0; cc < cn; cc++){
+						const double *cv = chunk + cc * 2 * NB;
+						double x = 0;
+#pragma unroll
+						for(int i = 0; i < NB; i++) x += vrow[q][i] * cv[NB + i] + wrow[q][i] * cv[i];
+						A[(size_t)(c0 + cc) * n + r] -= x;
+					}
+				}
+			}
+			__syncthreads();
+		}
+	}
+	// the last 2 x 2 block, already updated
+	if(tid == 0){
+		const unsigned a = n - 2, b = n - 1;
+		d[a] = A[(size_t)a * n + a];
+		e[a] = A[(size_t)a * n + b];
+		d[b] = A[(size_t)b * n + b];
 		e[b] = 0;
 		tau[a] = 0;
 		tau[b] = 0;
@@ -1030,9 +1224,14 @@ extern "C" int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamp
 		}
 		EPHMC_TRY(timer_begin(c, "tl_tridiag"));
 		{
-			const size_t lds = (3 * (size_t)n + TL_BT / 64) * 8;
-			if(n <= TL_BT) tl_tridiag_kernel<TL_BT, 1><<<nb_, TL_BT, lds, c->stream>>>(w.A, n, w.d, w.e, w.tau);
-			else tl_tridiag_kernel<TL_BT, 2><<<nb_, TL_BT, lds, c->stream>>>(w.A, n, w.d, w.e, w.tau);
+			if(n <= TL_BT){
+				const size_t lds = ((size_t)n + TL_BT / 64 + 2 * TL_PNB + (TL_BT / 64) * 2 * TL_PNB + 2 * TL_PNB + 64 * 2 * TL_PNB) * 8;
+				tl_tridiag_panel_kernel<TL_BT, 1, TL_PNB><<<nb_, TL_BT, lds, c->stream>>>(w.A, n, w.d, w.e, w.tau);
+			}else{
+				const size_t lds = (3 * (size_t)n + TL_BT / 64) * 8;
+				if(n <= TL_BT) tl_tridiag_kernel<TL_BT, 1><<<nb_, TL_BT, lds, c->stream>>>(w.A, n, w.d, w.e, w.tau);
+				else tl_tridiag_kernel<TL_BT, 2><<<nb_, TL_BT, lds, c->stream>>>(w.A, n, w.d, w.e, w.tau);
+			}
 			EPHMC_LAUNCHED();
 		}
 		EPHMC_TRY(timer_end(c, "tl_tridiag"));
