@@ -440,7 +440,7 @@ def main():
     ap.add_argument("--no-c2", action="store_true", help="skip the 16x16 stabilised-product leg")
     ap.add_argument("--no-k16", action="store_true", help="skip the chains with partial products every 16 slices")
     ap.add_argument("--batch-traj", type=int, default=3)
-    ap.add_argument("--tl-samples", type=int, default=1024)
+    ap.add_argument("--tl-samples", type=int, default=4096)
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the chains with the trajectory body replayed as a hipGraph (the headline value "
                          "then comes from the eager chain)")

@@ -44,6 +44,7 @@
 // greens array).
 #include "internal.h"
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 namespace ephmc {
@@ -1201,11 +1202,14 @@ extern "C" int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamp
 	if(n > 2 * TL_BT){ set_error("transient localisation supports N <= %d sites", 2 * TL_BT); return EPHMC_ERR_ARG; }
 	const size_t mat = (size_t)n * n;
 	const unsigned nblk = (n + TL_TILE - 1) / TL_TILE;
-	// samples per batch: about 48 GB of workspace (5.5 N^2 doubles per sample), at most 1024, so
-	// that the one-wave-per-sample QL recurrence has a wave on every SIMD
+	// samples per batch: about 100 GB of workspace (5.5 N^2 doubles per sample; EPHMC_TL_BATCH_MB
+	// sets another budget in MiB), at most 2048, so that the one-wave-per-sample QL recurrence has
+	// two waves on every SIMD
+	size_t mb = (size_t)100 << 10;
+	if(const char *v = getenv("EPHMC_TL_BATCH_MB")) mb = std::max<size_t>(1, strtoull(v, nullptr, 10));
 	const size_t per = (size_t)(5.5 * mat * 8) + (size_t)nblk * nblk * nbins * 8;
-	unsigned B = (unsigned)std::max<size_t>(1, ((size_t)48 << 30) / per);
-	B = std::min(B, 1024u);
+	unsigned B = (unsigned)std::max<size_t>(1, (mb << 20) / per);
+	B = std::min(B, 2048u);
 	B = std::min(B, nsamples);
 	EPHMC_TRY(tl_reserve(c, B, nbins));
 	TlWork &w = c->tl;

@@ -632,9 +632,9 @@ unsigned run_trans_loc(double *greens, unsigned l1, unsigned l2, unsigned nt, do
 
 	const double cm_default[6] = {0, 0, 0, 0, 0, 0};
 	struct organic_engine *e = engine_get(l1, l2, 1, t, kappa, mu, 1, 1, 1, 0, NULL);
-	/* samples are sent in batches of 1024 (the engine runs one wave per sample through the QL
-	 * recurrence, so a batch fills the GPU); each batch's normals are drawn in sample order */
-	const unsigned batch = 1024;
+	/* samples are sent in batches of 2048 (the engine runs one wave per sample through the QL
+	 * recurrence, so a batch puts two on every SIMD); each batch's normals are drawn in sample order */
+	const unsigned batch = 2048;
 	double *normals = malloc((size_t)batch * dim * sizeof(double));
 	for(unsigned s0 = 0; s0 < meas; s0 += batch){
 		const unsigned nb = meas - s0 < batch ? meas - s0 : batch;

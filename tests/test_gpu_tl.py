@@ -74,11 +74,13 @@ def test_tl_small_lattices_vs_oracle(l1, l2, ns_):
     np.testing.assert_allclose(g, go, rtol=1e-8, atol=1e-12 * np.abs(go).max())
 
 
-def test_tl_many_batches_vs_oracle():
-    """More samples than one batch holds (several batches, fixed-order accumulation)."""
+def test_tl_many_batches_vs_oracle(monkeypatch):
+    """More samples than one batch holds (a 10 MiB workspace budget: batches of 183 samples at 6x6,
+    four batches; fixed-order accumulation)."""
     nbins = 24
     om, width = tl_model(6, 6, nbins)
     xs = draws(om, 600, 5)
+    monkeypatch.setenv("EPHMC_TL_BATCH_MB", "10")
     say("tl 6x6, 600 samples")
     ctx = gpu_context(om)
     g, z = ctx.tl_hist(xs, CM, width, nbins)
