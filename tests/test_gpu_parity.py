@@ -170,6 +170,22 @@ def test_force_w0_order_bit_identical(monkeypatch, precision, l1, l2, nt):
     np.testing.assert_array_equal(F["0"], F["1"])
 
 
+@pytest.mark.parametrize("l1,l2", [(33, 32), (43, 43)])
+def test_force_fp64_default_width_near_lds_limit(l1, l2):
+    """FP64 force with the default column width at N = 1056 and 1849, the sizes where the sweep's
+    reduction scratch decides whether C = 4 (or 2) still fits the 160 KiB of LDS: the default must
+    pick a width that launches, and the force equals the oracle's."""
+    say("force FP64 default width", l1, l2)
+    om = OracleModel(l1, l2, 4, beta=4 * 40 / 32)
+    ctx = gpu_context(om, checkpoint=1)
+    x = field(om, 21)
+    ld, _, Minv = ctx.logdet_inv(ctx.M_hat(x))
+    Fg = ctx.force(x, Minv, ld)
+    Fo = om.force(x, Minv, ld)
+    say("rel err", rel(Fg, Fo))
+    assert rel(Fg, Fo) < 1e-10
+
+
 def test_force_mixed():
     say("force mixed 6x6x32")
     om = OracleModel(6, 6, 32)
