@@ -383,11 +383,10 @@ def tl_leg(E, nsamples):
             "greens_sum": float(g.sum())}
 
 
-def c4_leg(E, ntraj, nchains=64, graph=True):
-    """Independent chains of uneven size (L from {8,12,16,20,24}, Nt from {32,64,128}, drawn with
-    MT19937-64 seed 0; momentum normals from numpy generators seeded 1000+i), mixed precision,
-    each context on its own HIP stream, trajectories issued round-robin without waiting:
-    aggregate trajectories/s."""
+def c4_chains(E, ntraj, nchains, graph=True):
+    """The 64 uneven chains: L from {8,12,16,20,24}, Nt from {32,64,128}, drawn with MT19937-64
+    seed 0; momentum normals and uniforms from numpy generators seeded 1000+i; mixed precision,
+    autotuned unless --no-autotune; initialised."""
     import ctypes as C
     host = C.CDLL(os.path.join(ROOT, "electron-phonon-hmc_amd", "libephmc_host.so"))
     host.init_genrand64.argtypes = [C.c_ulong]
@@ -409,6 +408,13 @@ def c4_leg(E, ntraj, nchains=64, graph=True):
             ctx.autotune()
         ctx.chain_init(rng.standard_normal(dim))
         chains.append((ctx, draws, us, l, nt))
+    return chains
+
+
+def c4_leg(E, ntraj, nchains=64, graph=True):
+    """The 64 uneven chains (c4_chains), each context on its own HIP stream, trajectories issued
+    round-robin without waiting: aggregate trajectories/s."""
+    chains = c4_chains(E, ntraj, nchains, graph)
     for ctx, draws, us, _, _ in chains:              # warm-up (graph capture)
         ctx.trajectory(draws[0], float(us[0]), wait=False)
     for ctx, *_ in chains:
@@ -428,6 +434,26 @@ def c4_leg(E, ntraj, nchains=64, graph=True):
             "sizes": {"%dx%d,Nt=%d" % (l, l, nt): n for (l, nt), n in sorted(sizes.items())}}
 
 
+def c4_batch_leg(E, ntraj, nchains=64):
+    """The same 64 chains and inputs as c4_leg, advanced as one batch (ephmc_batch_*: every kernel
+    of a trajectory launched once over all chains of a group): aggregate trajectories/s."""
+    chains = c4_chains(E, ntraj, nchains)
+    ctxs = [c[0] for c in chains]
+    b = E.Batch(ctxs)
+    info = b.info()
+    b.trajectory([c[1][0] for c in chains], [float(c[2][0]) for c in chains])     # warm-up
+    t0 = time.perf_counter()
+    for k in range(1, ntraj + 1):
+        b.trajectory([c[1][k] for c in chains], [float(c[2][k]) for c in chains], wait=False)
+        b.collect()
+    el = time.perf_counter() - t0
+    b.close()
+    for ctx in ctxs:
+        ctx.close()
+    return {"traj_per_s": nchains * ntraj / el, "chains": nchains, "trajectories_per_chain": ntraj,
+            "seconds": el, "groups": info["groups"], "launches_per_trajectory": info["launches"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -439,7 +465,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-chain leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the 16x16 stabilised-product leg")
     ap.add_argument("--no-k16", action="store_true", help="skip the chains with partial products every 16 slices")
-    ap.add_argument("--batch-traj", type=int, default=3)
+    ap.add_argument("--batch-traj", type=int, default=10)
+    ap.add_argument("--c4-only", action="store_true",
+                    help="run only the 64-chain legs (streams and one batch) and print their JSON")
     ap.add_argument("--tl-samples", type=int, default=4096)
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the chains with the trajectory body replayed as a hipGraph (the headline value "
@@ -462,6 +490,11 @@ def main():
         return
     if args.force_only:
         force_only(args.force_only)
+        return
+    if args.c4_only:
+        import ephmc as E
+        print(json.dumps({"chains64": c4_leg(E, args.batch_traj),
+                          "chains64_batch": c4_batch_leg(E, args.batch_traj)}), flush=True)
         return
     if args.tl_only:
         import ephmc as E
@@ -570,6 +603,8 @@ def main():
     if not args.no_batch:
         out["chains64"] = c4_leg(E, args.batch_traj)
         log("chains64", out["chains64"])
+        out["chains64_batch"] = c4_batch_leg(E, args.batch_traj)
+        log("chains64_batch", out["chains64_batch"])
     if not args.no_c2:
         out["c2_stabilised"] = c2_stab_leg(E)
         log("c2 stabilised", out["c2_stabilised"])

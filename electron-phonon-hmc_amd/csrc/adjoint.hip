@@ -159,7 +159,7 @@ __host__ __device__ constexpr size_t adj_lds_bytes(unsigned ns)
 // DEG is the polynomial degree (a.deg), a template constant so that in the fp32 one-site form the
 // phase structure and the S_a recursion need no run-time tests
 template <typename R, int C, int BT, int S, int DEG>
-__global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
+__device__ __forceinline__ void adjoint_body(const AdjArgs<R> &a, unsigned bx)
 {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	const unsigned ns = a.ns;
@@ -182,13 +182,13 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 	if(a.red_src){
 		const unsigned total = (unsigned)a.red_ntau * 3 * ns;
 		const unsigned per = (total + a.nwg - 1) / a.nwg;
-		reduce_links(a.red_src, a.nwg, 3 * ns, blockIdx.x * per, per, total, a.red_tau_b, a.f, a.fscale, a.pdot, a.p, a.kick,
+		reduce_links(a.red_src, a.nwg, 3 * ns, bx * per, per, total, a.red_tau_b, a.f, a.fscale, a.pdot, a.p, a.kick,
 			rscratch, BT, a.clip);
 	}
 	R *wbs[2] = {wb0, wb1};
 
-	const unsigned col0 = blockIdx.x * C;
-	R *seg = a.seg + (size_t)blockIdx.x * a.seg_stride;
+	const unsigned col0 = bx * C;
+	R *seg = a.seg + (size_t)bx * a.seg_stride;
 	unsigned site[S];
 	bool act[S];
 	int nbr[S][6];
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 	R hb[S][3];
 	int tau_out = -1;
 	auto flush_bonds = [&](){
-		R *out = a.partial + ((size_t)(tau_out - a.tau_b) * a.nwg + blockIdx.x) * 3 * ns;
+		R *out = a.partial + ((size_t)(tau_out - a.tau_b) * a.nwg + bx) * 3 * ns;
 #pragma unroll
 		for(int j = 0; j < S; j++)
 			if(act[j])
@@ -483,6 +483,21 @@ __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 			if(act[j] && colok(c)) a.wbuf[(size_t)(col0 + c) * ns + site[j]] = wv[j][c];
 }
 
+template <typename R, int C, int BT, int S, int DEG>
+__global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
+{
+	adjoint_body<R, C, BT, S, DEG>(a, blockIdx.x);
+}
+
+// many chains in one launch: blockIdx.z = chain (its record in tab), blockIdx.x = its workgroup
+template <typename R, int C, int BT, int S, int DEG>
+__global__ __launch_bounds__(BT) void adjoint_multi_kernel(const AdjArgs<R> *__restrict__ tab)
+{
+	const AdjArgs<R> &a = tab[blockIdx.z];
+	if(blockIdx.x >= a.nwg) return;
+	adjoint_body<R, C, BT, S, DEG>(a, blockIdx.x);
+}
+
 template <typename P>
 __global__ __launch_bounds__(256) void force_reduce_kernel(const P *__restrict__ partial, unsigned nwg, unsigned loc,
 		int tau_b, int ntau, double f0, double f1, double f2, const double *__restrict__ fscale, double *__restrict__ pdot,
@@ -492,6 +507,30 @@ __global__ __launch_bounds__(256) void force_reduce_kernel(const P *__restrict__
 	const double f[3] = {f0, f1, f2};
 	const unsigned total = (unsigned)ntau * loc;
 	reduce_links(partial, nwg, loc, blockIdx.x * 64, 64, total, tau_b, f, fscale, pdot, p, kick, scratch, 256, clip);
+}
+
+template <typename P>
+struct ReduceArgs {
+	const P *partial;
+	unsigned nwg, loc;
+	int tau_b, ntau;
+	double f[3];
+	const double *fscale;
+	double *pdot, *p;
+	double kick;
+	int clip;
+	unsigned gx;
+};
+
+template <typename P>
+__global__ __launch_bounds__(256) void force_reduce_multi_kernel(const ReduceArgs<P> *__restrict__ tab)
+{
+	__shared__ double scratch[256];
+	const ReduceArgs<P> &a = tab[blockIdx.z];
+	if(blockIdx.x >= a.gx) return;
+	const unsigned total = (unsigned)a.ntau * a.loc;
+	reduce_links(a.partial, a.nwg, a.loc, blockIdx.x * 64, 64, total, a.tau_b, a.f, a.fscale, a.pdot, a.p, a.kick, scratch,
+		256, a.clip);
 }
 
 // CONSTANT_FORCE: every slice gets the force of slice 0 (organic_hmc.c:92-98,128)
@@ -571,15 +610,12 @@ static int launch_adjoint_c(ephmc_ctx *c, AdjArgs<R> a)
 	return EPHMC_OK;
 }
 
-// Segments s = nseg-1 .. 0 of the reverse sweep; W starts from minv (fp64) or, when minv is null,
-// from the W buffer (already in the precision R).  nt_eff < nt restricts the sweep to slices
-// [0, nt_eff) (CONSTANT_FORCE: nt_eff = 1 with W_0 prepared by the caller).
+// Arguments of the force sweep common to all its launches (per-segment fields: adj_segment_args).
 template <typename R>
-static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p,
-		const double *fscale, unsigned nt_eff)
+static AdjArgs<R> adj_base_args(ephmc_ctx *c, int prec, unsigned C, double *pdot, double *p, double kick,
+		const double *fscale)
 {
-	const unsigned ns = c->ns, nt = nt_eff, k = c->ckpt, span = c->span, C = adjoint_cols(c, prec);
-	const size_t mat = c->mat;
+	const unsigned ns = c->ns;
 	AdjArgs<R> a{};
 	a.coef = prec == EPHMC_FP64 ? (const R *)c->d_coef64 : (const R *)c->d_coef32;
 	a.nb = c->d_nb;
@@ -593,33 +629,54 @@ static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot,
 		a.a_quad[q] = (R)c->poly.a_quad[q];
 		a.cplx[q] = c->poly.cplx[q];
 	}
-	a.seg_stride = k * C * ns;
+	a.seg_stride = c->ckpt * C * ns;
 	a.nwg = (ns + C - 1) / C;
 	// W state between segments; T_nt is no longer needed once M_hat is formed
 	a.wbuf = prec == EPHMC_FP64 ? (R *)(void *)c->d_Tfin : (R *)(void *)c->d_Tfin32;
-	const R *ck = (const R *)c->d_ck;
-	const double f[3] = {-c->cfg.kappa[0] * c->cfg.flavors, -c->cfg.kappa[1] * c->cfg.flavors, -c->cfg.kappa[2] * c->cfg.flavors};
-	for(int q = 0; q < 3; q++) a.f[q] = f[q];
+	for(int q = 0; q < 3; q++) a.f[q] = -c->cfg.kappa[q] * c->cfg.flavors;
 	a.pdot = pdot;
 	a.p = p;
 	a.kick = kick;
 	a.fscale = fscale;
 	a.clip = c->cfg.schur_form_inv != 0;
+	return a;
+}
+
+// Segment s of nseg (slices [s span, (s+1) span) up to nt): it starts from minv when it is the
+// last one and reduces the partials of the segment before it (prev, or none)
+template <typename R>
+static void adj_segment_args(ephmc_ctx *c, AdjArgs<R> &a, int s, unsigned nseg, const double *minv, const R *prev,
+		int prev_b, int prev_n, unsigned nt)
+{
+	const unsigned k = c->ckpt, span = c->span;
+	a.tau_b = s * (int)span;
+	a.tau_e = std::min<int>((s + 1) * (int)span, (int)nt);
+	a.ck = (const R *)c->d_ck + (size_t)(a.tau_b / (int)k) * c->mat;
+	a.direct = k == 1;
+	a.w0_early = c->adj_w0_early;
+	a.seg = (R *)c->d_seg;
+	a.w_init = (s == (int)nseg - 1) ? minv : nullptr;
+	a.partial = (R *)(void *)c->d_partial[s & 1];
+	a.red_src = prev;
+	a.red_tau_b = prev_b;
+	a.red_ntau = prev_n;
+}
+
+// Segments s = nseg-1 .. 0 of the reverse sweep; W starts from minv (fp64) or, when minv is null,
+// from the W buffer (already in the precision R).  nt_eff < nt restricts the sweep to slices
+// [0, nt_eff) (CONSTANT_FORCE: nt_eff = 1 with W_0 prepared by the caller).
+template <typename R>
+static int run_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p,
+		const double *fscale, unsigned nt_eff)
+{
+	const unsigned nt = nt_eff, span = c->span, C = adjoint_cols(c, prec);
+	AdjArgs<R> a = adj_base_args<R>(c, prec, C, pdot, p, kick, fscale);
+	const double f[3] = {a.f[0], a.f[1], a.f[2]};
 	const unsigned nseg = (nt + span - 1) / span;
 	int prev_b = 0, prev_n = 0;
 	const R *prev = nullptr;
 	for(int s = (int)nseg - 1; s >= 0; s--){
-		a.tau_b = s * (int)span;
-		a.tau_e = std::min<int>((s + 1) * (int)span, (int)nt);
-		a.ck = ck + (size_t)(a.tau_b / (int)k) * mat;
-		a.direct = k == 1;
-		a.w0_early = c->adj_w0_early;
-		a.seg = (R *)c->d_seg;
-		a.w_init = (s == (int)nseg - 1) ? minv : nullptr;
-		a.partial = (R *)(void *)c->d_partial[s & 1];
-		a.red_src = prev;
-		a.red_tau_b = prev_b;
-		a.red_ntau = prev_n;
+		adj_segment_args<R>(c, a, s, nseg, minv, prev, prev_b, prev_n, nt);
 		int rc;
 		EPHMC_TRY(timer_begin(c, "adjoint_kernel"));
 		switch(C){
@@ -648,6 +705,76 @@ int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, doub
 {
 	if(prec == EPHMC_FP64) return run_adjoint<double>(c, prec, minv, pdot, kick, p, fscale, c->nt);
 	return run_adjoint<float>(c, prec, minv, pdot, kick, p, fscale, c->nt);
+}
+
+// ---- many chains in one launch (batch.hip): the sweep of force_adjoint(c, prec, minv, pdot, 0,
+// null, null) with one site per thread on a workgroup of `threads` >= ns; the bits do not depend
+// on the workgroup size (the columns per workgroup, which fix the partial sums, are the context's)
+template <typename R, int C, int BT>
+static int batch_adj_add(BatchOp &op, const AdjArgs<R> &a)
+{
+	const size_t lds = adj_lds_bytes<R, C, BT, 1>(a.ns);
+	if(lds > 160 * 1024){ set_error("batch adjoint: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
+	const void *fn = (const void *)adjoint_multi_kernel<R, C, BT, 1, 4>;
+	(void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max(lds, op.lds));
+	return op.add(fn, BT, lds, a.nwg, 1, a);
+}
+
+template <typename R, int C>
+static int batch_adj_bt(BatchOp &op, unsigned threads, const AdjArgs<R> &a)
+{
+	switch(threads){
+		case 256: return batch_adj_add<R, C, 256>(op, a);
+		case 512: return batch_adj_add<R, C, 512>(op, a);
+		case 1024: return batch_adj_add<R, C, 1024>(op, a);
+		default: set_error("batch adjoint: %u threads per workgroup", threads); return EPHMC_ERR_ARG;
+	}
+}
+
+template <typename R>
+static int batch_adjoint_r(std::vector<BatchOp> &segs, BatchOp &reduce, ephmc_ctx *c, int prec, unsigned threads,
+		const double *minv, double *pdot)
+{
+	if(c->poly.deg != 4){ set_error("batch: polynomial degree %d (the batch runs Taylor order 4)", c->poly.deg); return EPHMC_ERR_UNSUPPORTED; }
+	if(c->ns > threads){ set_error("batch adjoint: %u sites on %u threads", c->ns, threads); return EPHMC_ERR_ARG; }
+	const unsigned nt = c->nt, span = c->span, C = adjoint_cols(c, prec);
+	AdjArgs<R> a = adj_base_args<R>(c, prec, C, pdot, nullptr, 0, nullptr);
+	const unsigned nseg = (nt + span - 1) / span;
+	if(segs.size() < nseg) segs.resize(nseg);
+	int prev_b = 0, prev_n = 0;
+	const R *prev = nullptr;
+	for(int s = (int)nseg - 1; s >= 0; s--){
+		adj_segment_args<R>(c, a, s, nseg, minv, prev, prev_b, prev_n, nt);
+		BatchOp &op = segs[nseg - 1 - s];
+		int rc;
+		switch(C){
+			case 1: rc = batch_adj_bt<R, 1>(op, threads, a); break;
+			case 2: rc = batch_adj_bt<R, 2>(op, threads, a); break;
+			default: rc = batch_adj_bt<R, 4>(op, threads, a); break;
+		}
+		if(rc != EPHMC_OK) return rc;
+		prev = a.partial;
+		prev_b = a.tau_b;
+		prev_n = a.tau_e - a.tau_b;
+	}
+	ReduceArgs<R> r{};
+	r.partial = prev;
+	r.nwg = a.nwg;
+	r.loc = c->loc;
+	r.tau_b = prev_b;
+	r.ntau = prev_n;
+	for(int q = 0; q < 3; q++) r.f[q] = a.f[q];
+	r.pdot = pdot;
+	r.clip = a.clip;
+	r.gx = (unsigned)(((size_t)prev_n * c->loc + 63) / 64);       // the grid of run_adjoint's reduction
+	return reduce.add((const void *)force_reduce_multi_kernel<R>, 256, 0, r.gx, 1, r);
+}
+
+int batch_adjoint(std::vector<BatchOp> &segs, BatchOp &reduce, ephmc_ctx *c, int prec, unsigned threads,
+		const double *minv, double *pdot)
+{
+	if(prec == EPHMC_FP64) return batch_adjoint_r<double>(segs, reduce, c, prec, threads, minv, pdot);
+	return batch_adjoint_r<float>(segs, reduce, c, prec, threads, minv, pdot);
 }
 
 // CONSTANT_FORCE (organic_hmc.c:92-98 with construct_QM1P at time 0, organic_dense_matrix.c:267-309):

@@ -53,7 +53,7 @@ __device__ __forceinline__ unsigned mode_pos(unsigned n, const FftArgs &a)
 	return a.logn ? (__brev(n) >> (32 - a.logn)) : 0u;
 }
 
-__global__ __launch_bounds__(FFT_BT) void fa_fft_kernel(FftArgs a)
+__device__ __forceinline__ void fa_fft_body(const FftArgs &a, unsigned bx)
 {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	const unsigned nt = a.nt;
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(FFT_BT) void fa_fft_kernel(FftArgs a)
 	__shared__ double red[FFT_BT];
 	const int tid = threadIdx.x;
 	const unsigned pairs = (a.mode == FFT_EVOLVE) ? 1 : 2;         // real series per complex slot
-	const unsigned l0 = blockIdx.x * FFT_LW * pairs;
+	const unsigned l0 = bx * FFT_LW * pairs;
 
 	for(unsigned j = tid; j < nt; j += FFT_BT) tw[j] = make_double2(a.twid[2 * j], a.twid[2 * j + 1]);
 	// ---- load
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(FFT_BT) void fa_fft_kernel(FftArgs a)
 			if(tid < off) red[tid] += red[tid + off];
 			__syncthreads();
 		}
-		if(tid == 0) a.partial[blockIdx.x] = red[0];
+		if(tid == 0) a.partial[bx] = red[0];
 		return;
 	}
 	if(a.mode == FFT_FILTER){
@@ -205,6 +205,24 @@ __global__ __launch_bounds__(FFT_BT) void fa_fft_kernel(FftArgs a)
 	}
 }
 
+__global__ __launch_bounds__(FFT_BT) void fa_fft_kernel(FftArgs a)
+{
+	fa_fft_body(a, blockIdx.x);
+}
+
+// many chains in one launch: blockIdx.z = chain, with its own number of workgroups in gx
+struct FftMultiArgs {
+	FftArgs a;
+	unsigned gx;
+};
+
+__global__ __launch_bounds__(FFT_BT) void fa_fft_multi_kernel(const FftMultiArgs *__restrict__ tab)
+{
+	const FftMultiArgs &m = tab[blockIdx.z];
+	if(blockIdx.x >= m.gx) return;
+	fa_fft_body(m.a, blockIdx.x);
+}
+
 static FftArgs fft_args(ephmc_ctx *c, int mode)
 {
 	FftArgs a{};
@@ -220,19 +238,28 @@ static FftArgs fft_args(ephmc_ctx *c, int mode)
 	return a;
 }
 
-static int launch_fft(ephmc_ctx *c, const FftArgs &a)
+// workgroups and LDS bytes of one transform launch
+static int fft_shape(ephmc_ctx *c, const FftArgs &a, unsigned *blocks, size_t *lds)
 {
 	const unsigned per = (a.mode == FFT_EVOLVE) ? FFT_LW : 2 * FFT_LW;
-	const unsigned blocks = (c->loc + per - 1) / per;
-	const size_t lds = (size_t)c->nt * FFT_LW * sizeof(double2) * (a.pow2 ? 1 : 2) + (size_t)c->nt * sizeof(double2);
-	if(lds > 160 * 1024){ set_error("fft: nt = %u needs %zu B of LDS", c->nt, lds); return EPHMC_ERR_UNSUPPORTED; }
+	*blocks = (c->loc + per - 1) / per;
+	*lds = (size_t)c->nt * FFT_LW * sizeof(double2) * (a.pow2 ? 1 : 2) + (size_t)c->nt * sizeof(double2);
+	if(*lds > 160 * 1024){ set_error("fft: nt = %u needs %zu B of LDS", c->nt, *lds); return EPHMC_ERR_UNSUPPORTED; }
+	return EPHMC_OK;
+}
+
+static int launch_fft(ephmc_ctx *c, const FftArgs &a)
+{
+	unsigned blocks;
+	size_t lds;
+	EPHMC_TRY(fft_shape(c, a, &blocks, &lds));
 	(void)hipFuncSetAttribute((const void *)fa_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 	fa_fft_kernel<<<blocks, FFT_BT, lds, c->stream>>>(a);
 	EPHMC_LAUNCHED();
 	return EPHMC_OK;
 }
 
-int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick)
+static FftArgs evolve_args(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick)
 {
 	FftArgs a = fft_args(c, FFT_EVOLVE);
 	a.f2 = c->d_f2;
@@ -240,7 +267,12 @@ int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, 
 	a.kickF = pdot; a.kick = kick;
 	a.c = cos(h) / c->nt;
 	a.s = sin(h) / c->nt;
-	return launch_fft(c, a);
+	return a;
+}
+
+int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick)
+{
+	return launch_fft(c, evolve_args(c, x, p, h, pdot, kick));
 }
 
 // inverse_sqrt = 0: momentum refresh with the kinetic frequencies (organic_bosons.c:76-100,
@@ -465,6 +497,160 @@ int metropolis(ephmc_ctx *c, double uniform)
 	accept_copy_kernel<<<1024, 256, 0, c->stream>>>(c->d_scal, c->d_mhat, c->d_minv, c->mat);
 	EPHMC_LAUNCHED();
 	return EPHMC_OK;
+}
+
+// ---- many chains in one launch (batch.hip): the kernels above with the arguments of chain
+// blockIdx.z read from a table.  The reductions keep each chain's own grid (gx), so that every
+// partial sum covers the same elements as in the single-chain launch.
+static int batch_fft(BatchOp &op, ephmc_ctx *c, const FftArgs &a)
+{
+	FftMultiArgs m{a, 0};
+	size_t lds;
+	EPHMC_TRY(fft_shape(c, a, &m.gx, &lds));
+	(void)hipFuncSetAttribute((const void *)fa_fft_multi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+		(int)std::max(lds, op.lds));
+	return op.add((const void *)fa_fft_multi_kernel, FFT_BT, lds, m.gx, 1, m);
+}
+
+int batch_fa_evolve(BatchOp &op, ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick)
+{
+	return batch_fft(op, c, evolve_args(c, x, p, h, pdot, kick));
+}
+
+int batch_fa_filter(BatchOp &op, ephmc_ctx *c, const double *in, double *out)
+{
+	FftArgs a = fft_args(c, FFT_FILTER);
+	a.in0 = in; a.out0 = out;
+	a.gain = 0;
+	return batch_fft(op, c, a);
+}
+
+struct SumArgs {
+	const double *x;         // boson action: the field; sums: the partials
+	size_t dim;
+	unsigned loc, gx;
+	double *part;            // boson action: partials
+	double omega;
+	double *out;
+};
+
+__global__ __launch_bounds__(256) void boson_action_multi_kernel(const SumArgs *__restrict__ tab)
+{
+	__shared__ double rp[256], rk[256];
+	const SumArgs &a = tab[blockIdx.z];
+	if(blockIdx.x >= a.gx) return;
+	double pot = 0, kin = 0;
+	for(size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < a.dim; e += (size_t)a.gx * 256){
+		const double v = a.x[e];
+		const size_t nxt = (e + a.loc < a.dim) ? e + a.loc : e + a.loc - a.dim;
+		const double d = a.x[nxt] - v;
+		pot += v * v;
+		kin += d * d;
+	}
+	rp[threadIdx.x] = pot;
+	rk[threadIdx.x] = kin;
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1){
+		if((int)threadIdx.x < off){ rp[threadIdx.x] += rp[threadIdx.x + off]; rk[threadIdx.x] += rk[threadIdx.x + off]; }
+		__syncthreads();
+	}
+	if(threadIdx.x == 0){ a.part[2 * blockIdx.x] = rp[0]; a.part[2 * blockIdx.x + 1] = rk[0]; }
+}
+
+__global__ __launch_bounds__(256) void boson_action_final_multi_kernel(const SumArgs *__restrict__ tab)
+{
+	const SumArgs &a = tab[blockIdx.z];
+	__shared__ double rp[256], rk[256];
+	double pot = 0, kin = 0;
+	for(unsigned i = threadIdx.x; i < a.gx; i += 256){ pot += a.x[2 * i]; kin += a.x[2 * i + 1]; }
+	rp[threadIdx.x] = pot;
+	rk[threadIdx.x] = kin;
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1){
+		if((int)threadIdx.x < off){ rp[threadIdx.x] += rp[threadIdx.x + off]; rk[threadIdx.x] += rk[threadIdx.x + off]; }
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) *a.out = (rk[0] / a.omega + rp[0] * a.omega) / 2;
+}
+
+__global__ __launch_bounds__(256) void sum_partials_multi_kernel(const SumArgs *__restrict__ tab)
+{
+	const SumArgs &a = tab[blockIdx.z];
+	__shared__ double red[256];
+	double s = 0;
+	for(unsigned i = threadIdx.x; i < a.gx; i += 256) s += a.x[i];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1){
+		if((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) *a.out = red[0];
+}
+
+struct ScalArgs {
+	double *scal;
+	int logdet_slot, out_slot;
+	double flavors;
+	const double *uniform;
+};
+
+__global__ void hamilton_multi_kernel(const ScalArgs *__restrict__ tab)
+{
+	const ScalArgs &a = tab[blockIdx.z];
+	a.scal[a.out_slot] = -a.scal[a.logdet_slot] * a.flavors + a.scal[SCAL_EP] + a.scal[SCAL_BOSON];
+}
+
+__global__ void metropolis_multi_kernel(const ScalArgs *__restrict__ tab)
+{
+	const ScalArgs &a = tab[blockIdx.z];
+	double *scal = a.scal;
+	const double hold = scal[SCAL_H_OLD], hnew = scal[SCAL_H_NEW];
+	const double boltz = exp(hold - hnew);
+	const int acc = (boltz > *a.uniform) ? 1 : 0;
+	scal[SCAL_ACC] = acc;
+	scal[SCAL_BOLTZ] = boltz;
+	scal[SCAL_DH] = hnew - hold;
+	scal[SCAL_ENERGY] = acc ? hnew : hold;
+	if(acc) scal[SCAL_LOGDET_CHAIN] = scal[SCAL_LOGDET];
+}
+
+// hamilton_dev(c, x, p, logdet_slot, out_slot) with p != null in the Fourier-accelerated form:
+// kinetic energy (transform, partial sum), boson action (partials, final), H
+int batch_hamilton(BatchOp *ops, ephmc_ctx *c, const double *x, const double *p, int logdet_slot, int out_slot)
+{
+	if(c->cfg.no_fourier_acc){ set_error("batch: NO_FOURIER_ACC is not batched"); return EPHMC_ERR_UNSUPPORTED; }
+	FftArgs a = fft_args(c, FFT_ENERGY);
+	a.in0 = p;
+	a.partial = c->d_red;
+	EPHMC_TRY(batch_fft(ops[0], c, a));
+	SumArgs s{};
+	s.x = c->d_red;
+	s.gx = (c->loc + 2 * FFT_LW - 1) / (2 * FFT_LW);
+	s.out = c->d_scal + SCAL_EP;
+	EPHMC_TRY(ops[1].add((const void *)sum_partials_multi_kernel, 256, 0, 1, 1, s));
+	SumArgs b{};
+	b.x = x;
+	b.dim = c->dim;
+	b.loc = c->loc;
+	b.gx = c->red_blocks;
+	b.part = c->d_red;
+	EPHMC_TRY(ops[2].add((const void *)boson_action_multi_kernel, 256, 0, b.gx, 1, b));
+	SumArgs f{};
+	f.x = c->d_red;
+	f.gx = c->red_blocks;
+	f.omega = c->cfg.omega;
+	f.out = c->d_scal + SCAL_BOSON;
+	EPHMC_TRY(ops[3].add((const void *)boson_action_final_multi_kernel, 256, 0, 1, 1, f));
+	const ScalArgs h{c->d_scal, logdet_slot, out_slot, (double)c->cfg.flavors, nullptr};
+	return ops[4].add((const void *)hamilton_multi_kernel, 1, 0, 1, 1, h);
+}
+
+// metropolis(c, u) with u read from device memory (the accept copies are batch_copy records)
+int batch_metropolis(BatchOp &op, ephmc_ctx *c, const double *uniform)
+{
+	const ScalArgs a{c->d_scal, 0, 0, 0.0, uniform};
+	return op.add((const void *)metropolis_multi_kernel, 1, 0, 1, 1, a);
 }
 
 } // namespace ephmc

@@ -14,15 +14,17 @@
 #include "internal.h"
 #include "stencil.h"
 #include <cstdlib>
+#include <algorithm>
+#include <cstring>
 
 namespace ephmc {
 
-__global__ void coef_kernel(const double *__restrict__ x, const unsigned *__restrict__ link,
+__device__ __forceinline__ void coef_body(const double *__restrict__ x, const unsigned *__restrict__ link,
 		double t0, double t1, double t2, double k0, double k1, double k2,
-		unsigned ns, unsigned nt, double *__restrict__ c64, float *__restrict__ c32)
+		unsigned ns, unsigned nt, double *__restrict__ c64, float *__restrict__ c32, unsigned bx, unsigned gx)
 {
 	const size_t n = (size_t)nt * 6 * ns;
-	for(size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x){
+	for(size_t e = bx * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gx * blockDim.x){
 		const size_t tau = e / (6 * ns);
 		const unsigned r = (unsigned)(e - tau * 6 * ns);
 		const unsigned d = r / ns, s = r - d * ns;
@@ -33,6 +35,30 @@ __global__ void coef_kernel(const double *__restrict__ x, const unsigned *__rest
 		if(c64) c64[e] = f;
 		if(c32) c32[e] = (float)f;
 	}
+}
+
+__global__ void coef_kernel(const double *__restrict__ x, const unsigned *__restrict__ link,
+		double t0, double t1, double t2, double k0, double k1, double k2,
+		unsigned ns, unsigned nt, double *__restrict__ c64, float *__restrict__ c32)
+{
+	coef_body(x, link, t0, t1, t2, k0, k1, k2, ns, nt, c64, c32, blockIdx.x, gridDim.x);
+}
+
+// many chains in one launch (blockIdx.z = chain; each chain's own grid in gx)
+struct CoefArgs {
+	const double *x;
+	const unsigned *link;
+	double t[3], k[3];
+	unsigned ns, nt, gx;
+	double *c64;
+	float *c32;
+};
+
+__global__ void coef_multi_kernel(const CoefArgs *__restrict__ tab)
+{
+	const CoefArgs &a = tab[blockIdx.z];
+	if(blockIdx.x >= a.gx) return;
+	coef_body(a.x, a.link, a.t[0], a.t[1], a.t[2], a.k[0], a.k[1], a.k[2], a.ns, a.nt, a.c64, a.c32, blockIdx.x, a.gx);
 }
 
 int coefficients(ephmc_ctx *c, const double *x, bool want32)
@@ -105,14 +131,14 @@ constexpr size_t chain_lds_bytes(unsigned ns)
 }
 
 template <typename R, int C, int BT, int S>
-__global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
+__device__ __forceinline__ void chain_body(const ChainArgs<R> &a, unsigned bx)
 {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	constexpr bool ALL_LANES = chain_all_lanes<R, S>();
 	const unsigned ns = a.ns;
 	R *V = reinterpret_cast<R *>(smem_raw);
 	R *W = V + (ALL_LANES ? (size_t)C * BT : (size_t)C * ns);
-	const unsigned col0 = blockIdx.x * C;
+	const unsigned col0 = bx * C;
 	const size_t mat = (size_t)ns * ns;
 
 	unsigned site[S];
@@ -207,6 +233,22 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 			for(int c = 0; c < C; c++)
 				if(act[j] && col0 + c < ns) a.dst[(size_t)(col0 + c) * ns + site[j]] = v[j][c];
 	}
+}
+
+template <typename R, int C, int BT, int S>
+__global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
+{
+	chain_body<R, C, BT, S>(a, blockIdx.x);
+}
+
+// many chains in one launch: blockIdx.z = chain, blockIdx.x = its column block (blocks past the
+// chain's own ceil(ns / C) return)
+template <typename R, int C, int BT, int S>
+__global__ __launch_bounds__(BT) void chain_multi_kernel(const ChainArgs<R> *__restrict__ tab)
+{
+	const ChainArgs<R> &a = tab[blockIdx.z];
+	if(blockIdx.x >= (a.ns + C - 1) / C) return;
+	chain_body<R, C, BT, S>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -427,30 +469,29 @@ unsigned chain_cols(const ephmc_ctx *c, int prec)
 // Chain of slices over all columns: dst = B_{tau_last} ... B_{tau_first} src, with the
 // slices tau0, tau0+dir, ... (nt of them); src null = identity.  Stores the state before
 // every ck_every-th slice into ck.  prec selects fp64 or fp32 storage and arithmetic.
+template <typename R>
+static ChainArgs<R> forward_args(ephmc_ctx *c, const R *coef, const void *src, void *dst, void *ck, unsigned ck_every,
+		int descending, bool from_identity)
+{
+	ChainArgs<R> a = make_args<R>(c, coef);
+	a.src = from_identity ? nullptr : (const R *)src;
+	a.dst = (R *)dst;
+	a.ck = (R *)ck;
+	a.ck_every = ck_every ? ck_every : 1;
+	a.tau0 = descending ? (int)c->nt - 1 : 0;
+	a.dir = descending ? -1 : 1;
+	a.ntau = (int)c->nt;
+	return a;
+}
+
 int chain_forward(ephmc_ctx *c, int prec, const void *src, void *dst, void *ck, unsigned ck_every,
 		int descending, bool from_identity)
 {
-	if(prec == EPHMC_FP64){
-		ChainArgs<double> a = make_args<double>(c, c->d_coef64);
-		a.src = from_identity ? nullptr : (const double *)src;
-		a.dst = (double *)dst;
-		a.ck = (double *)ck;
-		a.ck_every = ck_every ? ck_every : 1;
-		a.tau0 = descending ? (int)c->nt - 1 : 0;
-		a.dir = descending ? -1 : 1;
-		a.ntau = (int)c->nt;
-		return launch_chain<double>(c, a, chain_cols(c, prec));
-	}else{
-		ChainArgs<float> a = make_args<float>(c, c->d_coef32);
-		a.src = from_identity ? nullptr : (const float *)src;
-		a.dst = (float *)dst;
-		a.ck = (float *)ck;
-		a.ck_every = ck_every ? ck_every : 1;
-		a.tau0 = descending ? (int)c->nt - 1 : 0;
-		a.dir = descending ? -1 : 1;
-		a.ntau = (int)c->nt;
-		return launch_chain<float>(c, a, chain_cols(c, prec));
-	}
+	if(prec == EPHMC_FP64)
+		return launch_chain<double>(c, forward_args<double>(c, c->d_coef64, src, dst, ck, ck_every, descending, from_identity),
+			chain_cols(c, prec));
+	return launch_chain<float>(c, forward_args<float>(c, c->d_coef32, src, dst, ck, ck_every, descending, from_identity),
+		chain_cols(c, prec));
 }
 
 // dst = B_{tau_last} ... B_{tau0} src over the slices tau0, tau0+dir, ... (ntau of them) in the
@@ -517,11 +558,11 @@ int apply_exp_partial(ephmc_ctx *c, const double *x_tau, double *M, int revert, 
 // ---------------------------------------------------------------------------------------------
 // M = T^T (+ identity): 32x32 tiles through LDS; T in R, M in fp64.
 template <typename R>
-__global__ __launch_bounds__(256) void transpose_kernel(const R *__restrict__ T, double *__restrict__ M,
-		unsigned n, int add_identity)
+__device__ __forceinline__ void transpose_body(const R *__restrict__ T, double *__restrict__ M, unsigned n, int add_identity,
+		unsigned bxi, unsigned byi)
 {
 	__shared__ double tile[32][33];
-	const unsigned bx = blockIdx.x * 32, by = blockIdx.y * 32;
+	const unsigned bx = bxi * 32, by = byi * 32;
 	const unsigned tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
 	for(unsigned r = ty; r < 32; r += 8){
 		const unsigned col = by + r, row = bx + tx;          // T[col][row] (column col of T)
@@ -532,6 +573,29 @@ __global__ __launch_bounds__(256) void transpose_kernel(const R *__restrict__ T,
 		const unsigned col = bx + r, row = by + tx;          // M column col = row col of T
 		if(col < n && row < n) M[(size_t)col * n + row] = tile[tx][r] + ((add_identity && col == row) ? 1.0 : 0.0);
 	}
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void transpose_kernel(const R *__restrict__ T, double *__restrict__ M,
+		unsigned n, int add_identity)
+{
+	transpose_body<R>(T, M, n, add_identity, blockIdx.x, blockIdx.y);
+}
+
+struct TransArgs {
+	const void *T;
+	double *M;
+	unsigned n;
+	int add_identity;
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void transpose_multi_kernel(const TransArgs *__restrict__ tab)
+{
+	const TransArgs &a = tab[blockIdx.z];
+	const unsigned g = (a.n + 31) / 32;
+	if(blockIdx.x >= g || blockIdx.y >= g) return;
+	transpose_body<R>((const R *)a.T, a.M, a.n, a.add_identity, blockIdx.x, blockIdx.y);
 }
 
 int transpose_plus_identity(ephmc_ctx *c, int prec, const void *T, double *M, int add_identity)
@@ -547,6 +611,17 @@ int transpose_plus_identity(ephmc_ctx *c, int prec, const void *T, double *M, in
 __global__ void add_identity_kernel(double *__restrict__ M, unsigned n)
 {
 	for(unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) M[(size_t)i * (n + 1)] += 1.0;
+}
+
+struct AddIdArgs {
+	double *M;
+	unsigned n;
+};
+
+__global__ void add_identity_multi_kernel(const AddIdArgs *__restrict__ tab)
+{
+	const AddIdArgs &a = tab[blockIdx.z];
+	for(unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) a.M[(size_t)i * (a.n + 1)] += 1.0;
 }
 
 int add_identity(ephmc_ctx *c, double *M)
@@ -623,6 +698,91 @@ int convert_f64_to_f32(ephmc_ctx *c, const double *a, float *b, size_t n)
 	f64_to_f32_kernel<<<blocks, 256, 0, c->stream>>>(a, b, n);
 	EPHMC_LAUNCHED();
 	return EPHMC_OK;
+}
+
+// ---- many chains in one launch (batch.hip).  The chain runs in the site form with 2 columns per
+// workgroup; every column is computed with the arithmetic of every other form and width, so a
+// chain gives the same bits in a batch as alone.
+int batch_coef(BatchOp &op, ephmc_ctx *c, const double *x, bool want32)
+{
+	CoefArgs a{};
+	a.x = x;
+	a.link = c->d_link;
+	for(int k = 0; k < 3; k++){ a.t[k] = c->cfg.t[k]; a.k[k] = c->cfg.kappa[k]; }
+	a.ns = c->ns;
+	a.nt = c->nt;
+	const size_t n = (size_t)c->nt * 6 * c->ns;
+	a.gx = (unsigned)std::min<size_t>((n + 255) / 256, 4096);     // the grid of coefficients()
+	a.c64 = c->d_coef64;
+	a.c32 = want32 ? c->d_coef32 : nullptr;
+	return op.add((const void *)coef_multi_kernel, 256, 0, a.gx, 1, a);
+}
+
+constexpr int BATCH_CHAIN_C = 2;
+
+template <typename R, int BT>
+static int batch_chain_t(BatchOp &op, const ChainArgs<R> &a)
+{
+	const size_t lds = chain_lds_bytes<R, BATCH_CHAIN_C, BT, 1>(a.ns);
+	if(lds > 160 * 1024){ set_error("batch chain: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
+	const void *fn = (const void *)chain_multi_kernel<R, BATCH_CHAIN_C, BT, 1>;
+	(void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max(lds, op.lds));
+	return op.add(fn, BT, lds, (a.ns + BATCH_CHAIN_C - 1) / BATCH_CHAIN_C, 1, a);
+}
+
+template <typename R>
+static int batch_chain_r(BatchOp &op, unsigned threads, const ChainArgs<R> &a)
+{
+	if(a.ns > threads){ set_error("batch chain: %u sites on %u threads", a.ns, threads); return EPHMC_ERR_ARG; }
+	switch(threads){
+		case 256: return batch_chain_t<R, 256>(op, a);
+		case 512: return batch_chain_t<R, 512>(op, a);
+		case 1024: return batch_chain_t<R, 1024>(op, a);
+		default: set_error("batch chain: %u threads per workgroup", threads); return EPHMC_ERR_ARG;
+	}
+}
+
+int batch_chain(BatchOp &op, ephmc_ctx *c, int prec, unsigned threads, void *dst, void *ck, unsigned ck_every,
+		int descending)
+{
+	if(prec == EPHMC_FP64)
+		return batch_chain_r<double>(op, threads, forward_args<double>(c, c->d_coef64, nullptr, dst, ck, ck_every, descending, true));
+	return batch_chain_r<float>(op, threads, forward_args<float>(c, c->d_coef32, nullptr, dst, ck, ck_every, descending, true));
+}
+
+int batch_transpose(BatchOp &op, ephmc_ctx *c, int prec, const void *T, double *M)
+{
+	const TransArgs a{T, M, c->ns, 1};
+	const unsigned g = (c->ns + 31) / 32;
+	const void *fn = prec == EPHMC_FP64 ? (const void *)transpose_multi_kernel<double> : (const void *)transpose_multi_kernel<float>;
+	return op.add(fn, 256, 0, g, g, a);
+}
+
+int batch_add_identity(BatchOp &op, ephmc_ctx *c, double *M)
+{
+	const AddIdArgs a{M, c->ns};
+	return op.add((const void *)add_identity_multi_kernel, 256, 0, (c->ns + 255) / 256, 1, a);
+}
+
+struct CopyArgs {
+	const double *src;
+	double *dst;
+	size_t n;
+	const double *gate;
+};
+
+// elementwise, so any grid gives the same result
+__global__ __launch_bounds__(256) void copy_multi_kernel(const CopyArgs *__restrict__ tab)
+{
+	const CopyArgs &a = tab[blockIdx.z];
+	if(a.gate && a.gate[SCAL_ACC] == 0) return;
+	for(size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < a.n; e += (size_t)gridDim.x * 256) a.dst[e] = a.src[e];
+}
+
+int batch_copy(BatchOp &op, const double *src, double *dst, size_t n, const double *gate)
+{
+	const CopyArgs a{src, dst, n, gate};
+	return op.add((const void *)copy_multi_kernel, 256, 0, (unsigned)std::min<size_t>((n + 255) / 256, 1024), 1, a);
 }
 
 } // namespace ephmc

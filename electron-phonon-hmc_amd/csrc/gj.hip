@@ -198,7 +198,7 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 }
 
 // Inversion of the first diagonal block A[0:nbe, 0:nbe] (one workgroup).
-__global__ __launch_bounds__(256) void gj_first_kernel(const double *__restrict__ A, unsigned n, unsigned nbe,
+__device__ __forceinline__ void gj_first_body(const double *__restrict__ A, unsigned n, unsigned nbe,
 		double *__restrict__ dinv_out, double *__restrict__ lm_out, double *__restrict__ diag, int *__restrict__ piv,
 		double *__restrict__ scal)
 {
@@ -221,6 +221,13 @@ __global__ __launch_bounds__(256) void gj_first_kernel(const double *__restrict_
 	gj_invert_block(a, nbe, sh, dinv_out, lm_out, diag, piv, 0, scal);
 }
 
+__global__ __launch_bounds__(256) void gj_first_kernel(const double *__restrict__ A, unsigned n, unsigned nbe,
+		double *__restrict__ dinv_out, double *__restrict__ lm_out, double *__restrict__ diag, int *__restrict__ piv,
+		double *__restrict__ scal)
+{
+	gj_first_body(A, n, nbe, dinv_out, lm_out, diag, piv, scal);
+}
+
 // Block step for K = [k0, k0+nbe), grid (row tiles + 1, column tiles) of 64 x 64 tiles, with
 // D^{-1} and L of D = A[K,K] read from dinv_cur / lm_cur.
 //
@@ -237,10 +244,10 @@ __global__ __launch_bounds__(256) void gj_first_kernel(const double *__restrict_
 // the next step, so that no step waits for an inversion of its own.
 constexpr int GJ_RB = 64, GJ_CB = 64, GJ_LD = 68;
 
-__global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__ A, double *__restrict__ An,
+__device__ __forceinline__ void gj_step_body(const double *__restrict__ A, double *__restrict__ An,
 		unsigned n, unsigned k0, unsigned nbe, const double *__restrict__ dinv_cur, const double *__restrict__ lm_cur,
 		double *__restrict__ dinv_next, double *__restrict__ lm_next, double *__restrict__ diag, int *__restrict__ piv,
-		double *__restrict__ scal, double *__restrict__ out_final)
+		double *__restrict__ scal, double *__restrict__ out_final, unsigned bx, unsigned by, unsigned ncb_grid)
 {
 	constexpr int NB = GJ_NB;
 	__shared__ __align__(16) double dinv[NB][NB + 2];
@@ -256,13 +263,13 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 	const unsigned nrb = (n + GJ_RB - 1) / GJ_RB;
 	const unsigned k1 = k0 + nbe;
-	if(blockIdx.x == nrb){
+	if(bx == nrb){
 		// ---- lookahead: D' = A[K',K'] + E[K',:] A[K,K'] with E[K',:] = -A[K',K] D^{-1}, both
 		// products on the FP64 matrix cores (v_mfma_f64_16x16x4): wave w forms the 16 x 16 quadrant
 		// rows 16 qi + m, columns 16 qj + n (qi = w / 2, qj = w % 2) in 8 steps of k = 4.  The tile
 		// that holds A'[K',K'] sums in another order, so D' may differ from it in the last bits: the
 		// inversion uses D', the next step's panels use the tile's values (rounding-level difference)
-		if(blockIdx.y != 0) return;
+		if(by != 0) return;
 		if(k1 >= n){
 			// last step: no next block; with out_final this workgroup sums log|det| and its sign
 			// from the pivots (the flag is final: the last block's multipliers were checked when it
@@ -346,7 +353,7 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 		gj_invert_block(a, nb1, sh, dinv_next, lm_next, diag, piv, k1, scal);
 		return;
 	}
-	const unsigned r0 = blockIdx.x * GJ_RB, c0 = blockIdx.y * GJ_CB;
+	const unsigned r0 = bx * GJ_RB, c0 = by * GJ_CB;
 	// the last step writes the inverse straight to out_final unless the pivoted fallback will run
 	// (it inverts the untouched input there)
 	double *__restrict__ dst = (out_final && scal[SCAL_GJ_FLAG] == 0) ? out_final : An;
@@ -354,12 +361,12 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	// first LDS store so that they share one round trip
 	// the LU multipliers of a row block are checked by all its tiles: tile y takes the columns
 	// j = y, y + ncb, ... of L (lm[m][jj] holds column y + ncb jj)
-	const unsigned ncb = gridDim.y;
+	const unsigned ncb = ncb_grid;
 	double dv[NB * NB / 256], lv[NB * NB / 256], ysv[NB * GJ_CB / 256], arv[GJ_RB * NB / 256];
 #pragma unroll
 	for(int i = 0; i < NB * NB / 256; i++){
 		const int e = tid + 256 * i, m = e % NB, jj = e / NB;
-		const unsigned j = blockIdx.y + ncb * (unsigned)jj;
+		const unsigned j = by + ncb * (unsigned)jj;
 		dv[i] = dinv_cur[e];
 		lv[i] = j < (unsigned)NB ? lm_cur[m * NB + j] : 0.0;
 	}
@@ -443,10 +450,10 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 	{
 		const int row = tid & 63, jg = tid >> 6;
 		const unsigned i = r0 + row;
-		if(i < n && i >= k1 && blockIdx.y + ncb * (unsigned)jg < (unsigned)NB){
+		if(i < n && i >= k1 && by + ncb * (unsigned)jg < (unsigned)NB){
 			bool bad = false;
-			for(int jj = jg; blockIdx.y + ncb * (unsigned)jj < (unsigned)NB; jj += 4){
-				const unsigned j = blockIdx.y + ncb * (unsigned)jj;
+			for(int jj = jg; by + ncb * (unsigned)jj < (unsigned)NB; jj += 4){
+				const unsigned j = by + ncb * (unsigned)jj;
 				double y = 0;
 #pragma unroll 8
 				for(int m = 0; m < NB; m++) y = fmac(Et[m][row], lm[m][jj], y);
@@ -486,6 +493,15 @@ __global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__
 				if(row < n) dst[(size_t)col * n + row] = colK ? Et[col - k0][rl] : acc[cb][rb][e];
 			}
 		}
+}
+
+__global__ __launch_bounds__(256) void gj_step_kernel(const double *__restrict__ A, double *__restrict__ An,
+		unsigned n, unsigned k0, unsigned nbe, const double *__restrict__ dinv_cur, const double *__restrict__ lm_cur,
+		double *__restrict__ dinv_next, double *__restrict__ lm_next, double *__restrict__ diag, int *__restrict__ piv,
+		double *__restrict__ scal, double *__restrict__ out_final)
+{
+	gj_step_body(A, An, n, k0, nbe, dinv_cur, lm_cur, dinv_next, lm_next, diag, piv, scal, out_final, blockIdx.x, blockIdx.y,
+		gridDim.y);
 }
 
 // log|det| and sign from the pivots of the fast path (no row exchanges).
@@ -528,7 +544,7 @@ __global__ void gj_copy_kernel(const double *__restrict__ W, double *__restrict_
 // reference's sum of log|U_ii| with a zero U_ii, organic_dense_matrix.c:582-584), the step is
 // recorded in SCAL_SINGULAR, and M is filled with NaN, so that a force or a Metropolis step built
 // on it rejects instead of using a partial inverse.
-__global__ __launch_bounds__(1024) void gj_fallback_kernel(double *__restrict__ M, unsigned n, double *__restrict__ scal,
+__device__ __forceinline__ void gj_fallback_body(double *__restrict__ M, unsigned n, double *__restrict__ scal,
 		int *__restrict__ piv_out)
 {
 	extern __shared__ double fsh[];
@@ -617,6 +633,12 @@ __global__ __launch_bounds__(1024) void gj_fallback_kernel(double *__restrict__ 
 	}
 }
 
+__global__ __launch_bounds__(1024) void gj_fallback_kernel(double *__restrict__ M, unsigned n, double *__restrict__ scal,
+		int *__restrict__ piv_out)
+{
+	gj_fallback_body(M, n, scal, piv_out);
+}
+
 // M (column-major, N x N, FP64) <- M^{-1}; log|det| and sign into the device scalars.
 int gj_logdet_inverse(ephmc_ctx *c, double *M)
 {
@@ -656,6 +678,85 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M)
 	gj_fallback_kernel<<<1, 1024, lds, c->stream>>>(M, n, c->d_scal, c->d_piv);
 	EPHMC_LAUNCHED();
 	return EPHMC_OK;
+}
+
+// ---- many chains in one launch (batch.hip): the kernels of gj_logdet_inverse with the arguments
+// of chain blockIdx.z read from a table; a chain's blocks past its own grid return.  Only the
+// multi-step form (N > 32, the inverse written by the last step) is batched.
+struct GjFirstArgs {
+	const double *A;
+	unsigned n, nbe;
+	double *dinv_out, *lm_out, *diag;
+	int *piv;
+	double *scal;
+};
+
+__global__ __launch_bounds__(256) void gj_first_multi_kernel(const GjFirstArgs *__restrict__ tab)
+{
+	const GjFirstArgs &a = tab[blockIdx.z];
+	gj_first_body(a.A, a.n, a.nbe, a.dinv_out, a.lm_out, a.diag, a.piv, a.scal);
+}
+
+struct GjStepArgs {
+	const double *A;
+	double *An;
+	unsigned n, k0, nbe;
+	const double *dinv_cur, *lm_cur;
+	double *dinv_next, *lm_next, *diag;
+	int *piv;
+	double *scal, *out_final;
+};
+
+__global__ __launch_bounds__(256) void gj_step_multi_kernel(const GjStepArgs *__restrict__ tab)
+{
+	const GjStepArgs &a = tab[blockIdx.z];
+	const unsigned nrb = (a.n + GJ_RB - 1) / GJ_RB, ncb = (a.n + GJ_CB - 1) / GJ_CB;
+	if(blockIdx.x > nrb || blockIdx.y >= ncb) return;
+	gj_step_body(a.A, a.An, a.n, a.k0, a.nbe, a.dinv_cur, a.lm_cur, a.dinv_next, a.lm_next, a.diag, a.piv, a.scal, a.out_final,
+		blockIdx.x, blockIdx.y, ncb);
+}
+
+struct GjFallbackArgs {
+	double *M;
+	unsigned n;
+	double *scal;
+	int *piv_out;
+};
+
+__global__ __launch_bounds__(1024) void gj_fallback_multi_kernel(const GjFallbackArgs *__restrict__ tab)
+{
+	const GjFallbackArgs &a = tab[blockIdx.z];
+	gj_fallback_body(a.M, a.n, a.scal, a.piv_out);
+}
+
+int batch_gj(BatchOp &first, std::vector<BatchOp> &steps, BatchOp &fallback, ephmc_ctx *c, double *M)
+{
+	const unsigned n = c->ns;
+	if(n <= GJ_NB || n > 4096){ set_error("batch gj: N = %u (the batch inverts 33 <= N <= 4096)", n); return EPHMC_ERR_UNSUPPORTED; }
+	double *dinv_g[2] = {c->d_dlu, c->d_dlu + GJ_NB * GJ_NB}, *lm_g[2] = {c->d_dlu + 2 * GJ_NB * GJ_NB, c->d_dlu + 3 * GJ_NB * GJ_NB};
+	const GjFirstArgs fa{M, n, (unsigned)GJ_NB, dinv_g[0], lm_g[0], c->d_diag, c->d_piv, c->d_scal};
+	EPHMC_TRY(first.add((const void *)gj_first_multi_kernel, 256, 0, 1, 1, fa));
+	const unsigned nrb = (n + GJ_RB - 1) / GJ_RB, ncb = (n + GJ_CB - 1) / GJ_CB;
+	const unsigned nsteps = (n + GJ_NB - 1) / GJ_NB;
+	if(steps.size() < nsteps) steps.resize(nsteps);
+	const double *cur = M;
+	double *buf[2] = {c->d_gj[0], c->d_gj[1]};
+	int which = 0, par = 0;
+	for(unsigned i = 0; i < nsteps; i++){
+		const unsigned k0 = i * GJ_NB, nbe = std::min<unsigned>(GJ_NB, n - k0);
+		double *next = buf[which];
+		GjStepArgs a{cur, next, n, k0, nbe, dinv_g[par], lm_g[par], dinv_g[par ^ 1], lm_g[par ^ 1], c->d_diag, c->d_piv,
+			c->d_scal, k0 + nbe >= n ? M : nullptr};
+		EPHMC_TRY(steps[i].add((const void *)gj_step_multi_kernel, 256, 0, nrb + 1, ncb, a));
+		cur = next;
+		which ^= 1;
+		par ^= 1;
+	}
+	const size_t lds = (size_t)n * (sizeof(double) + sizeof(int));
+	(void)hipFuncSetAttribute((const void *)gj_fallback_multi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+		(int)std::max(lds, fallback.lds));
+	const GjFallbackArgs fb{M, n, c->d_scal, c->d_piv};
+	return fallback.add((const void *)gj_fallback_multi_kernel, 1024, lds, 1, 1, fb);
 }
 
 } // namespace ephmc

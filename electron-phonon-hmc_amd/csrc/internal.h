@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 #include <map>
+#include <algorithm>
 #include "../../include/ephmc.h"
 
 namespace ephmc {
@@ -202,6 +203,35 @@ enum Scalar {            // layout of d_scal
 	SCAL_COUNT = 20
 };
 
+// One launch over many chains (batch.hip): workgroup (x, y, z) is block (x, y) of chain z, whose
+// argument record sits at index z of a device table; a chain's blocks past its own grid return.
+// The per-chain records are collected here on the host and uploaded once.
+struct BatchOp {
+	const void *fn = nullptr;
+	unsigned block = 0, gx = 0, gy = 1;
+	size_t lds = 0, rec = 0, off = 0;     // dynamic LDS (largest chain), record bytes, table offset
+	unsigned n = 0;                       // chains in this launch
+	std::vector<unsigned char> tab;
+	template <typename A>
+	int add(const void *f, unsigned b, size_t l, unsigned x, unsigned y, const A &a)
+	{
+		if(fn && (fn != f || block != b || rec != sizeof(A))){
+			set_error("batch: one launch would mix two kernels");
+			return EPHMC_ERR_ARG;
+		}
+		fn = f;
+		block = b;
+		rec = sizeof(A);
+		lds = std::max(lds, l);
+		gx = std::max(gx, x);
+		gy = std::max(gy, y);
+		const unsigned char *p = reinterpret_cast<const unsigned char *>(&a);
+		tab.insert(tab.end(), p, p + sizeof(A));
+		n++;
+		return EPHMC_OK;
+	}
+};
+
 // timing helpers
 int timer_begin(ephmc_ctx *c, const char *group);
 int timer_end(ephmc_ctx *c, const char *group);
@@ -221,9 +251,19 @@ int add_identity(ephmc_ctx *c, double *M);
 int log_trace(ephmc_ctx *c, int prec, const void *M, int ld_slot, int fs_slot);
 int set_identity(ephmc_ctx *c, int prec, void *M);
 int set_scalar(ephmc_ctx *c, int slot, double v);
+// batch records (threads: workgroup size of the site-form chain, 256 / 512 / 1024 >= ns)
+int batch_coef(BatchOp &op, ephmc_ctx *c, const double *x, bool want32);
+int batch_chain(BatchOp &op, ephmc_ctx *c, int prec, unsigned threads, void *dst, void *ck, unsigned ck_every,
+		int descending);
+int batch_transpose(BatchOp &op, ephmc_ctx *c, int prec, const void *T, double *M);
+int batch_add_identity(BatchOp &op, ephmc_ctx *c, double *M);
+// dst <- src (n doubles); with gate, only when gate[SCAL_ACC] != 0
+int batch_copy(BatchOp &op, const double *src, double *dst, size_t n, const double *gate);
 
 // gj.hip
 int gj_logdet_inverse(ephmc_ctx *c, double *M);
+// the same inversion as batch records: first block, steps[i] = i-th block step, pivoted fallback
+int batch_gj(BatchOp &first, std::vector<BatchOp> &steps, BatchOp &fallback, ephmc_ctx *c, double *M);
 
 // adjoint.hip
 int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, double kick, double *p, const double *fscale);
@@ -231,6 +271,12 @@ int force_constant(ephmc_ctx *c, int prec, const double *minv, double *pdot, con
 int broadcast_slice0(ephmc_ctx *c, double *pdot);
 // LDS bytes of one force-sweep workgroup at C columns (the launch shape used for ns sites)
 size_t adjoint_lds(unsigned ns, int prec, unsigned C);
+// columns per workgroup of the force sweep (the context's choice, else 2 in fp64 and 4 in fp32)
+unsigned adjoint_cols(const ephmc_ctx *c, int prec);
+// force_adjoint(c, prec, minv, pdot, 0, null, null) as batch records: segs[i] = i-th sweep launch,
+// then the final reduction
+int batch_adjoint(std::vector<BatchOp> &segs, BatchOp &reduce, ephmc_ctx *c, int prec, unsigned threads,
+		const double *minv, double *pdot);
 
 // bosons.hip
 int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick);
@@ -244,6 +290,12 @@ int boson_action(ephmc_ctx *c, const double *x, double *scal_out);
 int hamilton_dev(ephmc_ctx *c, const double *x, const double *p, int logdet_slot, int out_slot);
 int kinetic_energy(ephmc_ctx *c, const double *p, double *scal_out);
 int metropolis(ephmc_ctx *c, double uniform);
+// batch records of fa_evolve, fa_filter (momentum refresh), hamilton_dev (five launches, ops[0..4]) and of
+// the Metropolis decision with the uniform read from device memory
+int batch_fa_evolve(BatchOp &op, ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick);
+int batch_fa_filter(BatchOp &op, ephmc_ctx *c, const double *in, double *out);
+int batch_hamilton(BatchOp *ops, ephmc_ctx *c, const double *x, const double *p, int logdet_slot, int out_slot);
+int batch_metropolis(BatchOp &op, ephmc_ctx *c, const double *uniform);
 
 // tl.hip
 void tl_release(ephmc_ctx *c);

@@ -98,6 +98,11 @@ def _load():
         "ephmc_construct_all_QM1P": [vp, vp, vp, vp],
         "ephmc_boson_force": [vp, vp, vp, C.c_int],
         "ephmc_evolve_bosons_approx_step": [vp, vp, vp, C.c_double],
+        "ephmc_batch_create": [C.POINTER(vp), C.c_uint, C.POINTER(vp)],
+        "ephmc_batch_trajectory": [vp, C.POINTER(dp), dp],
+        "ephmc_batch_sync": [vp, C.POINTER(Record)],
+        "ephmc_batch_info": [vp, C.POINTER(C.c_uint), C.POINTER(C.c_uint)],
+        "ephmc_batch_destroy": [vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -399,3 +404,48 @@ class Context:
         ms, calls = C.c_double(), C.c_ulong()
         check(LIB.ephmc_timing_get(self.h, group.encode(), C.byref(ms), C.byref(calls)))
         return ms.value, calls.value
+
+
+class Batch:
+    """Many chains advanced together (ephmc_batch_*): every kernel of a trajectory is launched once
+    over all chains of a group.  The contexts keep their state and can be used alone between batch
+    trajectories (after collect()); close the batch before its contexts."""
+
+    def __init__(self, contexts):
+        self.ctxs = list(contexts)
+        arr = (C.c_void_p * len(self.ctxs))(*[c.h.value for c in self.ctxs])
+        h = C.c_void_p()
+        check(LIB.ephmc_batch_create(arr, len(self.ctxs), C.byref(h)))
+        self.h = h
+
+    def trajectory(self, normals, uniforms, wait=True):
+        """One trajectory of every chain; normals[i] and uniforms[i] as for Context.trajectory.
+        With wait the records are returned, else collect() returns them later."""
+        if len(normals) != len(self.ctxs) or len(uniforms) != len(self.ctxs):
+            raise ValueError("one normals array and one uniform per chain")
+        fields = [c._host_field(n, "normals") for c, n in zip(self.ctxs, normals)]
+        ptrs = (C.POINTER(C.c_double) * len(fields))(*[_dp(f) for f in fields])
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        check(LIB.ephmc_batch_trajectory(self.h, ptrs, _dp(u)))
+        return self.collect() if wait else None
+
+    def collect(self):
+        recs = (Record * len(self.ctxs))()
+        check(LIB.ephmc_batch_sync(self.h, recs))
+        return list(recs)
+
+    def info(self):
+        g, n = C.c_uint(), C.c_uint()
+        check(LIB.ephmc_batch_info(self.h, C.byref(g), C.byref(n)))
+        return {"groups": g.value, "launches": n.value}
+
+    def close(self):
+        if self.h:
+            LIB.ephmc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

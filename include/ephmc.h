@@ -221,6 +221,28 @@ int ephmc_chain_get_logdet(ephmc_ctx *ctx, double *logdet_host);
 /* device pointers of the chain state (valid until ephmc_destroy) */
 int ephmc_chain_buffers(ephmc_ctx *ctx, double **x_dev, double **p_dev, double **minv_dev);
 
+/* ---- many chains in one launch per phase ------------------------------------------------- *
+ * The reference runs one chain per process (organic_hmc.c:296-353, run_hmc).  A batch advances n
+ * chains (contexts after ephmc_chain_init) by one trajectory each, with each kernel of the
+ * trajectory launched once over all chains of a group: chains with the same arithmetic, force
+ * columns per workgroup, MD steps and workgroup class (N <= 256 / 512 / 1024).  Runs the
+ * Fourier-accelerated trajectory (no NO_FOURIER_ACC / CLASSICAL_FOURIER_ACC, stabilize,
+ * SINGLE_PARTICLE, ZERO_FILLING, CONSTANT_FORCE or SCHUR_FORM_INV) at Taylor order 4 for
+ * 33 <= N <= 1024, else EPHMC_ERR_UNSUPPORTED.  Each chain's records and state are bit for bit
+ * those of ephmc_trajectory with the same inputs.  The contexts keep their state: between batch
+ * trajectories (after ephmc_batch_sync) each can be used alone.  Destroy the batch before any of
+ * its contexts. */
+typedef struct ephmc_batch ephmc_batch;
+int ephmc_batch_create(ephmc_ctx *const *ctxs, unsigned n, ephmc_batch **out);
+/* One trajectory of every chain: normals[i] (dim_i values) and uniforms[i] as for
+ * ephmc_trajectory.  Asynchronous (the inputs are copied before it returns). */
+int ephmc_batch_trajectory(ephmc_batch *b, const double *const *normals, const double *uniforms);
+/* Waits for the trajectory in flight; recs (n records, may be null) receive its records. */
+int ephmc_batch_sync(ephmc_batch *b, ephmc_traj_record *recs);
+/* Number of groups and of kernel launches per batch trajectory. */
+int ephmc_batch_info(const ephmc_batch *b, unsigned *groups, unsigned *launches);
+int ephmc_batch_destroy(ephmc_batch *b);
+
 /* ---- measurement -------------------------------------------------------------------------- */
 /* Current-current correlator g_ab(tau) (nt x 3 x 3, tau-major) of field x with
  * M_inv = (1 + B..B)^{-1}; replaces cc_correlator (organic_dense_matrix.c:451-524).  FP64.

@@ -1,0 +1,106 @@
+"""Many chains advanced with one launch per phase (ephmc_batch_*).
+
+Contexts of uneven lattice and Nt (L 8..24, Nt 32..128), FP64 and mixed, with and without
+checkpoints, some autotuned and one with fewer MD steps, run the same random inputs first alone
+(ephmc_trajectory) and then as one batch: every chain's records, final field and M_hat^{-1} are
+bit-identical over several trajectories (the batch runs the single-chain kernels' bodies with each
+chain's own grid for every sum).  The solo path itself is pinned against the oracle by the
+configuration tests.  Modes the batch does not run are refused with EPHMC_ERR_UNSUPPORTED."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from _gpu import ephmc, gpu_context, say
+from _oracle import MT, OracleModel, oracle, ptr
+
+pytestmark = pytest.mark.gpu
+
+# (L, Nt, precision, MD steps, checkpoint k, autotune)
+CHAINS = [(8, 32, 1, 10, 0, False), (8, 64, 0, 10, 0, True), (12, 128, 1, 10, 16, False), (16, 32, 1, 10, 0, True),
+          (20, 64, 1, 10, 0, False), (24, 32, 1, 10, 16, True), (12, 64, 0, 10, 8, False), (16, 128, 1, 6, 0, False)]
+NTRAJ = 3
+
+
+def draws_for(seed, dim, n):
+    lib = oracle()
+    m = MT()
+    lib.orc_mt_seed(C.byref(m), seed)
+    out = []
+    for _ in range(n + 1):
+        v = np.zeros(dim)
+        lib.orc_random_vector(C.byref(m), ptr(v), dim)
+        out.append((v, lib.orc_mt_real2(C.byref(m))))
+    return out
+
+
+def rec_tuple(r):
+    return (r.accepted, r.boltzmann, r.energy, r.logdet, r.dH)
+
+
+def test_batch_equals_solo():
+    E = ephmc()
+    ctxs, inputs = [], []
+    for i, (l, nt, prec, steps, k, tune) in enumerate(CHAINS):
+        om = OracleModel(l, l, nt)
+        ctx = gpu_context(om, steps=steps, checkpoint=k, precision=prec, graph=1)
+        if tune:
+            ctx.autotune()
+        ctxs.append(ctx)
+        inputs.append(draws_for(2000 + i, om.dim, NTRAJ))
+    say("batch: %d contexts" % len(ctxs))
+    solo = []
+    for ctx, d in zip(ctxs, inputs):
+        ctx.chain_init(d[0][0])
+        recs = [rec_tuple(ctx.trajectory(nrm, u)) for nrm, u in d[1:]]
+        solo.append((recs, ctx.chain_x(), ctx.chain_minv()))
+    say("batch: solo runs done")
+    for ctx, d in zip(ctxs, inputs):
+        ctx.chain_init(d[0][0])
+    b = E.Batch(ctxs)
+    info = b.info()
+    say("batch:", info)
+    # groups: (FP64 or mixed) x workgroup class x force columns x MD steps
+    assert info["groups"] >= 4 and info["launches"] > 0
+    got = [[] for _ in ctxs]
+    for t in range(1, NTRAJ + 1):
+        recs = b.trajectory([d[t][0] for d in inputs], [d[t][1] for d in inputs], wait=(t != 2))
+        if recs is None:
+            recs = b.collect()
+        for j, r in enumerate(recs):
+            got[j].append(rec_tuple(r))
+    b.close()
+    for j, ctx in enumerate(ctxs):
+        assert got[j] == solo[j][0], (CHAINS[j], got[j], solo[j][0])
+        np.testing.assert_array_equal(ctx.chain_x(), solo[j][1])
+        np.testing.assert_array_equal(ctx.chain_minv(), solo[j][2])
+    acc = sum(r[0] for g in got for r in g)
+    say("batch: records, fields and inverses equal the solo runs (%d of %d accepted)" % (acc, NTRAJ * len(ctxs)))
+    # the contexts go on alone after the batch
+    for ctx, d in zip(ctxs, inputs):
+        ctx.trajectory(d[1][0], d[1][1])
+    for ctx in ctxs:
+        ctx.close()
+
+
+def test_batch_refuses_unsupported():
+    E = ephmc()
+    om = OracleModel(8, 8, 32)
+    ready = gpu_context(om, steps=4)
+    ready.chain_init(draws_for(7, om.dim, 0)[0][0])
+    fresh = gpu_context(om, steps=4)
+    with pytest.raises(E.EphmcError, match="error -1"):
+        E.Batch([ready, fresh])                 # no chain state yet
+    nofa = gpu_context(om, steps=4, no_fourier_acc=1)
+    nofa.chain_init(draws_for(8, om.dim, 0)[0][0])
+    with pytest.raises(E.EphmcError, match="error -3"):
+        E.Batch([ready, nofa])
+    small = OracleModel(4, 4, 32)
+    tiny = gpu_context(small, steps=4)
+    tiny.chain_init(draws_for(9, small.dim, 0)[0][0])
+    with pytest.raises(E.EphmcError, match="error -3"):
+        E.Batch([tiny])
+    with pytest.raises(E.EphmcError, match="error -1"):
+        E.Batch([ready, ready])                 # one context twice
+    for c in (ready, fresh, nofa, tiny):
+        c.close()
