@@ -468,6 +468,9 @@ def main():
     ap.add_argument("--batch-traj", type=int, default=10)
     ap.add_argument("--c4-only", action="store_true",
                     help="run only the 64-chain legs (streams and one batch) and print their JSON")
+    ap.add_argument("--c4-batch-only", action="store_true",
+                    help="run only the 64-chain batch leg and print its JSON (the workload of its kernel trace, "
+                         "with EPHMC_BATCH_EAGER=1 for plain launches)")
     ap.add_argument("--tl-samples", type=int, default=4096)
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the chains with the trajectory body replayed as a hipGraph (the headline value "
@@ -490,6 +493,10 @@ def main():
         return
     if args.force_only:
         force_only(args.force_only)
+        return
+    if args.c4_batch_only:
+        import ephmc as E
+        print(json.dumps({"chains64_batch": c4_batch_leg(E, args.batch_traj)}), flush=True)
         return
     if args.c4_only:
         import ephmc as E

@@ -159,7 +159,7 @@ __host__ __device__ constexpr size_t adj_lds_bytes(unsigned ns)
 // DEG is the polynomial degree (a.deg), a template constant so that in the fp32 one-site form the
 // phase structure and the S_a recursion need no run-time tests
 template <typename R, int C, int BT, int S, int DEG>
-__device__ __forceinline__ void adjoint_body(const AdjArgs<R> &a, unsigned bx)
+__device__ __forceinline__ void adjoint_body(const AdjArgs<R> &a, unsigned bx, unsigned nthr)
 {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	const unsigned ns = a.ns;
@@ -183,7 +183,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs<R> &a, unsigned bx)
 		const unsigned total = (unsigned)a.red_ntau * 3 * ns;
 		const unsigned per = (total + a.nwg - 1) / a.nwg;
 		reduce_links(a.red_src, a.nwg, 3 * ns, bx * per, per, total, a.red_tau_b, a.f, a.fscale, a.pdot, a.p, a.kick,
-			rscratch, BT, a.clip);
+			rscratch, nthr, a.clip);
 	}
 	R *wbs[2] = {wb0, wb1};
 
@@ -486,16 +486,19 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs<R> &a, unsigned bx)
 template <typename R, int C, int BT, int S, int DEG>
 __global__ __launch_bounds__(BT) void adjoint_kernel(AdjArgs<R> a)
 {
-	adjoint_body<R, C, BT, S, DEG>(a, blockIdx.x);
+	adjoint_body<R, C, BT, S, DEG>(a, blockIdx.x, BT);
 }
 
-// many chains in one launch: blockIdx.z = chain (its record in tab), blockIdx.x = its workgroup
+// many chains in one launch: blockIdx.z = chain (its record in tab), blockIdx.x = its workgroup.
+// With one site per thread the workgroup may have fewer threads than BT (at least ns): the LDS
+// layout stays that of BT, and the reduction's sums do not depend on the thread count.
 template <typename R, int C, int BT, int S, int DEG>
 __global__ __launch_bounds__(BT) void adjoint_multi_kernel(const AdjArgs<R> *__restrict__ tab)
 {
+	static_assert(S == 1, "one site per thread");
 	const AdjArgs<R> &a = tab[blockIdx.z];
 	if(blockIdx.x >= a.nwg) return;
-	adjoint_body<R, C, BT, S, DEG>(a, blockIdx.x);
+	adjoint_body<R, C, BT, S, DEG>(a, blockIdx.x, blockDim.x);
 }
 
 template <typename P>
@@ -708,35 +711,39 @@ int force_adjoint(ephmc_ctx *c, int prec, const double *minv, double *pdot, doub
 }
 
 // ---- many chains in one launch (batch.hip): the sweep of force_adjoint(c, prec, minv, pdot, 0,
-// null, null) with one site per thread on a workgroup of `threads` >= ns; the bits do not depend
-// on the workgroup size (the columns per workgroup, which fix the partial sums, are the context's)
+// null, null) with one site per thread on `block` threads (ns <= block <= threads, the LDS layout's
+// workgroup size); the bits do not depend on the workgroup size (the columns per workgroup, which
+// fix the partial sums, are the context's)
 template <typename R, int C, int BT>
-static int batch_adj_add(BatchOp &op, const AdjArgs<R> &a)
+static int batch_adj_add(BatchOp &op, unsigned block, const AdjArgs<R> &a)
 {
 	const size_t lds = adj_lds_bytes<R, C, BT, 1>(a.ns);
 	if(lds > 160 * 1024){ set_error("batch adjoint: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
 	const void *fn = (const void *)adjoint_multi_kernel<R, C, BT, 1, 4>;
 	(void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max(lds, op.lds));
-	return op.add(fn, BT, lds, a.nwg, 1, a);
+	return op.add(fn, block, lds, a.nwg, 1, a);
 }
 
 template <typename R, int C>
-static int batch_adj_bt(BatchOp &op, unsigned threads, const AdjArgs<R> &a)
+static int batch_adj_bt(BatchOp &op, unsigned threads, unsigned block, const AdjArgs<R> &a)
 {
 	switch(threads){
-		case 256: return batch_adj_add<R, C, 256>(op, a);
-		case 512: return batch_adj_add<R, C, 512>(op, a);
-		case 1024: return batch_adj_add<R, C, 1024>(op, a);
+		case 256: return batch_adj_add<R, C, 256>(op, block, a);
+		case 512: return batch_adj_add<R, C, 512>(op, block, a);
+		case 1024: return batch_adj_add<R, C, 1024>(op, block, a);
 		default: set_error("batch adjoint: %u threads per workgroup", threads); return EPHMC_ERR_ARG;
 	}
 }
 
 template <typename R>
 static int batch_adjoint_r(std::vector<BatchOp> &segs, BatchOp &reduce, ephmc_ctx *c, int prec, unsigned threads,
-		const double *minv, double *pdot)
+		unsigned block, const double *minv, double *pdot)
 {
 	if(c->poly.deg != 4){ set_error("batch: polynomial degree %d (the batch runs Taylor order 4)", c->poly.deg); return EPHMC_ERR_UNSUPPORTED; }
-	if(c->ns > threads){ set_error("batch adjoint: %u sites on %u threads", c->ns, threads); return EPHMC_ERR_ARG; }
+	if(c->ns > block || block > threads || block % 64){
+		set_error("batch adjoint: %u sites on %u of %u threads", c->ns, block, threads);
+		return EPHMC_ERR_ARG;
+	}
 	const unsigned nt = c->nt, span = c->span, C = adjoint_cols(c, prec);
 	AdjArgs<R> a = adj_base_args<R>(c, prec, C, pdot, nullptr, 0, nullptr);
 	const unsigned nseg = (nt + span - 1) / span;
@@ -748,9 +755,9 @@ static int batch_adjoint_r(std::vector<BatchOp> &segs, BatchOp &reduce, ephmc_ct
 		BatchOp &op = segs[nseg - 1 - s];
 		int rc;
 		switch(C){
-			case 1: rc = batch_adj_bt<R, 1>(op, threads, a); break;
-			case 2: rc = batch_adj_bt<R, 2>(op, threads, a); break;
-			default: rc = batch_adj_bt<R, 4>(op, threads, a); break;
+			case 1: rc = batch_adj_bt<R, 1>(op, threads, block, a); break;
+			case 2: rc = batch_adj_bt<R, 2>(op, threads, block, a); break;
+			default: rc = batch_adj_bt<R, 4>(op, threads, block, a); break;
 		}
 		if(rc != EPHMC_OK) return rc;
 		prev = a.partial;
@@ -771,10 +778,10 @@ static int batch_adjoint_r(std::vector<BatchOp> &segs, BatchOp &reduce, ephmc_ct
 }
 
 int batch_adjoint(std::vector<BatchOp> &segs, BatchOp &reduce, ephmc_ctx *c, int prec, unsigned threads,
-		const double *minv, double *pdot)
+		unsigned block, const double *minv, double *pdot)
 {
-	if(prec == EPHMC_FP64) return batch_adjoint_r<double>(segs, reduce, c, prec, threads, minv, pdot);
-	return batch_adjoint_r<float>(segs, reduce, c, prec, threads, minv, pdot);
+	if(prec == EPHMC_FP64) return batch_adjoint_r<double>(segs, reduce, c, prec, threads, block, minv, pdot);
+	return batch_adjoint_r<float>(segs, reduce, c, prec, threads, block, minv, pdot);
 }
 
 // CONSTANT_FORCE (organic_hmc.c:92-98 with construct_QM1P at time 0, organic_dense_matrix.c:267-309):

@@ -3,8 +3,8 @@
 //
 // The reference runs one chain per process (organic_hmc.c:296-353).  Here the chains of uneven
 // lattice and Nt are split into groups of chains that share the arithmetic, the force sweep's
-// columns per workgroup (which fix its partial sums), the workgroup size class of the site-form
-// kernels (256, 512 or 1024 threads, the smallest that holds N sites) and the number of MD steps.
+// columns per workgroup (which fix its partial sums), N rounded up to a wave (the site-form
+// kernels run one site per thread, on that many threads) and the number of MD steps.
 // Every kernel of a trajectory runs once per group with workgroups (x, y, z) = block (x, y) of
 // chain z: the argument records of all chains sit in device tables built once (every pointer is
 // fixed per context), and a chain's blocks past its own grid return.  Phases whose launch count
@@ -25,7 +25,8 @@ using namespace ephmc;
 struct ephmc_batch {
 	struct Group {
 		std::vector<unsigned> idx;        // chains (positions in the batch)
-		unsigned threads = 0;
+		unsigned threads = 0;             // LDS layout of the site-form kernels (256 / 512 / 1024)
+		unsigned block = 0;               // threads launched: N rounded up to a wave
 		std::vector<BatchOp> ops;
 		std::vector<unsigned> seq;        // the trajectory: op indices in launch order
 		hipStream_t stream = nullptr;
@@ -46,6 +47,7 @@ struct ephmc_batch {
 	hipStream_t lead = nullptr;           // inputs in, records out
 	hipEvent_t up = nullptr, fin = nullptr;
 	bool pending = false;
+	bool eager = false;                   // EPHMC_BATCH_EAGER=1: plain launches instead of the graphs (profiling)
 	int device = 0;
 };
 
@@ -101,14 +103,14 @@ int plan_group(Batch &b, Batch::Group &g)
 		EPHMC_TRY(batch_hamilton(&ops[ham_old[0]], c, c->d_xnew, c->d_p, SCAL_LOGDET_CHAIN, SCAL_H_OLD));
 		EPHMC_TRY(batch_fa_evolve(ops[evolve_half], c, c->d_xnew, c->d_p, 0.5 * h, nullptr, 0));
 		EPHMC_TRY(batch_coef(ops[coef_md], c, c->d_xnew, prec != EPHMC_FP64));
-		EPHMC_TRY(batch_chain(ops[chain_md], c, prec, g.threads, tfin, c->d_ck, c->ckpt, 0));
+		EPHMC_TRY(batch_chain(ops[chain_md], c, prec, g.threads, g.block, tfin, c->d_ck, c->ckpt, 0));
 		EPHMC_TRY(batch_transpose(ops[trans], c, prec, tfin, c->d_mhat));
 		EPHMC_TRY(batch_gj(ops[gj_first], gj_steps, ops[gj_fb], c, c->d_mhat));
-		EPHMC_TRY(batch_adjoint(adj_segs, ops[adj_red], c, prec, g.threads, c->d_mhat, c->d_pdot));
+		EPHMC_TRY(batch_adjoint(adj_segs, ops[adj_red], c, prec, g.threads, g.block, c->d_mhat, c->d_pdot));
 		EPHMC_TRY(batch_fa_evolve(ops[evolve_kick], c, c->d_xnew, c->d_p, h, c->d_pdot, h));
 		EPHMC_TRY(batch_fa_evolve(ops[evolve_last], c, c->d_xnew, c->d_p, 0.5 * h, c->d_pdot, h));
 		EPHMC_TRY(batch_coef(ops[coef_ex], c, c->d_xnew, false));
-		EPHMC_TRY(batch_chain(ops[chain_ex], c, EPHMC_FP64, g.threads, c->d_mhat, nullptr, 0, 1));
+		EPHMC_TRY(batch_chain(ops[chain_ex], c, EPHMC_FP64, g.threads, g.block, c->d_mhat, nullptr, 0, 1));
 		EPHMC_TRY(batch_add_identity(ops[addid], c, c->d_mhat));
 		EPHMC_TRY(batch_hamilton(&ops[ham_new[0]], c, c->d_xnew, c->d_p, SCAL_LOGDET, SCAL_H_NEW));
 		EPHMC_TRY(batch_metropolis(ops[metro], c, b.d_in + b.in_off[i] + c->dim));
@@ -209,6 +211,8 @@ void release(Batch *b)
 int create(ephmc_ctx *const *ctxs, unsigned n, Batch *b)
 {
 	b->ch.assign(ctxs, ctxs + n);
+	const char *eager = getenv("EPHMC_BATCH_EAGER");
+	b->eager = eager && atoi(eager) == 1;
 	b->device = ctxs[0]->device;
 	for(unsigned i = 0; i < n; i++){
 		if(!ctxs[i]){ set_error("batch: chain %u is null", i); return EPHMC_ERR_ARG; }
@@ -243,12 +247,12 @@ int create(ephmc_ctx *const *ctxs, unsigned n, Batch *b)
 	for(unsigned i = 0; i < n; i++){
 		ephmc_ctx *c = ctxs[i];
 		const int prec = c->cfg.precision;
-		const unsigned thr = threads_for(c->ns), cols = adjoint_cols(c, prec);
+		const unsigned thr = threads_for(c->ns), blk = (c->ns + 63) / 64 * 64, cols = adjoint_cols(c, prec);
 		Batch::Group *g = nullptr;
 		for(Batch::Group &q : b->groups){
 			ephmc_ctx *r = ctxs[q.idx[0]];
-			if(r->cfg.precision == prec && q.threads == thr && adjoint_cols(r, prec) == cols && r->cfg.steps == c->cfg.steps
-					&& (!by_nt || r->nt == c->nt)){
+			if(r->cfg.precision == prec && q.threads == thr && q.block == blk && adjoint_cols(r, prec) == cols
+					&& r->cfg.steps == c->cfg.steps && (!by_nt || r->nt == c->nt)){
 				g = &q;
 				break;
 			}
@@ -257,6 +261,7 @@ int create(ephmc_ctx *const *ctxs, unsigned n, Batch *b)
 			b->groups.emplace_back();
 			g = &b->groups.back();
 			g->threads = thr;
+			g->block = blk;
 		}
 		g->idx.push_back(i);
 	}
@@ -277,7 +282,7 @@ int create(ephmc_ctx *const *ctxs, unsigned n, Batch *b)
 		EPHMC_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
 		EPHMC_HIP(hipEventCreateWithFlags(&g.ready, hipEventDisableTiming));
 		EPHMC_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
-		EPHMC_TRY(capture(*b, g));
+		if(!b->eager) EPHMC_TRY(capture(*b, g));
 	}
 	return EPHMC_OK;
 }
@@ -326,7 +331,8 @@ int ephmc_batch_trajectory(ephmc_batch *b, const double *const *normals, const d
 	EPHMC_HIP(hipEventRecord(b->up, b->lead));
 	for(auto &g : b->groups){
 		EPHMC_HIP(hipStreamWaitEvent(g.stream, b->up, 0));
-		EPHMC_HIP(hipGraphLaunch(g.exec, g.stream));
+		if(g.exec) EPHMC_HIP(hipGraphLaunch(g.exec, g.stream));
+		else EPHMC_TRY(launch_seq(*b, g));
 		EPHMC_HIP(hipEventRecord(g.done, g.stream));
 		EPHMC_HIP(hipStreamWaitEvent(b->lead, g.done, 0));
 	}

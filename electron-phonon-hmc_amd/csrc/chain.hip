@@ -242,7 +242,8 @@ __global__ __launch_bounds__(BT) void chain_kernel(ChainArgs<R> a)
 }
 
 // many chains in one launch: blockIdx.z = chain, blockIdx.x = its column block (blocks past the
-// chain's own ceil(ns / C) return)
+// chain's own ceil(ns / C) return).  With one site per thread the workgroup may have fewer threads
+// than BT (at least ns); the LDS layout stays that of BT.
 template <typename R, int C, int BT, int S>
 __global__ __launch_bounds__(BT) void chain_multi_kernel(const ChainArgs<R> *__restrict__ tab)
 {
@@ -701,8 +702,9 @@ int convert_f64_to_f32(ephmc_ctx *c, const double *a, float *b, size_t n)
 }
 
 // ---- many chains in one launch (batch.hip).  The chain runs in the site form with 2 columns per
-// workgroup; every column is computed with the arithmetic of every other form and width, so a
-// chain gives the same bits in a batch as alone.
+// workgroup on `block` threads (ns <= block <= threads, the LDS layout's workgroup size); every
+// column is computed with the arithmetic of every other form and width, so a chain gives the same
+// bits in a batch as alone.
 int batch_coef(BatchOp &op, ephmc_ctx *c, const double *x, bool want32)
 {
 	CoefArgs a{};
@@ -721,33 +723,37 @@ int batch_coef(BatchOp &op, ephmc_ctx *c, const double *x, bool want32)
 constexpr int BATCH_CHAIN_C = 2;
 
 template <typename R, int BT>
-static int batch_chain_t(BatchOp &op, const ChainArgs<R> &a)
+static int batch_chain_t(BatchOp &op, unsigned block, const ChainArgs<R> &a)
 {
 	const size_t lds = chain_lds_bytes<R, BATCH_CHAIN_C, BT, 1>(a.ns);
 	if(lds > 160 * 1024){ set_error("batch chain: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
 	const void *fn = (const void *)chain_multi_kernel<R, BATCH_CHAIN_C, BT, 1>;
 	(void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max(lds, op.lds));
-	return op.add(fn, BT, lds, (a.ns + BATCH_CHAIN_C - 1) / BATCH_CHAIN_C, 1, a);
+	return op.add(fn, block, lds, (a.ns + BATCH_CHAIN_C - 1) / BATCH_CHAIN_C, 1, a);
 }
 
 template <typename R>
-static int batch_chain_r(BatchOp &op, unsigned threads, const ChainArgs<R> &a)
+static int batch_chain_r(BatchOp &op, unsigned threads, unsigned block, const ChainArgs<R> &a)
 {
-	if(a.ns > threads){ set_error("batch chain: %u sites on %u threads", a.ns, threads); return EPHMC_ERR_ARG; }
+	if(a.ns > block || block > threads || block % 64){
+		set_error("batch chain: %u sites on %u of %u threads", a.ns, block, threads);
+		return EPHMC_ERR_ARG;
+	}
 	switch(threads){
-		case 256: return batch_chain_t<R, 256>(op, a);
-		case 512: return batch_chain_t<R, 512>(op, a);
-		case 1024: return batch_chain_t<R, 1024>(op, a);
+		case 256: return batch_chain_t<R, 256>(op, block, a);
+		case 512: return batch_chain_t<R, 512>(op, block, a);
+		case 1024: return batch_chain_t<R, 1024>(op, block, a);
 		default: set_error("batch chain: %u threads per workgroup", threads); return EPHMC_ERR_ARG;
 	}
 }
 
-int batch_chain(BatchOp &op, ephmc_ctx *c, int prec, unsigned threads, void *dst, void *ck, unsigned ck_every,
-		int descending)
+int batch_chain(BatchOp &op, ephmc_ctx *c, int prec, unsigned threads, unsigned block, void *dst, void *ck,
+		unsigned ck_every, int descending)
 {
 	if(prec == EPHMC_FP64)
-		return batch_chain_r<double>(op, threads, forward_args<double>(c, c->d_coef64, nullptr, dst, ck, ck_every, descending, true));
-	return batch_chain_r<float>(op, threads, forward_args<float>(c, c->d_coef32, nullptr, dst, ck, ck_every, descending, true));
+		return batch_chain_r<double>(op, threads, block,
+			forward_args<double>(c, c->d_coef64, nullptr, dst, ck, ck_every, descending, true));
+	return batch_chain_r<float>(op, threads, block, forward_args<float>(c, c->d_coef32, nullptr, dst, ck, ck_every, descending, true));
 }
 
 int batch_transpose(BatchOp &op, ephmc_ctx *c, int prec, const void *T, double *M)

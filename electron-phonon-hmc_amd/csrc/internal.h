@@ -251,10 +251,11 @@ int add_identity(ephmc_ctx *c, double *M);
 int log_trace(ephmc_ctx *c, int prec, const void *M, int ld_slot, int fs_slot);
 int set_identity(ephmc_ctx *c, int prec, void *M);
 int set_scalar(ephmc_ctx *c, int slot, double v);
-// batch records (threads: workgroup size of the site-form chain, 256 / 512 / 1024 >= ns)
+// batch records (threads: workgroup size of the site-form kernels' LDS layout, 256 / 512 / 1024;
+// block: threads launched, a multiple of 64 with ns <= block <= threads)
 int batch_coef(BatchOp &op, ephmc_ctx *c, const double *x, bool want32);
-int batch_chain(BatchOp &op, ephmc_ctx *c, int prec, unsigned threads, void *dst, void *ck, unsigned ck_every,
-		int descending);
+int batch_chain(BatchOp &op, ephmc_ctx *c, int prec, unsigned threads, unsigned block, void *dst, void *ck,
+		unsigned ck_every, int descending);
 int batch_transpose(BatchOp &op, ephmc_ctx *c, int prec, const void *T, double *M);
 int batch_add_identity(BatchOp &op, ephmc_ctx *c, double *M);
 // dst <- src (n doubles); with gate, only when gate[SCAL_ACC] != 0
@@ -276,7 +277,7 @@ unsigned adjoint_cols(const ephmc_ctx *c, int prec);
 // force_adjoint(c, prec, minv, pdot, 0, null, null) as batch records: segs[i] = i-th sweep launch,
 // then the final reduction
 int batch_adjoint(std::vector<BatchOp> &segs, BatchOp &reduce, ephmc_ctx *c, int prec, unsigned threads,
-		const double *minv, double *pdot);
+		unsigned block, const double *minv, double *pdot);
 
 // bosons.hip
 int fa_evolve(ephmc_ctx *c, double *x, double *p, double h, const double *pdot, double kick);
