@@ -42,9 +42,11 @@ __device__ inline double rcp_nr(double x)
 // columns g..g+3): with P = L_P U_P (no exchanges), every column's new pivot rows are
 // nr = P^{-1} u (u: the column's entries in the pivot rows) by forward and back substitution,
 // and every other row r -= a_r,[g..g+3] nr.  Wave w holds rows 8w..8w+7 in a[], lane c column c
-// (lanes 32..63 end with D^{-1}); one barrier per step publishes the four pivot rows and the four
-// pivot columns.  P's factorisation is formed by every thread (the same values everywhere, so no
-// second barrier is needed to share it).  After the loop the pivots U_gg and the LU multipliers of
+// (lanes 32..63 end with D^{-1}); one barrier per step publishes the four pivot rows, from which
+// every thread forms P and its factorisation (the same values everywhere, so no second barrier is
+// needed to share it).  A row's entries in the pivot columns (a_r,[g..g+3]) are read from the
+// lanes g..g+3 of its own wave.  Each lane c < 32 keeps its column as it stood at the step that
+// pivots on it; the kept columns go to LDS after the loop.  After the loop the pivots U_gg and the LU multipliers of
 // D are formed from the kept pivot columns: row r's multipliers of step s are
 // l[r][g+i] = (a_r,g+i - sum_{q<i} l[r][g+q] U_P[q][i]) / U_P[i][i]  (they only feed the
 // |l| <= 1 check).  Writes dinv_out[r][c] = D^{-1}[r][c] and lm_out[m][j] = L[m][j] (unit lower
@@ -56,21 +58,11 @@ struct GjInvLds {
 	                                                // multiplier pass reads 8 steps at once)
 };
 
-// L and U of the 4 x 4 pivot block of step s (P[m][i] = row g+m, column g+i) without exchanges:
+// L and U of the 4 x 4 pivot block P (P[m][i] = row g+m, column g+i) without exchanges:
 // lo[m][k] (m > k) the multipliers, up[k][i] (i > k) U, rp[k] = 1 / U[k][k], pv[k] = U[k][k]
-__device__ __forceinline__ void gj_lu4(const GjInvLds &sh, int s, double (&lo)[4][4], double (&up)[4][4], double (&rp)[4],
+__device__ __forceinline__ void gj_lu4_of(double (&M)[4][4], double (&lo)[4][4], double (&up)[4][4], double (&rp)[4],
 		double (&pv)[4])
 {
-	const int g = 4 * s;
-	double M[4][4];
-#pragma unroll
-	for(int i = 0; i < 4; i++){
-		const double2 x0 = *(const double2 *)&sh.pcs[s][i][g], x1 = *(const double2 *)&sh.pcs[s][i][g + 2];
-		M[0][i] = x0.x;
-		M[1][i] = x0.y;
-		M[2][i] = x1.x;
-		M[3][i] = x1.y;
-	}
 #pragma unroll
 	for(int k = 0; k < 4; k++){
 		pv[k] = M[k][k];
@@ -87,11 +79,56 @@ __device__ __forceinline__ void gj_lu4(const GjInvLds &sh, int s, double (&lo)[4
 	}
 }
 
+// P of step s from the kept pivot columns
+__device__ __forceinline__ void gj_lu4(const GjInvLds &sh, int s, double (&lo)[4][4], double (&up)[4][4], double (&rp)[4],
+		double (&pv)[4])
+{
+	const int g = 4 * s;
+	double M[4][4];
+#pragma unroll
+	for(int i = 0; i < 4; i++){
+		const double2 x0 = *(const double2 *)&sh.pcs[s][i][g], x1 = *(const double2 *)&sh.pcs[s][i][g + 2];
+		M[0][i] = x0.x;
+		M[1][i] = x0.y;
+		M[2][i] = x1.x;
+		M[3][i] = x1.y;
+	}
+	gj_lu4_of(M, lo, up, rp, pv);
+}
+
+// P of the current step from its published pivot rows (the same values as its pivot columns)
+__device__ __forceinline__ void gj_lu4_rows(const GjInvLds &sh, int b, int g, double (&lo)[4][4], double (&up)[4][4],
+		double (&rp)[4], double (&pv)[4])
+{
+	double M[4][4];
+#pragma unroll
+	for(int m = 0; m < 4; m++){
+		const double2 x0 = *(const double2 *)&sh.prow[b][m][g], x1 = *(const double2 *)&sh.prow[b][m][g + 2];
+		M[m][0] = x0.x;
+		M[m][1] = x0.y;
+		M[m][2] = x1.x;
+		M[m][3] = x1.y;
+	}
+	gj_lu4_of(M, lo, up, rp, pv);
+}
+
+// the value of v in lane `src` of this wave (src uniform across the wave)
+__device__ __forceinline__ double lane_value(double v, int src)
+{
+	const long long b = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffLL), src);
+	const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, GjInvLds &sh, double *__restrict__ dinv_out,
 		double *__restrict__ lm_out, double *__restrict__ diag, int *__restrict__ piv, unsigned kb, double *__restrict__ scal)
 {
 	constexpr int NB = GJ_NB;
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+	double kept[8];
+#pragma unroll
+	for(int q = 0; q < 8; q++) kept[q] = 0;
 	for(int s = 0; s < NB / 4; s++){
 		const int g = 4 * s, b = s & 1, gq = g & 7;
 		if(g >= (int)nbe) break;
@@ -104,14 +141,13 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 					for(int i = 0; i < 4; i++) sh.prow[b][i][lane] = a[q + i];
 				}
 		}
-		if(lane >= g && lane < g + 4){
-			double *dst = &sh.pcs[s][lane - g][8 * wave];
+		if((lane >> 2) == s){
 #pragma unroll
-			for(int q = 0; q < 8; q += 2) *(double2 *)(dst + q) = make_double2(a[q], a[q + 1]);
+			for(int q = 0; q < 8; q++) kept[q] = a[q];
 		}
 		__syncthreads();
 		double lo[4][4], up[4][4], rp[4], pv[4];
-		gj_lu4(sh, s, lo, up, rp, pv);
+		gj_lu4_rows(sh, b, g, lo, up, rp, pv);
 		// (no store of U_P or the pivots here: a store inside the loop delays the next barrier; they
 		// are formed again from the kept pivot columns after the loop)
 		// new pivot rows of this column: y = L^{-1} u, nr = U^{-1} y
@@ -137,11 +173,7 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 #pragma unroll
 		for(int i = 0; i < 4; i++)
 #pragma unroll
-			for(int q = 0; q < 8; q += 2){
-				const double2 x = *(const double2 *)&sh.pcs[s][i][8 * wave + q];
-				c[q][i] = x.x;
-				c[q + 1][i] = x.y;
-			}
+			for(int q = 0; q < 8; q++) c[q][i] = lane_value(a[q], g + i);
 #pragma unroll
 		for(int q = 0; q < 8; q++){
 			const int r = 8 * wave + q;
@@ -155,6 +187,12 @@ __device__ __forceinline__ void gj_invert_block(double (&a)[8], unsigned nbe, Gj
 #pragma unroll
 		for(int q = 0; q < 8; q++) dinv_out[(8 * wave + q) * NB + lane - NB] = a[q];
 	}
+	if(lane < (int)nbe){
+		double *dst = &sh.pcs[lane >> 2][lane & 3][8 * wave];
+#pragma unroll
+		for(int q = 0; q < 8; q += 2) *(double2 *)(dst + q) = make_double2(kept[q], kept[q + 1]);
+	}
+	__syncthreads();
 	// the pivots and the multipliers: thread column j refactors the pivot block of its step s
 	// (the same arithmetic as inside the loop, so the same values) and takes row j & 3 of U_P
 	bool bad = false;
