@@ -55,6 +55,7 @@ struct ephmc_batch {
 	hipStream_t lead = nullptr;           // inputs in, records out
 	hipEvent_t up = nullptr, fin = nullptr;
 	bool pending = false;
+	bool counted = false;                 // batch_refs of the contexts raised
 	bool eager = false;                   // EPHMC_BATCH_EAGER=1: plain launches instead of the graphs (profiling)
 	int device = 0;
 };
@@ -215,6 +216,8 @@ void release(Batch *b)
 	if(b->d_rec) (void)hipFree(b->d_rec);
 	if(b->d_in) (void)hipFree(b->d_in);
 	if(b->d_tab) (void)hipFree(b->d_tab);
+	if(b->counted)
+		for(ephmc_ctx *c : b->ch) c->batch_refs--;
 	delete b;
 }
 
@@ -232,6 +235,8 @@ int create(ephmc_ctx *const *ctxs, unsigned n, Batch *b)
 		EPHMC_TRY(check_chain(ctxs[i], i));
 	}
 	EPHMC_HIP(hipSetDevice(b->device));
+	for(ephmc_ctx *c : b->ch) c->batch_refs++;
+	b->counted = true;
 	// inputs: normals then the uniform, per chain
 	b->in_off.resize(n);
 	for(unsigned i = 0; i < n; i++){

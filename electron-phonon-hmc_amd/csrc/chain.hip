@@ -720,16 +720,26 @@ int batch_coef(BatchOp &op, ephmc_ctx *c, const double *x, bool want32)
 	return op.add((const void *)coef_multi_kernel, 256, 0, a.gx, 1, a);
 }
 
-constexpr int BATCH_CHAIN_C = 2;
+template <typename R, int BT, int C>
+static int batch_chain_c(BatchOp &op, unsigned block, const ChainArgs<R> &a)
+{
+	const size_t lds = chain_lds_bytes<R, C, BT, 1>(a.ns);
+	if(lds > 160 * 1024){ set_error("batch chain: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
+	const void *fn = (const void *)chain_multi_kernel<R, C, BT, 1>;
+	(void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max(lds, op.lds));
+	return op.add(fn, block, lds, (a.ns + C - 1) / C, 1, a);
+}
 
+// columns per workgroup: 2, or EPHMC_BATCH_CHAIN_COLS = 1 / 4 for tuning sweeps (any width gives the
+// same bits)
 template <typename R, int BT>
 static int batch_chain_t(BatchOp &op, unsigned block, const ChainArgs<R> &a)
 {
-	const size_t lds = chain_lds_bytes<R, BATCH_CHAIN_C, BT, 1>(a.ns);
-	if(lds > 160 * 1024){ set_error("batch chain: %zu B of LDS exceeds 160 KiB", lds); return EPHMC_ERR_ARG; }
-	const void *fn = (const void *)chain_multi_kernel<R, BATCH_CHAIN_C, BT, 1>;
-	(void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max(lds, op.lds));
-	return op.add(fn, block, lds, (a.ns + BATCH_CHAIN_C - 1) / BATCH_CHAIN_C, 1, a);
+	const char *e = getenv("EPHMC_BATCH_CHAIN_COLS");
+	const int cols = e ? atoi(e) : 2;
+	if(cols == 1) return batch_chain_c<R, BT, 1>(op, block, a);
+	if(cols == 4) return batch_chain_c<R, BT, 4>(op, block, a);
+	return batch_chain_c<R, BT, 2>(op, block, a);
 }
 
 template <typename R>

@@ -452,6 +452,7 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 int ephmc_destroy(ephmc_ctx *c)
 {
 	if(!c) return EPHMC_OK;
+	if(c->batch_refs){ set_error("the context belongs to a batch: destroy the batch first"); return EPHMC_ERR_ARG; }
 	if(c->stream) (void)hipStreamSynchronize(c->stream);
 	void *bufs[] = {c->d_nb, c->d_link, c->d_coef64, c->d_coef32, c->d_ck, c->d_seg, c->d_partial[0], c->d_partial[1], c->d_Tfin,
 		c->d_Tfin32, c->d_gj[0], c->d_gj[1], c->d_piv, c->d_src, c->d_diag, c->d_dlu, c->d_scal, c->d_twiddle, c->d_f2, c->d_f2mom, c->d_red,
@@ -889,6 +890,8 @@ extern "C" {
 int ephmc_autotune(ephmc_ctx *c, ephmc_tune_result *res)
 {
 	if(!c){ set_error("null context"); return EPHMC_ERR_ARG; }
+	// a batch holds the context's buffers and launch shapes, which tuning may replace
+	if(c->batch_refs){ set_error("the context belongs to a batch: tune it before ephmc_batch_create"); return EPHMC_ERR_ARG; }
 	const int prec = c->cfg.precision;
 	const unsigned ns = c->ns;
 	EPHMC_HIP(hipStreamSynchronize(c->stream));

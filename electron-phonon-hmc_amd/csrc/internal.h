@@ -169,6 +169,7 @@ struct ephmc_ctx {
 	hipEvent_t stage_ev[2];
 	int stage_next;
 	bool chain_ready;
+	unsigned batch_refs;              // batches holding this context's buffers (ephmc_batch_create)
 
 	// captured trajectory body (ephmc_config.graph)
 	hipGraph_t graph;

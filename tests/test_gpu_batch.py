@@ -5,7 +5,8 @@ checkpoints, some autotuned and one with fewer MD steps, run the same random inp
 (ephmc_trajectory) and then as one batch: every chain's records, final field and M_hat^{-1} are
 bit-identical over several trajectories (the batch runs the single-chain kernels' bodies with each
 chain's own grid for every sum).  The solo path itself is pinned against the oracle by the
-configuration tests.  Modes the batch does not run are refused with EPHMC_ERR_UNSUPPORTED."""
+configuration tests.  Modes the batch does not run are refused with EPHMC_ERR_UNSUPPORTED, and a
+context held by a batch can be neither tuned nor destroyed until the batch is."""
 import ctypes as C
 
 import numpy as np
@@ -102,5 +103,14 @@ def test_batch_refuses_unsupported():
         E.Batch([tiny])
     with pytest.raises(E.EphmcError, match="error -1"):
         E.Batch([ready, ready])                 # one context twice
-    for c in (ready, fresh, nofa, tiny):
+    # a context held by a batch can be neither tuned nor destroyed
+    other = gpu_context(om, steps=4)
+    other.chain_init(draws_for(10, om.dim, 0)[0][0])
+    b = E.Batch([ready, other])
+    with pytest.raises(E.EphmcError, match="belongs to a batch"):
+        ready.autotune()
+    assert E.LIB.ephmc_destroy(other.h) == -1
+    b.close()
+    ready.autotune()
+    for c in (ready, fresh, nofa, tiny, other):
         c.close()
