@@ -267,10 +267,56 @@ __global__ __launch_bounds__(BT) void tl_tridiag_kernel(double *__restrict__ Aal
 //     column j:   a = A[:, j] - sum_i (v_i w_i[j] + w_i v_i[j]),
 //     product:    A' v = A v - V (W^T v) - W (V^T v),     w = tau A' v - (tau^2 / 2)(v^T A' v) v,
 // so each step reads the trailing block once and writes nothing; one update
-// A -= V W^T + W V^T per panel writes it.  Thread r keeps row r of the panel's V and W in registers
+// A -= V W^T + W V^T per panel writes it.  Only the lower triangle (row >= column) is read and
+// written: wave w owns rows 64w .. 64w + 63 and walks its row block's lower part in tiles of 64 x 8;
+// lane r adds A[r][c] v[c] to its row's product and A[r][c] v[r] goes to column c's, whose 8 sums
+// over the wave's rows a butterfly reduce-scatter forms (10 lane exchanges per tile) and an LDS
+// triangle of per-wave column sums holds; each row then adds the column sums of the waves below it
+// in wave order, a fixed order.  Thread r keeps row r of the panel's V and W in registers
 // (v_i[r], w_i[r]); row j of them goes through LDS when column j needs it, and 64 rows at a time
 // when the panel's update needs them.  Reflectors, d, e and tau come out as from tl_tridiag_kernel.
 constexpr int TL_PNB = 8;        // reflectors per panel
+
+// LDS doubles of the panel kernel for n rows on BT threads: vb, block sums, row j of V and W, wave
+// dot partials, their sums, the update's rows of V and W, and the triangle of per-wave column sums
+template <int BT, int NB>
+constexpr size_t tl_panel_lds_doubles(unsigned n)
+{
+	constexpr size_t NW = BT / 64;
+	return (size_t)n + NW + 2 * NB + NW * 2 * NB + 2 * NB + 64 * 2 * NB + 64 * NW * (NW + 1) / 2;
+}
+
+// Sums over the 64 lanes of a wave of x[0..7] (one value per lane each): returns, in lane l, the
+// sum of x[l >> 3] (the same in the eight lanes that differ from l in bits 0..2).  Butterfly
+// reduce-scatter: at each exchange a lane keeps half of its columns and adds its partner's half,
+// in a fixed order.
+__device__ __forceinline__ double tl_reduce8(const double (&x)[8])
+{
+	const unsigned lane = threadIdx.x & 63;
+	double a[4], b[2], d;
+	const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+#pragma unroll
+	for(int i = 0; i < 4; i++){
+		const double send = h5 ? x[i] : x[4 + i];
+		const double keep = h5 ? x[4 + i] : x[i];
+		a[i] = keep + __shfl_xor(send, 32, 64);
+	}
+#pragma unroll
+	for(int i = 0; i < 2; i++){
+		const double send = h4 ? a[i] : a[2 + i];
+		const double keep = h4 ? a[2 + i] : a[i];
+		b[i] = keep + __shfl_xor(send, 16, 64);
+	}
+	{
+		const double send = h3 ? b[0] : b[1];
+		const double keep = h3 ? b[1] : b[0];
+		d = keep + __shfl_xor(send, 8, 64);
+	}
+	d += __shfl_xor(d, 4, 64);
+	d += __shfl_xor(d, 2, 64);
+	d += __shfl_xor(d, 1, 64);
+	return d;
+}
 
 template <int BT, int RPT, int NB>
 __global__ __launch_bounds__(BT) void tl_tridiag_panel_kernel(double *__restrict__ Aall, unsigned n, double *__restrict__ dall,
@@ -284,6 +330,8 @@ __global__ __launch_bounds__(BT) void tl_tridiag_panel_kernel(double *__restrict
 	double *dots = rowvw + 2 * NB;           // NW x 2 NB wave partials
 	double *dsum = dots + NW * 2 * NB;       // 2 NB: W^T v, V^T v
 	double *chunk = dsum + 2 * NB;           // 64 x 2 NB: rows of V and W for the panel update
+	double *zsum = chunk + 64 * 2 * NB;      // wave w's column sums at 64 w (w + 1) / 2 + column
+	static_assert(RPT == 1, "the lower-triangle product holds one row per thread");
 	const unsigned s = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	double *A = Aall + (size_t)s * n * n, *d = dall + (size_t)s * n, *e = eall + (size_t)s * n, *tau = tall + (size_t)s * n;
 	if(n <= 2){
@@ -331,32 +379,53 @@ __global__ __launch_bounds__(BT) void tl_tridiag_panel_kernel(double *__restrict
 			}
 			__syncthreads();
 			const double tj = tl_reflector<BT, RPT>(A, n, j, vb, red, e, tau);
-			// A v over the panel-start trailing block, and the panel's W^T v, V^T v
-			double acc[RPT], pw[NB], pv[NB];
+			// A v over the panel-start trailing block from its lower triangle, and the panel's
+			// W^T v, V^T v.  Wave `wave` walks the tiles of its rows left of and on the diagonal;
+			// lanes past n or above j + 1 load nothing and add zeros.
+			double acc[RPT];
+			{
+				const unsigned r = tid;
+				const bool live = r >= j + 1 && r < n;
+				const double vr = live ? vb[r] : 0.0;
+				double a0 = 0, a1 = 0;
+				if(64 * wave + 63 >= j + 1 && 64 * wave < n){
+					double *zw = zsum + 32 * (size_t)wave * (wave + 1);
+					for(unsigned c0 = (j + 1) & ~7u; c0 < 64 * wave + 64; c0 += 8){
+						double x[8];
+#pragma unroll
+						for(int k = 0; k < 8; k++){
+							const unsigned c = c0 + k;
+							x[k] = (live && c >= j + 1 && c <= r) ? A[(size_t)c * n + r] : 0.0;
+						}
+						// (v outside [j + 1, n) is not read: those entries of vb are stale or past its end)
+#pragma unroll
+						for(int k = 0; k < 8; k += 2){
+							const unsigned c = c0 + k;
+							a0 = fma(x[k], (c >= j + 1 && c < n) ? vb[c] : 0.0, a0);
+							a1 = fma(x[k + 1], (c + 1 >= j + 1 && c + 1 < n) ? vb[c + 1] : 0.0, a1);
+						}
+						// then the column part, in place
+#pragma unroll
+						for(int k = 0; k < 8; k++) x[k] = c0 + k < r ? x[k] * vr : 0.0;
+						const double zc = tl_reduce8(x);
+						if((lane & 7) == 0) zw[c0 + (lane >> 3)] = zc;
+					}
+				}
+				acc[0] = a0 + a1;
+				__syncthreads();
+				// the column sums of the rows below r: waves r / 64 .. the last with rows < n
+				if(live)
+					for(unsigned w2 = r >> 6; 64 * w2 < n; w2++) acc[0] += zsum[32 * (size_t)w2 * (w2 + 1) + r];
+			}
+			double pw[NB], pv[NB];
 #pragma unroll
 			for(int i = 0; i < NB; i++){ pw[i] = 0; pv[i] = 0; }
+			if(tid >= j + 1 && tid < n){
+				const double vr = vb[tid];
 #pragma unroll
-			for(int q = 0; q < RPT; q++){
-				const unsigned r = tid + q * BT;
-				acc[q] = 0;
-				if(r >= j + 1 && r < n){
-					const double vr = vb[r];
-					double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-					unsigned c = j + 1;
-					for(; c + 4 <= n; c += 4){
-						const size_t o = (size_t)c * n + r;
-						a0 += A[o] * vb[c];
-						a1 += A[o + n] * vb[c + 1];
-						a2 += A[o + 2 * (size_t)n] * vb[c + 2];
-						a3 += A[o + 3 * (size_t)n] * vb[c + 3];
-					}
-					for(; c < n; c++) a0 += A[(size_t)c * n + r] * vb[c];
-					acc[q] = (a0 + a1) + (a2 + a3);
-#pragma unroll
-					for(int i = 0; i < NB; i++){
-						pw[i] = fma(wrow[q][i], vr, pw[i]);
-						pv[i] = fma(vrow[q][i], vr, pv[i]);
-					}
+				for(int i = 0; i < NB; i++){
+					pw[i] = fma(wrow[0][i], vr, pw[i]);
+					pv[i] = fma(vrow[0][i], vr, pv[i]);
 				}
 			}
 			if(k > 0){
@@ -428,7 +497,9 @@ __global__ __launch_bounds__(BT) void tl_tridiag_panel_kernel(double *__restrict
 			for(int q = 0; q < RPT; q++){
 				const unsigned r = tid + q * BT;
 				if(r >= jn && r < n){
-					for(unsigned cc = 0; cc < cn; cc++){
+					// the lower triangle only (column <= row)
+					const unsigned cend = r + 1 - c0 < cn ? r + 1 - c0 : cn;
+					for(unsigned cc = 0; cc < cend && r >= c0; cc++){
 						const double *cv = chunk + cc * 2 * NB;
 						double x = 0;
 #pragma unroll
@@ -1229,7 +1300,9 @@ extern "C" int ephmc_tl_hist(ephmc_ctx *c, const double *normals, unsigned nsamp
 		EPHMC_TRY(timer_begin(c, "tl_tridiag"));
 		{
 			if(n <= TL_BT){
-				const size_t lds = ((size_t)n + TL_BT / 64 + 2 * TL_PNB + (TL_BT / 64) * 2 * TL_PNB + 2 * TL_PNB + 64 * 2 * TL_PNB) * 8;
+				const size_t lds = tl_panel_lds_doubles<TL_BT, TL_PNB>(n) * 8;
+				(void)hipFuncSetAttribute((const void *)tl_tridiag_panel_kernel<TL_BT, 1, TL_PNB>,
+					hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 				tl_tridiag_panel_kernel<TL_BT, 1, TL_PNB><<<nb_, TL_BT, lds, c->stream>>>(w.A, n, w.d, w.e, w.tau);
 			}else{
 				const size_t lds = (3 * (size_t)n + TL_BT / 64) * 8;
