@@ -390,13 +390,19 @@ __global__ __launch_bounds__(BT) void tl_tridiag_panel_kernel(double *__restrict
 				double a0 = 0, a1 = 0;
 				if(64 * wave + 63 >= j + 1 && 64 * wave < n){
 					double *zw = zsum + 32 * (size_t)wave * (wave + 1);
-					for(unsigned c0 = (j + 1) & ~7u; c0 < 64 * wave + 64; c0 += 8){
-						double x[8];
+					// tiles of 8 columns; the next tile's loads are issued before this tile's sums
+					const unsigned cbeg = (j + 1) & ~7u, cend = 64 * wave + 64;
+					double x[8], xn[8];
+					auto load_tile = [&](unsigned c0, double (&dst)[8]){
 #pragma unroll
 						for(int k = 0; k < 8; k++){
 							const unsigned c = c0 + k;
-							x[k] = (live && c >= j + 1 && c <= r) ? A[(size_t)c * n + r] : 0.0;
+							dst[k] = (live && c >= j + 1 && c <= r) ? A[(size_t)c * n + r] : 0.0;
 						}
+					};
+					load_tile(cbeg, x);
+					for(unsigned c0 = cbeg; c0 < cend; c0 += 8){
+						if(c0 + 8 < cend) load_tile(c0 + 8, xn);
 						// (v outside [j + 1, n) is not read: those entries of vb are stale or past its end)
 #pragma unroll
 						for(int k = 0; k < 8; k += 2){
@@ -409,6 +415,8 @@ __global__ __launch_bounds__(BT) void tl_tridiag_panel_kernel(double *__restrict
 						for(int k = 0; k < 8; k++) x[k] = c0 + k < r ? x[k] * vr : 0.0;
 						const double zc = tl_reduce8(x);
 						if((lane & 7) == 0) zw[c0 + (lane >> 3)] = zc;
+#pragma unroll
+						for(int k = 0; k < 8; k++) x[k] = xn[k];
 					}
 				}
 				acc[0] = a0 + a1;
