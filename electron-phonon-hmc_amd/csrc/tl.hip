@@ -1075,17 +1075,30 @@ __global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict_
 		brange[0] = TL_NONE;
 		brange[1] = 0;
 	}
-	for(unsigned r0 = 0; r0 < n; r0 += TL_KC){
+	// the next stage's rows are loaded into registers while this stage's products run
+	double pv[4], pw0[4], pw1[4];
+	auto fetch = [&](unsigned r0){
 #pragma unroll
 		for(int q = 0; q < 4; q++){
 			const unsigned e = tid + q * 256, cc = e / TL_KC, rr = e % TL_KC;
 			const unsigned r = r0 + rr;
 			const unsigned jc = j0 + cc, ic = i0 + cc;
-			Vs[rr][cc] = (jc < n && r < n) ? V[(size_t)jc * n + r] : 0.0;
-			W0s[rr][cc] = (ic < n && r < n) ? W0[(size_t)ic * n + r] : 0.0;
-			W1s[rr][cc] = (ic < n && r < n) ? W1[(size_t)ic * n + r] : 0.0;
+			pv[q] = (jc < n && r < n) ? V[(size_t)jc * n + r] : 0.0;
+			pw0[q] = (ic < n && r < n) ? W0[(size_t)ic * n + r] : 0.0;
+			pw1[q] = (ic < n && r < n) ? W1[(size_t)ic * n + r] : 0.0;
+		}
+	};
+	fetch(0);
+	for(unsigned r0 = 0; r0 < n; r0 += TL_KC){
+#pragma unroll
+		for(int q = 0; q < 4; q++){
+			const unsigned e = tid + q * 256, cc = e / TL_KC, rr = e % TL_KC;
+			Vs[rr][cc] = pv[q];
+			W0s[rr][cc] = pw0[q];
+			W1s[rr][cc] = pw1[q];
 		}
 		__syncthreads();
+		if(r0 + TL_KC < n) fetch(r0 + TL_KC);
 #pragma unroll
 		for(int kq = 0; kq < TL_KC / 4; kq++){
 			const unsigned kk = 4 * kq + k4;
