@@ -52,7 +52,7 @@ namespace ephmc {
 constexpr int TL_BT = 1024;     // tridiagonalisation / sort workgroup
 constexpr int TL_G = 16;        // sweeps per wavefront pass
 constexpr int TL_PF = 8;        // columns loaded ahead by the rotation pass
-constexpr int TL_QC = 64;       // rotation steps staged in LDS at a time
+constexpr int TL_QC = 75;       // rotation steps staged in LDS at a time (a multiple of TL_G + 1 + TL_PF)
 constexpr int TL_TILE = 64;
 constexpr int TL_KC = 16;       // rows of V / W per LDS stage of the pair tiles
 constexpr unsigned TL_NONE = 0xffffffffu;
@@ -647,14 +647,66 @@ __global__ __launch_bounds__(64) void tl_ql_scalar_kernel(const double *__restri
 
 // ---- rotations applied to Z = I (thread = row, column-major Z) ------------------------------
 // G sweeps per pass: sweep k acts at position q + k while the window steps q downwards, so the
-// pass holds G+1 columns in registers.  Column q-1-j is loaded PF steps ahead.  The rotations of
-// the next QC steps of all G sweeps are staged in LDS by the whole workgroup (every row of the
-// sample applies the same rotations); a sweep outside its range at a step gets the identity
-// rotation (c, s) = (1, 0), which leaves both columns unchanged.
+// pass holds G+1 columns in registers, and PF more columns are loaded ahead.  The G+1+PF columns
+// live in a ring of registers whose slot for logical column k at step t is (k - t) mod (G+1+PF):
+// the step loop is unrolled by the ring length, so every slot index is a constant and the window
+// moves without register copies (the column leaving the window is stored and its slot takes the
+// next column loaded ahead).  The rotations of the next QC steps (a multiple of the ring length)
+// of all G sweeps are staged in LDS by the whole workgroup (every row of the sample applies the
+// same rotations); a sweep outside its range at a step gets the identity rotation (c, s) = (1, 0),
+// which leaves both columns unchanged.
+// One step at ring phase PH: the G rotations of the step on the window, then either the window
+// written out (the last step, q = qend: returns true) or column q + G stored and its slot loaded
+// with column q - 1 - PF.
+template <int G, int PF, int PH>
+__device__ __forceinline__ bool tl_ql_step(double (&R)[G + 1 + PF], const double2 *rst, int q, int qend, double *__restrict__ Z,
+		unsigned n, unsigned r, bool act)
+{
+	constexpr int NR = G + 1 + PF;
+#pragma unroll
+	for(int k = 0; k < G; k++){
+		const double2 cs = rst[k];
+		double &a = R[(k - PH + 2 * NR) % NR], &f = R[(k + 1 - PH + 2 * NR) % NR];
+		const double av = a, fv = f;
+		f = cs.y * av + cs.x * fv;
+		a = cs.x * av - cs.y * fv;
+	}
+	if(q <= qend){
+#pragma unroll
+		for(int k = 0; k <= G; k++){
+			const int col = qend + k;
+			if(act && col >= 0 && col < (int)n) Z[(size_t)col * n + r] = R[(k - PH + 2 * NR) % NR];
+		}
+		return true;
+	}
+	constexpr int out = (G - PH + 2 * NR) % NR;
+	const int col = q + G;
+	if(act && col < (int)n) Z[(size_t)col * n + r] = R[out];
+	const int pcol = q - 1 - PF;
+	R[out] = (act && pcol >= 0) ? Z[(size_t)pcol * n + r] : 0.0;
+	return false;
+}
+
+// Steps t0 + PH .. t0 + NR - 1 of a stage (one turn of the ring from phase PH); true when the
+// pass ended in it.
+template <int G, int PF, int QC, int PH>
+__device__ __forceinline__ bool tl_ql_turn(double (&R)[G + 1 + PF], const double2 (*rs)[G], int t0, int qc, int qend,
+		double *__restrict__ Z, unsigned n, unsigned r, bool act)
+{
+	if constexpr(PH < G + 1 + PF){
+		if(tl_ql_step<G, PF, PH>(R, rs[t0 + PH], qc - t0 - PH, qend, Z, n, r, act)) return true;
+		return tl_ql_turn<G, PF, QC, PH + 1>(R, rs, t0, qc, qend, Z, n, r, act);
+	}else{
+		return false;
+	}
+}
+
 template <int G, int PF, int QC>
 __global__ __launch_bounds__(256) void tl_ql_apply_kernel(double *__restrict__ Zall, unsigned n, const double2 *__restrict__ rotall,
 		size_t rot_cap, const int4 *__restrict__ swall, unsigned sw_cap, const unsigned *__restrict__ nsw_all)
 {
+	constexpr int NR = G + 1 + PF;
+	static_assert(QC % NR == 0, "a stage is whole turns of the ring");
 	__shared__ double2 rs[QC][G];
 	const unsigned s = blockIdx.y;
 	const unsigned r = blockIdx.x * 256 + threadIdx.x;
@@ -683,53 +735,30 @@ __global__ __launch_bounds__(256) void tl_ql_apply_kernel(double *__restrict__ Z
 				hi[k] = 0;
 			}
 		}
-		double W[G + 1], pf[PF];
+		// step 0: window columns P .. P+G in slots 0 .. G, columns P-1-j ahead in slot NR-1-j
+		double R[NR];
 #pragma unroll
 		for(int k = 0; k <= G; k++){
 			const int col = P + k;
-			W[k] = (act && col < (int)n) ? Z[(size_t)col * n + r] : 0.0;
+			R[k] = (act && col < (int)n) ? Z[(size_t)col * n + r] : 0.0;
 		}
 #pragma unroll
-		for(int j = 0; j < PF; j++){
-			const int col = P - 1 - j;
-			pf[j] = (act && col >= 0) ? Z[(size_t)col * n + r] : 0.0;
+		for(int jj = 0; jj < PF; jj++){
+			const int col = P - 1 - jj;
+			R[NR - 1 - jj] = (act && col >= 0) ? Z[(size_t)col * n + r] : 0.0;
 		}
-		// steps q = P, P-1, ..., qend in chunks of QC
-		for(int qc = P; qc >= qend; qc -= QC){
-			__syncthreads();                 // the previous chunk's rotations are consumed
+		// steps q = P, P-1, ..., qend (P >= qend) in stages of QC
+		bool done = false;
+		for(int qc = P; !done; qc -= QC){
+			__syncthreads();                 // the previous stage's rotations are consumed
 			for(int e = threadIdx.x; e < QC * G; e += 256){
 				const int t = e / G, k = e % G, pos = qc - t + k;
 				const bool in = qc - t >= qend && pos >= lo[k] && pos <= hi[k];
 				rs[t][k] = in ? rot[base[k] + (hi[k] - pos)] : make_double2(1.0, 0.0);
 			}
 			__syncthreads();
-			const int tn = min(QC, qc - qend + 1);
-			for(int t = 0; t < tn; t++){
-				const int q = qc - t;
-#pragma unroll
-				for(int k = 0; k < G; k++){
-					const double2 cs = rs[t][k];
-					const double a = W[k], f = W[k + 1];
-					W[k + 1] = cs.y * a + cs.x * f;
-					W[k] = cs.x * a - cs.y * f;
-				}
-				if(q <= qend) break;
-				const int col = q + G;
-				if(act && col < (int)n) Z[(size_t)col * n + r] = W[G];
-#pragma unroll
-				for(int k = G; k > 0; k--) W[k] = W[k - 1];
-				W[0] = pf[0];
-#pragma unroll
-				for(int j = 0; j + 1 < PF; j++) pf[j] = pf[j + 1];
-				const int pcol = q - 1 - PF;
-				pf[PF - 1] = (act && pcol >= 0) ? Z[(size_t)pcol * n + r] : 0.0;
-			}
-		}
-		// the window ends at q = qend: columns qend .. qend + G
-#pragma unroll
-		for(int k = 0; k <= G; k++){
-			const int col = qend + k;
-			if(act && col >= 0 && col < (int)n) Z[(size_t)col * n + r] = W[k];
+			for(int u = 0; u < QC / NR && !done; u++)
+				done = tl_ql_turn<G, PF, QC, 0>(R, rs, u * NR, qc, qend, Z, n, r, act);
 		}
 	}
 }
