@@ -50,9 +50,9 @@
 namespace ephmc {
 
 constexpr int TL_BT = 1024;     // tridiagonalisation / sort workgroup
-constexpr int TL_G = 16;        // sweeps per wavefront pass
+constexpr int TL_G = 24;        // sweeps per wavefront pass
 constexpr int TL_PF = 8;        // columns loaded ahead by the rotation pass
-constexpr int TL_QC = 75;       // rotation steps staged in LDS at a time (a multiple of TL_G + 1 + TL_PF)
+constexpr int TL_QC = 66;       // rotation steps staged in LDS at a time (a multiple of TL_G + 1 + TL_PF)
 constexpr int TL_TILE = 64;
 constexpr int TL_KC = 16;       // rows of V / W per LDS stage of the pair tiles
 constexpr unsigned TL_NONE = 0xffffffffu;
