@@ -283,7 +283,7 @@ template <int BT, int NB>
 constexpr size_t tl_panel_lds_doubles(unsigned n)
 {
 	constexpr size_t NW = BT / 64;
-	return (size_t)n + NW + 2 * NB + NW * 2 * NB + 2 * NB + 64 * 2 * NB + 64 * NW * (NW + 1) / 2;
+	return (size_t)n + NW + 2 * NB + NW * 2 * NB + 2 * NB + 64 * 2 * NB + 64 * NW * (NW + 1) / 2 + 64 * NW;
 }
 
 // Sums over the 64 lanes of a wave of x[0..7] (one value per lane each): returns, in lane l, the
@@ -330,7 +330,8 @@ __global__ __launch_bounds__(BT) void tl_tridiag_panel_kernel(double *__restrict
 	double *dots = rowvw + 2 * NB;           // NW x 2 NB wave partials
 	double *dsum = dots + NW * 2 * NB;       // 2 NB: W^T v, V^T v
 	double *chunk = dsum + 2 * NB;           // 64 x 2 NB: rows of V and W for the panel update
-	double *zsum = chunk + 64 * 2 * NB;      // wave w's column sums at 64 w (w + 1) / 2 + column
+	double *zsum = chunk + 64 * 2 * NB;      // row block I's column sums at 64 I (I + 1) / 2 + column
+	double *rowpart = zsum + 64 * NW * (NW + 1) / 2;   // row sums of block I's tiles done by its partner wave
 	static_assert(RPT == 1, "the lower-triangle product holds one row per thread");
 	const unsigned s = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	double *A = Aall + (size_t)s * n * n, *d = dall + (size_t)s * n, *e = eall + (size_t)s * n, *tau = tall + (size_t)s * n;
@@ -380,48 +381,75 @@ __global__ __launch_bounds__(BT) void tl_tridiag_panel_kernel(double *__restrict
 			__syncthreads();
 			const double tj = tl_reflector<BT, RPT>(A, n, j, vb, red, e, tau);
 			// A v over the panel-start trailing block from its lower triangle, and the panel's
-			// W^T v, V^T v.  Wave `wave` walks the tiles of its rows left of and on the diagonal;
-			// lanes past n or above j + 1 load nothing and add zeros.
+			// W^T v, V^T v.  Row block I (64 rows) has one tile of 64 x 8 per 8 columns from the
+			// chunk of column j + 1 to its diagonal, so blocks far down hold more tiles; waves are
+			// paired (w, NW-1-w) and the pair's tiles split evenly: wave w < NW/2 takes its own
+			// block's tiles and the partner block's first ones, whose row sums it leaves in LDS
+			// (rowpart) for the partner to add after its own.  Column sums go to the triangle
+			// zsum by row block.  Lanes past n or above j + 1 load nothing and add zeros.
 			double acc[RPT];
 			{
-				const unsigned r = tid;
-				const bool live = r >= j + 1 && r < n;
-				const double vr = live ? vb[r] : 0.0;
-				double a0 = 0, a1 = 0;
-				if(64 * wave + 63 >= j + 1 && 64 * wave < n){
-					double *zw = zsum + 32 * (size_t)wave * (wave + 1);
-					// tiles of 8 columns; the next tile's loads are issued before this tile's sums
-					const unsigned cbeg = (j + 1) & ~7u, cend = 64 * wave + 64;
-					double x[8], xn[8];
-					auto load_tile = [&](unsigned c0, double (&dst)[8]){
+				const unsigned cstart = (j + 1) & ~7u;
+				auto chunks_of = [&](unsigned I) -> unsigned {
+					const unsigned cend = 64 * I + 64;
+					return (64 * I < n && cend > cstart) ? (cend - cstart) / 8 : 0u;
+				};
+				// (the wave index as a wave-uniform value, so that the tile bookkeeping stays scalar)
+				const unsigned wv = (unsigned)__builtin_amdgcn_readfirstlane((int)wave);
+				const unsigned partner = NW - 1 - wv;
+				// the lower block of the pair, its chunks, and the upper block's first chunks that the
+				// lower block's wave takes
+				const unsigned lo_blk = wv < NW / 2 ? wv : partner, hi_blk = NW - 1 - lo_blk;
+				const unsigned c_lo = chunks_of(lo_blk), c_hi = chunks_of(hi_blk);
+				const unsigned t_lo = (c_lo + c_hi) / 2;
+				const unsigned lent = t_lo > c_lo ? t_lo - c_lo : 0u;
+				// a run of `cnt` tiles of row block I from chunk `first`: row sums into s0, s1
+				auto run = [&](unsigned I, unsigned first, unsigned cnt, double &s0, double &s1){
+					const unsigned r = 64 * I + lane;
+					const bool live = r >= j + 1 && r < n;
+					const double vr = live ? vb[r] : 0.0;
+					double *zw = zsum + 32 * (size_t)I * (I + 1);
+					const unsigned cbeg = cstart + 8 * first, cend = cbeg + 8 * cnt;
+					for(unsigned c0 = cbeg; c0 < cend; c0 += 8){
+						double x[8];
 #pragma unroll
 						for(int k = 0; k < 8; k++){
 							const unsigned c = c0 + k;
-							dst[k] = (live && c >= j + 1 && c <= r) ? A[(size_t)c * n + r] : 0.0;
+							x[k] = (live && c >= j + 1 && c <= r) ? A[(size_t)c * n + r] : 0.0;
 						}
-					};
-					load_tile(cbeg, x);
-					for(unsigned c0 = cbeg; c0 < cend; c0 += 8){
-						if(c0 + 8 < cend) load_tile(c0 + 8, xn);
 						// (v outside [j + 1, n) is not read: those entries of vb are stale or past its end)
 #pragma unroll
 						for(int k = 0; k < 8; k += 2){
 							const unsigned c = c0 + k;
-							a0 = fma(x[k], (c >= j + 1 && c < n) ? vb[c] : 0.0, a0);
-							a1 = fma(x[k + 1], (c + 1 >= j + 1 && c + 1 < n) ? vb[c + 1] : 0.0, a1);
+							s0 = fma(x[k], (c >= j + 1 && c < n) ? vb[c] : 0.0, s0);
+							s1 = fma(x[k + 1], (c + 1 >= j + 1 && c + 1 < n) ? vb[c + 1] : 0.0, s1);
 						}
 						// then the column part, in place
 #pragma unroll
 						for(int k = 0; k < 8; k++) x[k] = c0 + k < r ? x[k] * vr : 0.0;
 						const double zc = tl_reduce8(x);
 						if((lane & 7) == 0) zw[c0 + (lane >> 3)] = zc;
-#pragma unroll
-						for(int k = 0; k < 8; k++) x[k] = xn[k];
 					}
+				};
+				// one or two runs through one copy of the tile loop: the wave's own block, then (lower
+				// waves) the partner block's first `lent` tiles
+				double own = 0;
+#pragma nounroll
+				for(int seg = 0; seg < (wv < NW / 2 ? 2 : 1); seg++){
+					const unsigned I = seg ? partner : wv;
+					const unsigned first = wv < NW / 2 ? 0u : lent;
+					const unsigned cnt = seg ? lent : (wv < NW / 2 ? c_lo : c_hi - lent);
+					double s0 = 0, s1 = 0;
+					run(I, first, cnt, s0, s1);
+					if(seg) rowpart[partner * 64 + lane] = s0 + s1;
+					else own = s0 + s1;
 				}
-				acc[0] = a0 + a1;
+				acc[0] = own;
 				__syncthreads();
-				// the column sums of the rows below r: waves r / 64 .. the last with rows < n
+				const unsigned r = tid;
+				const bool live = r >= j + 1 && r < n;
+				if(wv >= NW / 2 && lent) acc[0] += rowpart[wv * 64 + lane];
+				// the column sums of the rows below r: row blocks r / 64 .. the last with rows < n
 				if(live)
 					for(unsigned w2 = r >> 6; 64 * w2 < n; w2++) acc[0] += zsum[32 * (size_t)w2 * (w2 + 1) + r];
 			}
