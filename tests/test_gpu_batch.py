@@ -5,8 +5,9 @@ checkpoints, some autotuned and one with fewer MD steps, run the same random inp
 (ephmc_trajectory) and then as one batch: every chain's records, final field and M_hat^{-1} are
 bit-identical over several trajectories (the batch runs the single-chain kernels' bodies with each
 chain's own grid for every sum).  The solo path itself is pinned against the oracle by the
-configuration tests.  Modes the batch does not run are refused with EPHMC_ERR_UNSUPPORTED, and a
-context held by a batch can be neither tuned nor destroyed until the batch is."""
+configuration tests.  The same holds at the 64-chain workload's shape (64 chains of L 8..24, Nt 32..128).
+Modes the batch does not run are refused with EPHMC_ERR_UNSUPPORTED, and a context held by a batch can
+be neither tuned nor destroyed until the batch is."""
 import ctypes as C
 
 import numpy as np
@@ -114,3 +115,37 @@ def test_batch_refuses_unsupported():
     ready.autotune()
     for c in (ready, fresh, nofa, tiny, other):
         c.close()
+
+
+def test_batch_64_uneven_chains_equal_solo():
+    """The 64-chain workload's shape (L from {8, 12, 16, 20, 24}, Nt from {32, 64, 128}, mixed precision,
+    Nmd = 10): two trajectories of every chain as one batch equal the chains run alone, bit for bit."""
+    E = ephmc()
+    rng = np.random.default_rng(0)
+    Ls, Nts = (8, 12, 16, 20, 24), (32, 64, 128)
+    ctxs, inputs = [], []
+    for i in range(64):
+        l, nt = Ls[rng.integers(5)], Nts[rng.integers(3)]
+        om = OracleModel(l, l, nt)
+        ctxs.append(gpu_context(om, steps=10, checkpoint=0, precision=1, graph=1))
+        inputs.append(draws_for(3000 + i, om.dim, 2))
+    solo = []
+    for ctx, d in zip(ctxs, inputs):
+        ctx.chain_init(d[0][0])
+        recs = [rec_tuple(ctx.trajectory(nrm, u)) for nrm, u in d[1:]]
+        solo.append((recs, ctx.chain_x()))
+    for ctx, d in zip(ctxs, inputs):
+        ctx.chain_init(d[0][0])
+    b = E.Batch(ctxs)
+    say("batch of 64:", b.info())
+    got = [[] for _ in ctxs]
+    for t in (1, 2):
+        for j, r in enumerate(b.trajectory([d[t][0] for d in inputs], [d[t][1] for d in inputs])):
+            got[j].append(rec_tuple(r))
+    b.close()
+    for j, ctx in enumerate(ctxs):
+        assert got[j] == solo[j][0], j
+        np.testing.assert_array_equal(ctx.chain_x(), solo[j][1])
+    say("batch of 64: records and fields equal the solo runs")
+    for ctx in ctxs:
+        ctx.close()
