@@ -885,6 +885,19 @@ static int reserve_partials(ephmc_ctx *c, unsigned nwg)
 	return EPHMC_OK;
 }
 
+// Columns per workgroup of the force sweep set to C (partial sums reserved for its workgroup count;
+// a captured trajectory graph, which holds the old shape, is dropped).
+int ephmc::set_force_cols(ephmc_ctx *c, unsigned C)
+{
+	if(c->cols == C) return EPHMC_OK;
+	const unsigned nwg = (c->ns + C - 1) / C;
+	EPHMC_TRY(reserve_partials(c, nwg));
+	drop_graph(c);
+	c->cols = C;
+	c->nwg = nwg;
+	return EPHMC_OK;
+}
+
 extern "C" {
 
 int ephmc_autotune(ephmc_ctx *c, ephmc_tune_result *res)

@@ -275,6 +275,8 @@ int broadcast_slice0(ephmc_ctx *c, double *pdot);
 size_t adjoint_lds(unsigned ns, int prec, unsigned C);
 // columns per workgroup of the force sweep (the context's choice, else 2 in fp64 and 4 in fp32)
 unsigned adjoint_cols(const ephmc_ctx *c, int prec);
+// ctx.hip: sets the context's force-sweep columns per workgroup (reserving its partial sums)
+int set_force_cols(ephmc_ctx *c, unsigned C);
 // force_adjoint(c, prec, minv, pdot, 0, null, null) as batch records: segs[i] = i-th sweep launch,
 // then the final reduction
 int batch_adjoint(std::vector<BatchOp> &segs, BatchOp &reduce, ephmc_ctx *c, int prec, unsigned threads,

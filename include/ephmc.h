@@ -229,8 +229,10 @@ int ephmc_chain_buffers(ephmc_ctx *ctx, double **x_dev, double **p_dev, double *
  * Fourier-accelerated trajectory (no NO_FOURIER_ACC / CLASSICAL_FOURIER_ACC, stabilize,
  * SINGLE_PARTICLE, ZERO_FILLING, CONSTANT_FORCE or SCHUR_FORM_INV) at Taylor order 4 for
  * 33 <= N <= 1024, else EPHMC_ERR_UNSUPPORTED.  Each chain's records and state are bit for bit
- * those of ephmc_trajectory with the same inputs.  The contexts keep their state: between batch
- * trajectories (after ephmc_batch_sync) each can be used alone.  While a batch holds a context,
+ * those of ephmc_trajectory with the same inputs.  ephmc_batch_create sets each context's force
+ * sweep to four columns per workgroup where its LDS allows (the width that runs many chains
+ * fastest; ephmc_autotune's choice is for a chain alone), and the contexts keep it.  The contexts
+ * keep their state: between batch trajectories (after ephmc_batch_sync) each can be used alone.  While a batch holds a context,
  * ephmc_destroy and ephmc_autotune refuse it (EPHMC_ERR_ARG): destroy the batch first. */
 typedef struct ephmc_batch ephmc_batch;
 int ephmc_batch_create(ephmc_ctx *const *ctxs, unsigned n, ephmc_batch **out);

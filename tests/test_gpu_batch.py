@@ -1,10 +1,10 @@
 """Many chains advanced with one launch per phase (ephmc_batch_*).
 
 Contexts of uneven lattice and Nt (L 8..24, Nt 32..128), FP64 and mixed, with and without
-checkpoints, some autotuned and one with fewer MD steps, run the same random inputs first alone
-(ephmc_trajectory) and then as one batch: every chain's records, final field and M_hat^{-1} are
-bit-identical over several trajectories (the batch runs the single-chain kernels' bodies with each
-chain's own grid for every sum).  The solo path itself is pinned against the oracle by the
+checkpoints, some autotuned and one with fewer MD steps, run the same random inputs alone
+(ephmc_trajectory, with the force width the batch set) and as one batch: every chain's records, final
+field and M_hat^{-1} are bit-identical over several trajectories (the batch runs the single-chain
+kernels' bodies with each chain's own grid for every sum).  The solo path itself is pinned against the oracle by the
 configuration tests.  The same holds at the 64-chain workload's shape (64 chains of L 8..24, Nt 32..128).
 Modes the batch does not run are refused with EPHMC_ERR_UNSUPPORTED, and a context held by a batch can
 be neither tuned nor destroyed until the batch is."""
@@ -51,6 +51,10 @@ def test_batch_equals_solo():
         ctxs.append(ctx)
         inputs.append(draws_for(2000 + i, om.dim, NTRAJ))
     say("batch: %d contexts" % len(ctxs))
+    for ctx, d in zip(ctxs, inputs):
+        ctx.chain_init(d[0][0])
+    # the batch sets the contexts' force width; the contexts then run alone between batch trajectories
+    b = E.Batch(ctxs)
     solo = []
     for ctx, d in zip(ctxs, inputs):
         ctx.chain_init(d[0][0])
@@ -59,7 +63,6 @@ def test_batch_equals_solo():
     say("batch: solo runs done")
     for ctx, d in zip(ctxs, inputs):
         ctx.chain_init(d[0][0])
-    b = E.Batch(ctxs)
     info = b.info()
     say("batch:", info)
     # groups: (FP64 or mixed) x workgroup class x force columns x MD steps
@@ -129,6 +132,10 @@ def test_batch_64_uneven_chains_equal_solo():
         om = OracleModel(l, l, nt)
         ctxs.append(gpu_context(om, steps=10, checkpoint=0, precision=1, graph=1))
         inputs.append(draws_for(3000 + i, om.dim, 2))
+    for ctx, d in zip(ctxs, inputs):
+        ctx.chain_init(d[0][0])
+    b = E.Batch(ctxs)
+    say("batch of 64:", b.info())
     solo = []
     for ctx, d in zip(ctxs, inputs):
         ctx.chain_init(d[0][0])
@@ -136,8 +143,6 @@ def test_batch_64_uneven_chains_equal_solo():
         solo.append((recs, ctx.chain_x()))
     for ctx, d in zip(ctxs, inputs):
         ctx.chain_init(d[0][0])
-    b = E.Batch(ctxs)
-    say("batch of 64:", b.info())
     got = [[] for _ in ctxs]
     for t in (1, 2):
         for j, r in enumerate(b.trajectory([d[t][0] for d in inputs], [d[t][1] for d in inputs])):
