@@ -55,6 +55,7 @@ constexpr int TL_PF = 8;        // columns loaded ahead by the rotation pass
 constexpr int TL_QC = 66;       // rotation steps staged in LDS at a time (a multiple of TL_G + 1 + TL_PF)
 constexpr int TL_TILE = 64;
 constexpr int TL_KC = 16;       // rows of V / W per LDS stage of the pair tiles
+constexpr unsigned TL_HB = 512; // bins a pair tile accumulates in LDS (wider tiles bin by search)
 constexpr unsigned TL_NONE = 0xffffffffu;
 
 __device__ inline double wave_sum(double v)
@@ -1106,6 +1107,7 @@ __global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict_
 	__shared__ double eval[TL_TILE * TL_TILE];
 	__shared__ unsigned brange[2];
 	__shared__ uint2 rrange[TL_TILE];
+	__shared__ double hacc[TL_HB];
 	const unsigned ib = blockIdx.x, jb = blockIdx.y, s = blockIdx.z, nblk = gridDim.x;
 	const unsigned tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 	const unsigned i0 = ib * TL_TILE, j0 = jb * TL_TILE;
@@ -1209,8 +1211,7 @@ __global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict_
 	__syncthreads();
 	// Along a row (fixed j) the eigenvalues E_i ascend, so the bins are TL_NONE (difference past
 	// the last bin), then non-increasing, then TL_NONE (E_i > E_j or i >= n): each bin is one run
-	// per row.  Row ranges first, then every bin adds its runs row by row, i ascending - the
-	// order of a scan of the whole tile, found by binary search.
+	// per row.  Row ranges first, then the runs are added to their bins row by row.
 	if(tid < TL_TILE){
 		const unsigned *rb = ebin + tid * TL_TILE;
 		unsigned lo = TL_TILE, hi = 0;
@@ -1223,6 +1224,31 @@ __global__ __launch_bounds__(256) void tl_pairs_kernel(const double *__restrict_
 	}
 	__syncthreads();
 	const unsigned blo = brange[0], bhi = brange[1];
+	if(blo != TL_NONE && bhi - blo < TL_HB){
+		// A row's runs fall in distinct bins, so one wave adds the runs of row after row (the LDS
+		// operations of a wave complete in order): every bin adds its rows' run sums in row order,
+		// each run summed with i ascending.
+		for(unsigned b = tid; b < TL_HB; b += 256) hacc[b] = 0;
+		__syncthreads();
+		if(wave == 0)
+			for(unsigned jl = 0; jl < TL_TILE; jl++){
+				const uint2 rr = rrange[jl];
+				const unsigned *rb = ebin + jl * TL_TILE;
+				const unsigned il = lane;
+				if(il >= rr.x && il < rr.y){
+					const unsigned bin = rb[il];
+					if(bin != TL_NONE && (il == rr.x || rb[il - 1] != bin)){
+						double sum = 0;
+						for(unsigned q = il; q < rr.y && rb[q] == bin; q++) sum += eval[jl * TL_TILE + q];
+						hacc[bin - blo] += sum;
+					}
+				}
+			}
+		__syncthreads();
+		for(unsigned b = tid; b < nbins; b += 256) out[b] = (b >= blo && b <= bhi) ? hacc[b - blo] : 0.0;
+		return;
+	}
+	// wide tiles: every bin finds its run in each row by binary search and adds it row by row
 	for(unsigned b = tid; b < nbins; b += 256){
 		double sum = 0;
 		if(blo != TL_NONE && b >= blo && b <= bhi)
