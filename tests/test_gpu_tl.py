@@ -108,6 +108,20 @@ def test_tl_large_vs_numpy(l1, l2, nsamples):
     assert rel(g, gn) < 1e-9
 
 
+def test_tl_wide_tiles_vs_numpy():
+    """4096 bins at N = 256: pair tiles whose bins span 512 or more take the binary-search binning,
+    the others the row-ordered histogram; both against LAPACK eigh."""
+    nbins = 4096
+    om, width = tl_model(16, 16, nbins)
+    xs = draws(om, 2, 13)
+    ctx = gpu_context(om)
+    g, z = ctx.tl_hist(xs, CM, width, nbins)
+    gn = np.zeros(nbins)
+    for x in xs:
+        gn += numpy_tl(om, x, width, nbins)[0]
+    assert rel(g, gn) < 1e-9
+
+
 def test_tl_repeatable_and_additive():
     """Two identical calls give identical bins (no atomics); bins of a split sample set add up."""
     nbins = 64
