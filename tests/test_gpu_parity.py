@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import _dense as D
-from _gpu import gpu_context, rel, say
+from _gpu import ephmc, gpu_context, rel, say
 from _oracle import MT, OracleModel, oracle, ptr
 
 pytestmark = pytest.mark.gpu
@@ -184,6 +184,40 @@ def test_force_fp64_default_width_near_lds_limit(l1, l2):
     Fo = om.force(x, Minv, ld)
     say("rel err", rel(Fg, Fo))
     assert rel(Fg, Fo) < 1e-10
+
+
+def test_maximum_lattice_size():
+    """The largest lattice a context accepts, 64 x 32 = 2048 sites (Nt = 8; the chain and the sweep
+    give each thread two sites): M_hat against the oracle (1e-13), log det and inverse against numpy
+    (1e-10), and the force against the oracle's with the same inverse, FP64 (1e-10) with every slice
+    stored and recomputed from checkpoints every 4 slices, mixed precision (1e-5).  One lattice row
+    more (65 x 32 = 2080 sites) is refused as unsupported."""
+    say("maximum lattice 64 x 32")
+    om = OracleModel(64, 32, 8, beta=8 * 40 / 32)
+    x = field(om, 31)
+    n = om.ns
+    Mo = om.M_hat(x)
+    ctx = gpu_context(om, checkpoint=1)
+    Mg = ctx.M_hat(x)
+    say("M_hat rel err", rel(Mg, Mo))
+    assert rel(Mg, Mo) < 1e-13
+    ld, sg, Minv = ctx.logdet_inv(Mg)
+    A, Ai = Mg.reshape(n, n).T, Minv.reshape(n, n).T
+    sg_np, ld_np = np.linalg.slogdet(A)
+    say("log det", ld, ld_np, "residual", np.abs(A @ Ai - np.eye(n)).max())
+    assert sg == sg_np and abs(ld - ld_np) <= 1e-10 * max(1.0, abs(ld_np))
+    assert np.abs(A @ Ai - np.eye(n)).max() < 1e-10
+    Fo = om.force(x, Minv, ld)
+    for prec, k, tol in ((0, 1, 1e-10), (0, 4, 1e-10), (1, 1, 1e-5)):
+        c = ctx if (prec, k) == (0, 1) else gpu_context(om, precision=prec, checkpoint=k)
+        Fg = c.force(x, Minv, ld)
+        say("force precision %d checkpoint %d rel err %.2e" % (prec, k, rel(Fg, Fo)))
+        assert rel(Fg, Fo) < tol
+        if c is not ctx:
+            c.close()
+    ctx.close()
+    with pytest.raises(ephmc().EphmcError, match="error -3"):
+        gpu_context(OracleModel(65, 32, 8, beta=8 * 40 / 32))
 
 
 def test_force_mixed():
