@@ -98,6 +98,29 @@ def test_logdet_inverse_random(n):
     assert (ctx.gj_fallbacks() > 0) == needs
 
 
+@pytest.mark.parametrize("n,r,j", [(256, 100, 97), (256, 200, 97), (1024, 610, 608), (1024, 900, 608)])
+def test_gj_single_exchange_in_a_middle_block(n, r, j):
+    """A diagonally dominant matrix with one entry |A[r][j]| > |A[j][j]| below the diagonal in the
+    pivot column j of a block that is neither the first nor the last: LU needs one row exchange
+    there (its multiplier is about 1.5), inside the diagonal block (r in the block of j) or below
+    it.  The check must raise the flag, the pivoted fallback must run, and log det, sign and
+    inverse equal numpy's."""
+    say("gj exchange in a middle block", n, r, j)
+    om = OracleModel(n, 1, 1)
+    ctx = gpu_context(om)
+    A = np.random.default_rng(n + r).standard_normal((n, n)) * 0.01 / np.sqrt(n) + np.eye(n) * 2
+    A[r, j] = 3.0
+    ld, sg, inv = ctx.logdet_inv(A.T.copy().ravel())
+    s0, l0 = np.linalg.slogdet(A)
+    assert ctx.gj_fallbacks() == 1
+    assert ld == pytest.approx(l0, rel=1e-12) and sg == int(s0)
+    assert rel(inv.reshape(n, n).T, np.linalg.inv(A)) < 1e-12
+    A[r, j] = 1.5                              # multiplier 0.75: no exchange, fast path
+    ld, sg, inv = ctx.logdet_inv(A.T.copy().ravel())
+    assert ctx.gj_fallbacks() == 1
+    assert ld == pytest.approx(np.linalg.slogdet(A)[1], rel=1e-12)
+
+
 def test_logdet_inverse_diagonally_dominant_no_fallback():
     """A matrix whose multipliers are all below 1 (no exchanges) stays on the fast path."""
     n = 300
