@@ -226,9 +226,9 @@ int create(ephmc_ctx *const *ctxs, unsigned n, Batch *b)
 	b->ch.assign(ctxs, ctxs + n);
 	const char *eager = getenv("EPHMC_BATCH_EAGER");
 	b->eager = eager && atoi(eager) == 1;
-	b->device = ctxs[0]->device;
 	for(unsigned i = 0; i < n; i++){
 		if(!ctxs[i]){ set_error("batch: chain %u is null", i); return EPHMC_ERR_ARG; }
+		if(i == 0) b->device = ctxs[0]->device;
 		if(ctxs[i]->device != b->device){ set_error("batch: chains on two devices"); return EPHMC_ERR_ARG; }
 		for(unsigned j = 0; j < i; j++)
 			if(ctxs[j] == ctxs[i]){ set_error("batch: chain %u appears twice", i); return EPHMC_ERR_ARG; }
@@ -324,6 +324,27 @@ int create(ephmc_ctx *const *ctxs, unsigned n, Batch *b)
 	return EPHMC_OK;
 }
 
+// Everything a batch trajectory enqueues after its inputs are staged in h (slot s).
+int submit(Batch &b, const double *h, int s)
+{
+	const unsigned n = (unsigned)b.ch.size();
+	EPHMC_HIP(hipMemcpyAsync(b.d_in, h, b.in_len * sizeof(double), hipMemcpyHostToDevice, b.lead));
+	EPHMC_HIP(hipEventRecord(b.in_ev[s], b.lead));
+	EPHMC_HIP(hipEventRecord(b.up, b.lead));
+	for(auto &g : b.lanes){
+		EPHMC_HIP(hipStreamWaitEvent(g.stream, b.up, 0));
+		if(g.exec) EPHMC_HIP(hipGraphLaunch(g.exec, g.stream));
+		else EPHMC_TRY(launch_seq(b, g));
+		EPHMC_HIP(hipEventRecord(g.done, g.stream));
+		EPHMC_HIP(hipStreamWaitEvent(b.lead, g.done, 0));
+	}
+	EPHMC_HIP(hipMemcpyAsync(b.h_rec, b.d_rec, (size_t)n * SCAL_COUNT * sizeof(double), hipMemcpyDeviceToHost, b.lead));
+	EPHMC_HIP(hipEventRecord(b.fin, b.lead));
+	// each context's later work runs after this trajectory
+	for(unsigned i = 0; i < n; i++) EPHMC_HIP(hipStreamWaitEvent(b.ch[i]->stream, b.fin, 0));
+	return EPHMC_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -363,20 +384,14 @@ int ephmc_batch_trajectory(ephmc_batch *b, const double *const *normals, const d
 		EPHMC_HIP(hipEventRecord(b->ch_ev[i], b->ch[i]->stream));
 		EPHMC_HIP(hipStreamWaitEvent(b->lead, b->ch_ev[i], 0));
 	}
-	EPHMC_HIP(hipMemcpyAsync(b->d_in, h, b->in_len * sizeof(double), hipMemcpyHostToDevice, b->lead));
-	EPHMC_HIP(hipEventRecord(b->in_ev[s], b->lead));
-	EPHMC_HIP(hipEventRecord(b->up, b->lead));
-	for(auto &g : b->lanes){
-		EPHMC_HIP(hipStreamWaitEvent(g.stream, b->up, 0));
-		if(g.exec) EPHMC_HIP(hipGraphLaunch(g.exec, g.stream));
-		else EPHMC_TRY(launch_seq(*b, g));
-		EPHMC_HIP(hipEventRecord(g.done, g.stream));
-		EPHMC_HIP(hipStreamWaitEvent(b->lead, g.done, 0));
+	const int rc = submit(*b, h, s);
+	if(rc != EPHMC_OK){
+		// an enqueue failed part way: wait for whatever did start, so that no batch work is left
+		// running unordered with the contexts' own streams
+		(void)hipStreamSynchronize(b->lead);
+		for(auto &g : b->lanes) (void)hipStreamSynchronize(g.stream);
+		return rc;
 	}
-	EPHMC_HIP(hipMemcpyAsync(b->h_rec, b->d_rec, (size_t)n * SCAL_COUNT * sizeof(double), hipMemcpyDeviceToHost, b->lead));
-	EPHMC_HIP(hipEventRecord(b->fin, b->lead));
-	// each context's later work runs after this trajectory
-	for(unsigned i = 0; i < n; i++) EPHMC_HIP(hipStreamWaitEvent(b->ch[i]->stream, b->fin, 0));
 	b->pending = true;
 	return EPHMC_OK;
 }

@@ -107,6 +107,9 @@ def test_batch_refuses_unsupported():
         E.Batch([tiny])
     with pytest.raises(E.EphmcError, match="error -1"):
         E.Batch([ready, ready])                 # one context twice
+    for arr in ((C.c_void_p * 2)(None, ready.h.value), (C.c_void_p * 2)(ready.h.value, None)):
+        out = C.c_void_p()
+        assert E.LIB.ephmc_batch_create(arr, 2, C.byref(out)) == -1 and not out.value   # a null chain
     # a context held by a batch can be neither tuned nor destroyed
     other = gpu_context(om, steps=4)
     other.chain_init(draws_for(10, om.dim, 0)[0][0])
