@@ -89,10 +89,11 @@ def test_tl_many_batches_vs_oracle(monkeypatch):
     assert rel(g, go) < 1e-9
 
 
-@pytest.mark.parametrize("l1,l2,nsamples", [(16, 16, 3), (32, 32, 2), (32, 40, 1)])
+@pytest.mark.parametrize("l1,l2,nsamples", [(16, 16, 3), (19, 27, 2), (23, 31, 2), (32, 32, 2), (32, 40, 1)])
 def test_tl_large_vs_numpy(l1, l2, nsamples):
-    """N = 256, 1024 (the 32x32 size; both with the compact-WY back-transformation) and 1280 (two
-    rows per thread, reflectors one at a time) against LAPACK eigh."""
+    """N = 256, 513 and 713 (ragged: 9 and 12 row blocks of the lower-triangle panels, the last
+    one of 1 and 9 rows), 1024 (the 32x32 size; all with the compact-WY back-transformation) and
+    1280 (two rows per thread, reflectors one at a time) against LAPACK eigh."""
     nbins = 512
     om, width = tl_model(l1, l2, nbins)
     xs = draws(om, nsamples, 7)
