@@ -5,7 +5,8 @@ checkpoints, some autotuned and one with fewer MD steps, run the same random inp
 (ephmc_trajectory, with the force width the batch set) and as one batch: every chain's records, final
 field and M_hat^{-1} are bit-identical over several trajectories (the batch runs the single-chain
 kernels' bodies with each chain's own grid for every sum).  The solo path itself is pinned against the oracle by the
-configuration tests.  The same holds at the 64-chain workload's shape (64 chains of L 8..24, Nt 32..128).
+configuration tests.  The same holds at the 64-chain workload's shape (64 chains of L 8..24, Nt 32..128) and for
+non-square lattices across the batch's range (N = 33 .. 1024).
 Modes the batch does not run are refused with EPHMC_ERR_UNSUPPORTED, and a context held by a batch can
 be neither tuned nor destroyed until the batch is."""
 import ctypes as C
@@ -155,5 +156,41 @@ def test_batch_64_uneven_chains_equal_solo():
         assert got[j] == solo[j][0], j
         np.testing.assert_array_equal(ctx.chain_x(), solo[j][1])
     say("batch of 64: records and fields equal the solo runs")
+    for ctx in ctxs:
+        ctx.close()
+
+
+def test_batch_ragged_sizes_equal_solo():
+    """Non-square lattices across the batch's range, 33 <= N <= 1024 (N = 33, 63, 221, 675 and 1024:
+    partial waves, partial Gauss-Jordan blocks, every workgroup class), FP64 and mixed: two
+    trajectories as one batch equal the chains run alone, bit for bit."""
+    E = ephmc()
+    specs = [(3, 11, 32, 1), (7, 9, 64, 0), (13, 17, 32, 1), (25, 27, 32, 1), (32, 32, 32, 0), (3, 11, 64, 0)]
+    ctxs, inputs = [], []
+    for i, (l1, l2, nt, prec) in enumerate(specs):
+        om = OracleModel(l1, l2, nt)
+        ctxs.append(gpu_context(om, steps=6, checkpoint=0, precision=prec, graph=1))
+        inputs.append(draws_for(4000 + i, om.dim, 2))
+    for ctx, d in zip(ctxs, inputs):
+        ctx.chain_init(d[0][0])
+    b = E.Batch(ctxs)
+    say("batch ragged:", b.info())
+    solo = []
+    for ctx, d in zip(ctxs, inputs):
+        ctx.chain_init(d[0][0])
+        recs = [rec_tuple(ctx.trajectory(nrm, u)) for nrm, u in d[1:]]
+        solo.append((recs, ctx.chain_x(), ctx.chain_minv()))
+    for ctx, d in zip(ctxs, inputs):
+        ctx.chain_init(d[0][0])
+    got = [[] for _ in ctxs]
+    for t in (1, 2):
+        for j, r in enumerate(b.trajectory([d[t][0] for d in inputs], [d[t][1] for d in inputs])):
+            got[j].append(rec_tuple(r))
+    b.close()
+    for j, ctx in enumerate(ctxs):
+        assert got[j] == solo[j][0], specs[j]
+        np.testing.assert_array_equal(ctx.chain_x(), solo[j][1])
+        np.testing.assert_array_equal(ctx.chain_minv(), solo[j][2])
+    say("batch ragged: records, fields and inverses equal the solo runs")
     for ctx in ctxs:
         ctx.close()
