@@ -334,6 +334,9 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 	if(!cfg || !out){ set_error("null argument"); return EPHMC_ERR_ARG; }
 	*out = nullptr;
 	if(cfg->l1 == 0 || cfg->l2 == 0 || cfg->nt == 0){ set_error("empty lattice"); return EPHMC_ERR_ARG; }
+	// f_n = omega + (2 sin(pi n / nt))^2 / omega (organic_bosons.c:137) divides by omega (negative
+	// under static disorder, where the units scale by -beta: organic_main.c:50-53)
+	if(!(fabs(cfg->omega) > 0)){ set_error("omega must be nonzero, got %g", cfg->omega); return EPHMC_ERR_ARG; }
 	if(cfg->precision != EPHMC_FP64 && cfg->precision != EPHMC_MIXED){ set_error("precision"); return EPHMC_ERR_ARG; }
 	if(cfg->classical_fourier_acc && cfg->cfa_steps == 0){ set_error("CLASSICAL_FOURIER_ACC needs cfa_steps >= 1"); return EPHMC_ERR_ARG; }
 	if(cfg->no_fourier_acc && cfg->classical_fourier_acc){ set_error("NO_FOURIER_ACC and CLASSICAL_FOURIER_ACC exclude each other"); return EPHMC_ERR_ARG; }
@@ -349,8 +352,11 @@ int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out)
 		set_error("stabilize: the stabilised product forms 1 + B..B, not provided for ZERO_FILLING / SINGLE_PARTICLE");
 		return EPHMC_ERR_UNSUPPORTED;
 	}
+	if((unsigned long long)cfg->l1 * cfg->l2 > 2048){
+		set_error("lattice of %llu sites exceeds 2048", (unsigned long long)cfg->l1 * cfg->l2);
+		return EPHMC_ERR_UNSUPPORTED;
+	}
 	const unsigned ns = cfg->l1 * cfg->l2;
-	if(ns > 2048){ set_error("lattice of %u sites exceeds 2048", ns); return EPHMC_ERR_UNSUPPORTED; }
 	ephmc_ctx *c = new ephmc_ctx();
 	c->cfg = *cfg;
 	c->chain_strip_set[0] = c->chain_strip_set[1] = -1;

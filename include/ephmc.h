@@ -78,6 +78,8 @@ typedef struct ephmc_traj_record {
 } ephmc_traj_record;
 
 /* ---- lifecycle ---------------------------------------------------------------------- */
+/* EPHMC_ERR_ARG for an empty lattice, omega zero or NaN, or an unknown precision, before any device
+ * call; EPHMC_ERR_UNSUPPORTED beyond 2048 sites or for excluded flag combinations. */
 int ephmc_create(const ephmc_config *cfg, ephmc_ctx **out);
 int ephmc_destroy(ephmc_ctx *ctx);
 int ephmc_synchronize(ephmc_ctx *ctx);

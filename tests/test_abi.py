@@ -61,3 +61,24 @@ def test_host_library_exports_the_reference_interface():
     missing = [n for n in declared_host if not hasattr(lib, n)]
     assert not missing, missing
     assert set(REFERENCE_HOST_API) <= set(declared_host)
+
+
+def test_create_refuses_bad_configurations_before_any_device_call():
+    """ephmc_create validates its configuration before touching the device (so this runs without
+    a GPU): an empty lattice, omega zero or NaN (the Fourier masses divide by it; it is negative
+    under static disorder), a lattice beyond 2048 sites, also where l1 * l2 wraps around 32 bits,
+    are refused and no context is returned."""
+    import sys
+    sys.path.insert(0, PKG)
+    import ephmc as E
+    base = dict(l1=4, l2=4, nt=8, omega=0.01, precision=E.FP64)
+    cases = [(dict(l1=0), -1), (dict(nt=0), -1), (dict(omega=0.0), -1), (dict(omega=float("nan")), -1), (dict(l1=65, l2=32), -3), (dict(l1=65536, l2=65536), -3),
+             (dict(l1=1 << 31, l2=2), -3), (dict(precision=7), -1)]
+    for change, code in cases:
+        cfg = E.Config()
+        for k, v in {**base, **change}.items():
+            setattr(cfg, k, v)
+        h = C.c_void_p()
+        assert E.LIB.ephmc_create(C.byref(cfg), C.byref(h)) == code, change
+        assert not h.value, change
+
