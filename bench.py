@@ -183,6 +183,19 @@ def force_only(n):
     ctx.close()
 
 
+def graph_only(n):
+    """n mixed-precision trajectories at the bench size with the trajectory body replayed as the
+    captured hipGraph (the headline path; no torch): the workload of the counter passes on it."""
+    import ephmc as E
+    rng = np.random.default_rng(SEED)
+    dim = NT * 3 * L * L
+    normals_list = [rng.standard_normal(dim) for _ in range(min(n + 1, 8))]
+    uniforms = rng.random(len(normals_list))
+    ctx, el, _, _ = run_chain(E, E.MIXED, normals_list, uniforms, rng.standard_normal(dim), 1, n, graph=True)
+    log("graph-only: %d trajectories, %.2f ms each" % (n, 1e3 * el / max(n, 1)))
+    ctx.close()
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -480,6 +493,9 @@ def main():
     ap.add_argument("--force-only", type=int, default=0, metavar="N",
                     help="run N mixed-precision forces at 32x32, Nt=128 (eager, no graphs) and exit: the "
                          "workload of the hardware-counter passes")
+    ap.add_argument("--graph-only", type=int, default=0, metavar="N",
+                    help="run N mixed-precision trajectories at 32x32, Nt=128 replayed as the captured hipGraph "
+                         "(the headline path) and exit: the workload of the counter passes on it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy (known bytes) for HBM counter calibration")
     ap.add_argument("--copy-only", action="store_true", help="run only the calibration copy and exit")
@@ -493,6 +509,9 @@ def main():
         return
     if args.force_only:
         force_only(args.force_only)
+        return
+    if args.graph_only:
+        graph_only(args.graph_only)
         return
     if args.c4_batch_only:
         import ephmc as E
