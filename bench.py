@@ -220,11 +220,11 @@ def cpu_baseline(threads, x_state, gpu):
     """The oracle (the C/OpenMP restatement of the reference algorithm) timed on this host's cores,
     and used as the checker of the GPU at the full size.
 
-    Timed sample (32x32, Nt=128, `threads` OpenMP threads): (i) one complete trajectory with Nmd=1
-    through the oracle's trajectory driver (momentum draw and colouring, leapfrog with one force, two
-    M_hat + LU, Metropolis), and (ii) one M_hat + LU and one force at the chain's field x; the time of
-    one trajectory at Nmd=10 is t(i) + 9 (t_M+LU + t_force).  The oracle's own LU and FFT run (no
-    LAPACK or FFTW).  Also at 1 thread: the 6x6, Nt=32 chain (Nmd=10), 100 trajectories in full.
+    Timed sample (32x32, Nt=128, `threads` OpenMP threads): one complete trajectory with Nmd=10
+    through the oracle's trajectory driver (momentum draw and colouring, leapfrog with ten forces,
+    eleven M_hat + LU, Metropolis), about 23 s on 16 threads; one M_hat + LU and one force at the
+    chain's field x are timed as well (they are the checks below).  The oracle's own LU and FFT run
+    (no LAPACK or FFTW).  Also at 1 thread: the 6x6, Nt=32 chain (Nmd=10), 100 trajectories in full.
 
     Checks at x (the GPU values computed in the accuracy leg): the GPU's FP64 log det and its FP64
     and mixed molecular-dynamics forces against the oracle's."""
@@ -235,11 +235,11 @@ def cpu_baseline(threads, x_state, gpu):
     _omp_threads(threads)
     om = OracleModel(L, L, NT)
     h = HMC()
-    assert lib.orc_hmc_init(C.byref(h), C.byref(om.m), SEED, 1, 0.0, None) == 0
+    assert lib.orc_hmc_init(C.byref(h), C.byref(om.m), SEED, NMD, 0.0, None) == 0
     dH = C.c_double()
     t0 = time.perf_counter()
     lib.orc_trajectory(C.byref(h), None, None, C.byref(dH), None, None)
-    t_traj1 = time.perf_counter() - t0
+    t_traj = time.perf_counter() - t0
     lib.orc_hmc_free(C.byref(h))
     t0 = time.perf_counter()
     ld, _, minv = om.logdet_inv(om.M_hat(x_state))
@@ -247,7 +247,6 @@ def cpu_baseline(threads, x_state, gpu):
     t0 = time.perf_counter()
     F = om.force(x_state, minv, ld)
     t_force = time.perf_counter() - t0
-    t_traj = t_traj1 + (NMD - 1) * (t_mlu + t_force)
     checks = {"logdet_rel_err_fp64_vs_oracle": abs(gpu["logdet"] - ld) / abs(ld),
               "force_rel_err_fp64_vs_oracle": float(np.linalg.norm(gpu["F64"] - F) / np.linalg.norm(F)),
               "force_rel_err_mixed_vs_oracle": float(np.linalg.norm(gpu["F32"] - F) / np.linalg.norm(F))}
@@ -264,10 +263,9 @@ def cpu_baseline(threads, x_state, gpu):
     lib.orc_hmc_free(C.byref(h))
     _omp_threads(threads)
     base = {"value": 1.0 / t_traj, "unit": "traj/s", "cores": threads, "kind": "port",
-            "sample": "32x32, Nt=128 on %d OpenMP threads: one complete Nmd=1 trajectory (%.2f s) plus one "
-                      "M_hat+LU (%.2f s) and one force (%.2f s); Nmd=10 trajectory = %.1f s.  Oracle's own LU "
-                      "and FFT.  CPU %s (%s logical CPUs visible)"
-                      % (threads, t_traj1, t_mlu, t_force, t_traj, cpu_model(), os.cpu_count())}
+            "sample": "32x32, Nt=128 on %d OpenMP threads: one complete Nmd=10 trajectory (%.1f s; one M_hat+LU "
+                      "%.2f s, one force %.2f s).  Oracle's own LU and FFT.  CPU %s (%s logical CPUs visible)"
+                      % (threads, t_traj, t_mlu, t_force, cpu_model(), os.cpu_count())}
     c1 = {"traj_per_s": 1.0 / t_c1, "threads": 1, "trajectories": n1,
           "config": "6x6, Nt=32, Nmd=10, seed 1, FP64 (oracle, complete trajectories)"}
     return base, c1, checks
