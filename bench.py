@@ -319,6 +319,48 @@ def c2_stab_leg(E, ndraw=10, ntherm=5):
     return out
 
 
+def c2_chains(E, ntraj=500):
+    """16x16, Nt=64, Nmd=10, seed 2: the FP64 chain (the drop-in) and the mixed-precision chain run
+    over the same ntraj trajectories' random inputs (numpy generator seeded 2), trajectory body
+    replayed as a hipGraph, each trajectory's record read back before the next: rates, acceptance,
+    the fraction of identical accept/reject decisions,
+    the largest log det difference, <exp(-dH)> and the mean log det per site of each chain (binned
+    errors, bins of 25) with the z-score of the difference."""
+    l, nt = 16, 64
+    t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, nt)
+    dim = nt * 3 * l * l
+    out = {"config": "16x16, Nt=64, Nmd=10, seed 2, %d trajectories per chain, same random inputs" % ntraj}
+    recs = {}
+    for name, prec in (("fp64", E.FP64), ("mixed", E.MIXED)):
+        ctx = E.Context(l, l, nt, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=prec, checkpoint=K_CKPT,
+                        graph=1)
+        rng = np.random.default_rng(2)
+        ctx.chain_init(rng.standard_normal(dim))
+        r = []
+        t0 = time.perf_counter()
+        for _ in range(ntraj):
+            rec = ctx.trajectory(rng.standard_normal(dim), float(rng.random()))
+            r.append((rec.accepted, rec.dH, rec.logdet))
+        out[name + "_traj_per_s"] = ntraj / (time.perf_counter() - t0)
+        recs[name] = np.array(r, dtype=float)
+        ctx.close()
+
+    def binned(v, b=25):
+        m = len(v) // b
+        bm = v[:m * b].reshape(m, b).mean(axis=1)
+        return float(v.mean()), float(bm.std(ddof=1) / np.sqrt(m))
+
+    for name, a in recs.items():
+        out[name] = {"acceptance": float(a[:, 0].mean()), "exp_minus_dH": binned(np.exp(-a[:, 1])),
+                     "logdet_per_site": binned(a[:, 2] / (l * l))}
+    a, b = recs["fp64"], recs["mixed"]
+    out["decisions_equal"] = float((a[:, 0] == b[:, 0]).mean())
+    out["max_logdet_rel_diff"] = float(np.max(np.abs(a[:, 2] - b[:, 2]) / np.abs(a[:, 2])))
+    (m0, e0), (m1, e1) = out["fp64"]["logdet_per_site"], out["mixed"]["logdet_per_site"]
+    out["logdet_per_site_z"] = float(abs(m0 - m1) / np.hypot(e0, e1)) if e0 > 0 or e1 > 0 else 0.0
+    return out
+
+
 def c1_gpu(E, ntraj=100):
     """The 6x6, Nt=32 FP64 chain (Nmd=10) on the GPU, seed-1 normals from numpy, graph replay."""
     t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, 32)
@@ -482,6 +524,8 @@ def main():
     ap.add_argument("--c4-batch-only", action="store_true",
                     help="run only the 64-chain batch leg and print its JSON (the workload of its kernel trace, "
                          "with EPHMC_BATCH_EAGER=1 for plain launches)")
+    ap.add_argument("--c2-only", action="store_true",
+                    help="run only the 16x16, Nt=64 legs (stabilised product, 500-trajectory chains) and print their JSON")
     ap.add_argument("--tl-samples", type=int, default=4096)
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the chains with the trajectory body replayed as a hipGraph (the headline value "
@@ -519,6 +563,10 @@ def main():
         import ephmc as E
         print(json.dumps({"chains64": c4_leg(E, args.batch_traj),
                           "chains64_batch": c4_batch_leg(E, args.batch_traj)}), flush=True)
+        return
+    if args.c2_only:
+        import ephmc as E
+        print(json.dumps({"c2_stabilised": c2_stab_leg(E), "c2_chains": c2_chains(E)}), flush=True)
         return
     if args.tl_only:
         import ephmc as E
@@ -631,6 +679,7 @@ def main():
         log("chains64_batch", out["chains64_batch"])
     if not args.no_c2:
         out["c2_stabilised"] = c2_stab_leg(E)
+        out["c2_chains"] = c2_chains(E)
         log("c2 stabilised", out["c2_stabilised"])
     if not args.no_tl:
         out["tl"] = tl_leg(E, args.tl_samples)
