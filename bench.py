@@ -224,7 +224,7 @@ def cpu_baseline(threads, x_state, gpu):
     through the oracle's trajectory driver (momentum draw and colouring, leapfrog with ten forces,
     eleven M_hat + LU, Metropolis), about 23 s on 16 threads; one M_hat + LU and one force at the
     chain's field x are timed as well (they are the checks below).  The oracle's own LU and FFT run
-    (no LAPACK or FFTW).  Also at 1 thread: the 6x6, Nt=32 chain (Nmd=10), 100 trajectories in full.
+    (no LAPACK or FFTW).  Also at 1 thread: the 6x6, Nt=32 chain (Nmd=10), 200 trajectories in full.
 
     Checks at x (the GPU values computed in the accuracy leg): the GPU's FP64 log det and its FP64
     and mixed molecular-dynamics forces against the oracle's."""
@@ -255,7 +255,7 @@ def cpu_baseline(threads, x_state, gpu):
     om1 = OracleModel(6, 6, 32)
     h = HMC()
     assert lib.orc_hmc_init(C.byref(h), C.byref(om1.m), 1, NMD, 0.0, None) == 0
-    n1 = 100
+    n1 = 200
     t0 = time.perf_counter()
     for _ in range(n1):
         lib.orc_trajectory(C.byref(h), None, None, C.byref(dH), None, None)
@@ -361,7 +361,7 @@ def c2_chains(E, ntraj=500):
     return out
 
 
-def c1_gpu(E, ntraj=100):
+def c1_gpu(E, ntraj=200):
     """The 6x6, Nt=32 FP64 chain (Nmd=10) on the GPU, seed-1 normals from numpy, graph replay."""
     t, kappa, mu, omega = E.scale_params(BETA, J, LAM, MU0, OMEGA0, 32)
     ctx = E.Context(6, 6, 32, t, kappa, mu, omega, flavors=FLAVORS, steps=NMD, precision=E.FP64, graph=1)
