@@ -30,8 +30,7 @@ __global__ __launch_bounds__(QR_BT) void qr_prepivot_kernel(const R *__restrict_
 {
 	extern __shared__ double qsh[];
 	double *key = qsh;                              // n: log of the scaled column norms
-	__shared__ double red[QR_BT / 64];
-	__shared__ double bcast[2];
+	__shared__ double nsig[2], nalpha[2];           // sigma and alpha of the column of the next step
 	const unsigned tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = QR_BT / 64;
 	// column keys: log(d_c) + log|Y_c| (logs: the scales may span far more than an FP64 square)
 	for(unsigned c = tid; c < n; c += QR_BT){
@@ -59,51 +58,93 @@ __global__ __launch_bounds__(QR_BT) void qr_prepivot_kernel(const R *__restrict_
 		zr[e] = d[src] * (double)Y[(size_t)src * n + r];
 	}
 	__syncthreads();
-	int refl = 0;
-	for(unsigned j = 0; j < n; j++){
-		double *zj = zr + (size_t)j * n;
-		// sigma = sum_{r > j} z_rj^2
+	// Each step j needs column j below the diagonal (staged in LDS), its squared norm sigma and its
+	// diagonal entry alpha; the wave that updates column j + 1 in step j stages them for step j + 1, so
+	// a step is one pass over the trailing columns and one barrier.  Column 0 first:
+	double *const vb0 = qsh + n;                    // two n-double buffers at qsh + n + b n
+	if(wave == 0){
 		double s = 0;
-		for(unsigned r = j + 1 + tid; r < n; r += QR_BT) s = fma(zj[r], zj[r], s);
+		for(unsigned r = 1 + lane; r < n; r += 64){
+			const double v = zr[r];
+			vb0[r] = v;
+			s = fma(v, v, s);
+		}
 #pragma unroll
 		for(int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-		if(lane == 0) red[wave] = s;
-		__syncthreads();
+		if(lane == 0){
+			nsig[0] = s;
+			nalpha[0] = zr[0];
+		}
+	}
+	__syncthreads();
+	int refl = 0;
+	for(unsigned j = 0; j < n; j++){
+		const int b = j & 1;
+		const double *vc = vb0 + (size_t)b * n;
+		double *vn = vb0 + (size_t)(b ^ 1) * n;
+		double *zj = zr + (size_t)j * n;
+		// H_j = 1 - t v v^T with v_j = 1, v_r = z_rj / (alpha - beta) (the same values in every thread)
+		const double sig = nsig[b], alpha = nalpha[b];
+		double t = 0, scale = 0, beta = alpha;
+		if(sig > 0){
+			const double nrm = sqrt(fma(alpha, alpha, sig));
+			beta = alpha >= 0 ? -nrm : nrm;
+			t = (beta - alpha) / beta;
+			scale = 1.0 / (alpha - beta);
+		}
 		if(tid == 0){
-			double sig = 0;
-			for(unsigned w = 0; w < nw; w++) sig += red[w];
-			const double alpha = zj[j];
-			double t = 0, scale = 0, beta = alpha;
-			if(sig > 0){
-				const double nrm = sqrt(fma(alpha, alpha, sig));
-				beta = alpha >= 0 ? -nrm : nrm;
-				t = (beta - alpha) / beta;
-				scale = 1.0 / (alpha - beta);
-			}
 			tau[j] = t;
 			zj[j] = beta;
-			bcast[0] = t;
-			bcast[1] = scale;
 		}
-		__syncthreads();
-		const double t = bcast[0], scale = bcast[1];
 		if(t != 0){
 			refl++;
-			for(unsigned r = j + 1 + tid; r < n; r += QR_BT) zj[r] *= scale;     // v below the diagonal
-		}
-		__syncthreads();
-		if(t != 0){
-			// columns c > j: w = z_jc + sum_{r > j} v_r z_rc;  z_.c -= t w v   (one wave per column)
+			for(unsigned r = j + 1 + tid; r < n; r += QR_BT) zj[r] = vc[r] * scale;     // the stored v
+			// columns c > j: w = z_jc + sum_{r > j} v_r z_rc;  z_.c -= t w v   (one wave per column;
+			// wave 0 takes column j + 1 first and stages it for step j + 1)
 			for(unsigned c = j + 1 + wave; c < n; c += nw){
 				double *zc = zr + (size_t)c * n;
 				double w = 0;
-				for(unsigned r = j + 1 + lane; r < n; r += 64) w = fma(zj[r], zc[r], w);
+				for(unsigned r = j + 1 + lane; r < n; r += 64) w = fma(vc[r] * scale, zc[r], w);
 #pragma unroll
 				for(int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
 				w += zc[j];
 				const double tw = t * w;
 				if(lane == 0) zc[j] -= tw;
-				for(unsigned r = j + 1 + lane; r < n; r += 64) zc[r] = fma(-tw, zj[r], zc[r]);
+				if(c == j + 1){
+					double s = 0, diag = 0;
+					for(unsigned r = j + 1 + lane; r < n; r += 64){
+						const double nv = fma(-tw, vc[r] * scale, zc[r]);
+						zc[r] = nv;
+						if(r == j + 1) diag = nv;
+						else{
+							vn[r] = nv;
+							s = fma(nv, nv, s);
+						}
+					}
+#pragma unroll
+					for(int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+					if(lane == 0){
+						nsig[b ^ 1] = s;
+						nalpha[b ^ 1] = diag;
+					}
+				}else{
+					for(unsigned r = j + 1 + lane; r < n; r += 64) zc[r] = fma(-tw, vc[r] * scale, zc[r]);
+				}
+			}
+		}else if(wave == 0 && j + 1 < n){
+			// no reflector: column j + 1 is staged as it stands
+			const double *zc = zr + (size_t)(j + 1) * n;
+			double s = 0;
+			for(unsigned r = j + 2 + lane; r < n; r += 64){
+				const double v = zc[r];
+				vn[r] = v;
+				s = fma(v, v, s);
+			}
+#pragma unroll
+			for(int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+			if(lane == 0){
+				nsig[b ^ 1] = s;
+				nalpha[b ^ 1] = zc[j + 1];
 			}
 		}
 		__syncthreads();
@@ -112,30 +153,49 @@ __global__ __launch_bounds__(QR_BT) void qr_prepivot_kernel(const R *__restrict_
 }
 
 // Q = H_0 H_1 ... H_{n-1} (the reflectors stored in zr, tau) into q, column-major; one workgroup.
+// Reflector j is read from LDS; reflector j - 1 is loaded while j is applied and staged after it
+// (two n-double buffers), so a step is one pass over the columns and one barrier.
 __global__ __launch_bounds__(QR_BT) void qr_form_q_kernel(const double *__restrict__ zr, const double *__restrict__ tau,
 		unsigned n, double *__restrict__ q)
 {
+	extern __shared__ double vsh[];                 // 2 n: v_r (r > j) of reflector j at vsh + (j & 1) n
+	constexpr unsigned PR = 2048 / QR_BT;           // rows of the next reflector per thread (n <= 2048)
 	const unsigned tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = QR_BT / 64;
 	for(size_t e = tid; e < (size_t)n * n; e += QR_BT){
 		const unsigned c = (unsigned)(e / n), r = (unsigned)(e - (size_t)c * n);
 		q[e] = r == c ? 1.0 : 0.0;
 	}
+	// reflector n - 1 has no rows below its diagonal
 	__syncthreads();
 	for(int j = (int)n - 1; j >= 0; j--){
+		const unsigned b = (unsigned)j & 1u;
+		const double *vc = vsh + (size_t)b * n;
+		double *vn = vsh + (size_t)(b ^ 1u) * n;
+		// the next reflector's rows r > j - 1, in flight during this step's columns
+		double pre[PR];
+#pragma unroll
+		for(unsigned k = 0; k < PR; k++){
+			const unsigned r = (unsigned)j + tid + QR_BT * k;
+			pre[k] = (j > 0 && r < n) ? zr[(size_t)(j - 1) * n + r] : 0.0;
+		}
 		const double t = tau[j];
 		if(t != 0){
-			const double *v = zr + (size_t)j * n;      // v_j = 1, v_r (r > j) below the diagonal
 			for(unsigned c = (unsigned)j + wave; c < n; c += nw){
 				double *qc = q + (size_t)c * n;
 				double w = 0;
-				for(unsigned r = j + 1 + lane; r < n; r += 64) w = fma(v[r], qc[r], w);
+				for(unsigned r = j + 1 + lane; r < n; r += 64) w = fma(vc[r], qc[r], w);
 #pragma unroll
 				for(int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
 				w += qc[j];
 				const double tw = t * w;
 				if(lane == 0) qc[j] -= tw;
-				for(unsigned r = j + 1 + lane; r < n; r += 64) qc[r] = fma(-tw, v[r], qc[r]);
+				for(unsigned r = j + 1 + lane; r < n; r += 64) qc[r] = fma(-tw, vc[r], qc[r]);
 			}
+		}
+#pragma unroll
+		for(unsigned k = 0; k < PR; k++){
+			const unsigned r = (unsigned)j + tid + QR_BT * k;
+			if(j > 0 && r < n) vn[r] = pre[k];
 		}
 		__syncthreads();
 	}
@@ -308,11 +368,11 @@ int stable_logdet_inverse(ephmc_ctx *c, int prec, unsigned k, double *minv)
 		// Y = B_{hi-cnt+1} ... B_hi U: slices hi down to hi-cnt+1, each applied from the left
 		EPHMC_TRY(chain_range(c, prec, w.U, Y, hi, -1, cnt));
 		if(prec == EPHMC_FP64)
-			qr_prepivot_kernel<double><<<1, QR_BT, qlds, c->stream>>>((const double *)Y, w.d, n, w.ZR, w.tau, w.perm, w.nref);
+			qr_prepivot_kernel<double><<<1, QR_BT, 3 * qlds, c->stream>>>((const double *)Y, w.d, n, w.ZR, w.tau, w.perm, w.nref);
 		else
-			qr_prepivot_kernel<float><<<1, QR_BT, qlds, c->stream>>>((const float *)Y, w.d, n, w.ZR, w.tau, w.perm, w.nref);
+			qr_prepivot_kernel<float><<<1, QR_BT, 3 * qlds, c->stream>>>((const float *)Y, w.d, n, w.ZR, w.tau, w.perm, w.nref);
 		EPHMC_LAUNCHED();
-		qr_form_q_kernel<<<1, QR_BT, 0, c->stream>>>(w.ZR, w.tau, n, w.U);
+		qr_form_q_kernel<<<1, QR_BT, 2 * qlds, c->stream>>>(w.ZR, w.tau, n, w.U);
 		EPHMC_LAUNCHED();
 		qr_update_v_kernel<<<n, 256, qlds, c->stream>>>(w.ZR, w.perm, n, w.V, w.d);
 		EPHMC_LAUNCHED();
