@@ -587,7 +587,8 @@ __device__ __forceinline__ void gj_fallback_body(double *__restrict__ M, unsigne
 {
 	extern __shared__ double fsh[];
 	double *colg = fsh;                                   // n: multipliers of the step
-	int *piv = reinterpret_cast<int *>(fsh + n);          // n
+	double *rowg = fsh + n;                               // n: the scaled pivot row of the step
+	int *piv = reinterpret_cast<int *>(fsh + 2 * n);      // n
 	__shared__ double rv[16];
 	__shared__ int ri[16];
 	__shared__ double ld_acc;
@@ -640,17 +641,33 @@ __device__ __forceinline__ void gj_fallback_body(double *__restrict__ M, unsigne
 		if(tid == 0){ ld_acc += log(fabs(d)); neg_acc += (d < 0) + (p != g); }
 		__syncthreads();
 		const double inv = 1.0 / d;
-		for(unsigned cc = tid; cc < n; cc += 1024) if(cc != g) M[(size_t)cc * n + g] *= inv;
-		__syncthreads();
-		for(unsigned cc = 0; cc < n; cc++){
-			const double u = cc == g ? 0.0 : M[(size_t)cc * n + g];
-			for(unsigned r = tid; r < n; r += 1024){
-				if(r == g) continue;
-				if(cc == g) M[(size_t)cc * n + r] = -colg[r] * inv;
-				else M[(size_t)cc * n + r] -= colg[r] * u;
+		for(unsigned cc = tid; cc < n; cc += 1024){
+			double u = 0.0;
+			if(cc != g){
+				u = M[(size_t)cc * n + g] * inv;
+				M[(size_t)cc * n + g] = u;
 			}
+			rowg[cc] = u;
 		}
-		if(tid == 0) M[(size_t)g * n + g] = inv;
+		__syncthreads();
+		// rank-1 update over all threads: thread (r, c0) with r = tid % n, c0 = tid / n takes the columns
+		// c0, c0 + cpp, ... (n <= 1024); for larger n every thread takes rows tid, tid + 1024 of all columns
+		{
+			const bool small = n <= 1024;
+			const unsigned cpp = small ? 1024 / n : 1, rstep = small ? n : 1024;
+			const unsigned r0 = small ? tid % n : tid, c0 = small ? tid / n : 0;
+			if(c0 < cpp)
+				for(unsigned cc = c0; cc < n; cc += cpp){
+					double *mc = M + (size_t)cc * n;
+					const double u = rowg[cc];
+					for(unsigned r = r0; r < n; r += rstep){
+						if(r == g) continue;
+						if(cc == g) mc[r] = -colg[r] * inv;
+						else mc[r] -= colg[r] * u;
+					}
+				}
+		}
+		if(tid == 0) M[(size_t)g * n + g] = inv;         // (no thread writes row g in the update)
 		__syncthreads();
 	}
 	// undo the row exchanges on the columns, in reverse order
@@ -711,7 +728,7 @@ int gj_logdet_inverse(ephmc_ctx *c, double *M)
 		gj_copy_kernel<<<1024, 256, 0, c->stream>>>(cur, M, c->mat, c->d_scal);
 		EPHMC_LAUNCHED();
 	}
-	const size_t lds = (size_t)n * (sizeof(double) + sizeof(int));
+	const size_t lds = (size_t)n * (2 * sizeof(double) + sizeof(int));
 	(void)hipFuncSetAttribute((const void *)gj_fallback_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 	gj_fallback_kernel<<<1, 1024, lds, c->stream>>>(M, n, c->d_scal, c->d_piv);
 	EPHMC_LAUNCHED();
@@ -790,7 +807,7 @@ int batch_gj(BatchOp &first, std::vector<BatchOp> &steps, BatchOp &fallback, eph
 		which ^= 1;
 		par ^= 1;
 	}
-	const size_t lds = (size_t)n * (sizeof(double) + sizeof(int));
+	const size_t lds = (size_t)n * (2 * sizeof(double) + sizeof(int));
 	(void)hipFuncSetAttribute((const void *)gj_fallback_multi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
 		(int)std::max(lds, fallback.lds));
 	const GjFallbackArgs fb{M, n, c->d_scal, c->d_piv};
